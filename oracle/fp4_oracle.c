@@ -1,0 +1,641 @@
+/*
+ * CPU restatement of FourPhonon's four-phonon hot path -- TEST INFRASTRUCTURE.
+ *
+ * Only the tests, the smoke check and the benchmark's CPU-baseline leg use
+ * this file; it is the checker, never the thing measured on the GPU or
+ * shipped.  Build with -O2 -ffp-contract=off -fopenmp (oracle/Makefile).
+ *
+ * Reference lines followed (FourPhonon v1.3, Src/):
+ *   constants            data.f90:5-8, processes.f90:20 (hbarp = hbar*1e22,
+ *                        the literal 1e22 is a default-real (float32) value)
+ *   phexp                misc.f90:70-77
+ *   base_sigma           config.f90:481-499
+ *   Index_N / IJK        processes.f90:2277-2283, config.f90:458-478
+ *   Vp_pp/pm/mm          processes.f90:142-330
+ *   NP_plusplus/..       processes.f90:2263-2476, NP_driver_4ph 2749-2847
+ *   Ind_only_*           processes.f90:2854-3074, Ind_only_driver_4ph 3080-3246
+ *   Ind_* (iterative)    processes.f90:3254-3545, Ind_driver_4ph 3551-3822
+ *   RTA_*                processes.f90:3828-4129, RTA_driver_4ph 5174-5311
+ *   iteration0           iterations.f90:13-30
+ *   iteration_4ph        iterations.f90:95-215 (4ph terms; 3ph lists empty)
+ *   TConduct             conductivity.f90:11-37
+ *
+ * Float32-promoted literals of the reference (unsuffixed reals are default
+ * REAL): 0.001 (sigma clamp), 3.37617087 (4ph rate units), 1e22 (hbarp),
+ * 1e21 (kappa prefactor).  1.25, 0.5, 2. and 8.d0 are exact.
+ */
+#include "fp4_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI_D 3.141592653589793238
+#define KB_D 1.380648813e-23
+#define HBAR_D 1.05457172647e-22
+
+static const float F_CLAMP = 0.001f;
+static const float F_GAMMA_UNITS = 3.37617087f;
+static const float F_1E22 = 1e22f;
+static const float F_1E21 = 1e21f;
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx cmul(cplx a, cplx b) {
+  cplx r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.re * b.im + a.im * b.re;
+  return r;
+}
+static inline cplx rmul(double a, cplx b) {
+  cplx r;
+  r.re = a * b.re;
+  r.im = a * b.im;
+  return r;
+}
+static inline cplx phexp(double x) {
+  cplx r;
+  r.re = cos(x);
+  r.im = sin(x);
+  return r;
+}
+static inline cplx conj_(cplx a) {
+  a.im = -a.im;
+  return a;
+}
+
+#define E(c, q, b) ((c)->energy[(size_t)((q)-1) + (size_t)(c)->nptk * ((b)-1)])
+#define VEL(c, q, b, a) \
+  ((c)->velocity[(size_t)((q)-1) + (size_t)(c)->nptk * ((size_t)((b)-1) + (size_t)(c)->nbands * ((a)-1))])
+
+static inline cplx EV(const orc_case *c, int q, int b, int comp) {
+  size_t idx = (size_t)(q - 1) + (size_t)c->nptk * ((size_t)(b - 1) + (size_t)c->nbands * (size_t)(comp - 1));
+  cplx r;
+  r.re = c->eigenvect[2 * idx];
+  r.im = c->eigenvect[2 * idx + 1];
+  return r;
+}
+
+static inline int fmodulo(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+/* config.f90:481-499 */
+double orc_base_sigma(const orc_case *c, const double v[3]) {
+  double s[3];
+  for (int nu = 0; nu < 3; nu++) {
+    const double *b = c->rlattvec + 3 * nu;
+    double d = 0.0;
+    d = d + b[0] * v[0];
+    d = d + b[1] * v[1];
+    d = d + b[2] * v[2];
+    double t = d / (double)c->ngrid[nu];
+    s[nu] = t * t;
+  }
+  double m = s[0];
+  if (s[1] > m) m = s[1];
+  if (s[2] > m) m = s[2];
+  return sqrt(m / 2.0);
+}
+
+/* matmul(rlattvec, q/dble(ngrid)) */
+static void realq_of(const orc_case *c, int qidx, double out[3]) {
+  const int32_t *g = c->IJK + 3 * (size_t)(qidx - 1);
+  double x[3];
+  for (int k = 0; k < 3; k++) x[k] = (double)g[k] / (double)c->ngrid[k];
+  for (int i = 0; i < 3; i++) {
+    double s = 0.0;
+    for (int k = 0; k < 3; k++) s = s + c->rlattvec[i + 3 * k] * x[k];
+    out[i] = s;
+  }
+}
+
+static inline double bose(const orc_case *c, double w) {
+  return 1.0 / (exp(HBAR_D * w / KB_D / c->T) - 1.0);
+}
+
+/* --------------------------------------------------------------------------
+ * Candidate enumeration shared by the count, list, RTA and iterative loops.
+ * Order and tests follow processes.f90:2291-2330 (++), 2382-2409 (+-),
+ * 2456-2474 (--).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  int ii, jj, ss, j, k, l;
+  double omega, omegap, omegadp, omegatp;
+  double delta;  /* the energy mismatch tested against sigma */
+  double sigma;  /* after the 0.001 clamp */
+} orc_hit;
+
+typedef void (*visit_fn)(const orc_case *c, const orc_hit *h, void *ud);
+
+static void enumerate(const orc_case *c, int ch, int mm, int ii_lo, int ii_hi, visit_fn f, void *ud) {
+  const int nb = c->nbands, nptk = c->nptk;
+  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
+  const int i = fmodulo(mm - 1, nb) + 1;
+  const int ll = (mm - 1) / nb + 1;
+  const int qi = c->List[ll - 1];
+  const int32_t *q = c->IJK + 3 * (size_t)(qi - 1);
+  orc_hit h;
+  h.omega = E(c, qi, i);
+  if (!(h.omega != 0.0)) return;
+  if (ii_lo < 1) ii_lo = 1;
+  if (ii_hi > nptk) ii_hi = nptk;
+  for (int ii = ii_lo; ii <= ii_hi; ii++) {
+    const int32_t *qp = c->IJK + 3 * (size_t)(ii - 1);
+    for (int jj = (ch == 1 ? 1 : ii); jj <= nptk; jj++) {
+      const int32_t *qdp = c->IJK + 3 * (size_t)(jj - 1);
+      int t0, t1, t2;
+      if (ch == 0) {
+        t0 = q[0] + qp[0] + qdp[0]; t1 = q[1] + qp[1] + qdp[1]; t2 = q[2] + qp[2] + qdp[2];
+      } else if (ch == 1) {
+        t0 = q[0] + qp[0] - qdp[0]; t1 = q[1] + qp[1] - qdp[1]; t2 = q[2] + qp[2] - qdp[2];
+      } else {
+        t0 = q[0] - qp[0] - qdp[0]; t1 = q[1] - qp[1] - qdp[1]; t2 = q[2] - qp[2] - qdp[2];
+      }
+      t0 = fmodulo(t0, n1); t1 = fmodulo(t1, n2); t2 = fmodulo(t2, n3);
+      const int ss = (t2 * n2 + t1) * n1 + t0 + 1;
+      if (ch == 0 ? !(ss >= 1) : !(ss >= jj)) continue;
+      for (int j = 1; j <= nb; j++) {
+        const int kstart = (ch != 1 && jj == ii) ? j : 1;
+        for (int k = kstart; k <= nb; k++) {
+          const int lstart = (ch != 0 && ss == jj) ? k : 1;
+          for (int l = lstart; l <= nb; l++) {
+            const double op = E(c, ii, j), odp = E(c, jj, k), otp = E(c, ss, l);
+            if (!((op != 0.0) && (odp != 0.0) && (otp != 0.0))) continue;
+            double dv[3];
+            for (int a = 1; a <= 3; a++) {
+              if (ch == 0)
+                dv[a - 1] = -VEL(c, jj, k, a) + VEL(c, ss, l, a);
+              else
+                dv[a - 1] = VEL(c, jj, k, a) - VEL(c, ss, l, a);
+            }
+            double sigma = c->scalebroad * orc_base_sigma(c, dv);
+            if (ch == 1) {
+              if (!((qi != jj || i != k) && (qi != ss || i != l) && (ii != jj || j != k) && (ii != ss || j != l)))
+                continue;
+            }
+            double delta;
+            if (ch == 0)
+              delta = h.omega + op + odp - otp;
+            else if (ch == 1)
+              delta = h.omega + op - odp - otp;
+            else
+              delta = h.omega - op - odp - otp;
+            if (fabs(delta) <= sigma) {
+              if (sigma <= (double)F_CLAMP) sigma = 1.0 / sqrt(PI_D);
+              h.ii = ii; h.jj = jj; h.ss = ss; h.j = j; h.k = k; h.l = l;
+              h.omegap = op; h.omegadp = odp; h.omegatp = otp;
+              h.delta = delta;
+              h.sigma = sigma;
+              f(c, &h, ud);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- counts */
+typedef struct { int64_t n; double p; double dn3, dnb4; } np_acc;
+
+/* P += exp(-d**2/sigma**2)/(sigma*sqrt(Pi)*dble(nptk)**3*nbands**4) */
+static void visit_np(const orc_case *c, const orc_hit *h, void *ud) {
+  np_acc *a = (np_acc *)ud;
+  (void)c;
+  a->n = a->n + 1;
+  const double num = exp(-(h->delta * h->delta) / (h->sigma * h->sigma));
+  const double den = h->sigma * sqrt(PI_D) * a->dn3 * a->dnb4;
+  a->p = a->p + num / den;
+}
+
+void orc_np_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P) {
+  np_acc a;
+  const double n = (double)c->nptk;
+  a.n = 0;
+  a.p = 0.0;
+  a.dn3 = n * n * n;
+  a.dnb4 = (double)(c->nbands * c->nbands * c->nbands * c->nbands);
+  enumerate(c, ch, mm, 1, c->nptk, visit_np, &a);
+  *N = a.n;
+  *P = a.p;
+}
+
+/* ----------------------------------------------------------------- lists */
+typedef struct { int64_t n, cap; int64_t *i2, *i3, *i4; int nb; } list_acc;
+
+static void visit_list(const orc_case *c, const orc_hit *h, void *ud) {
+  list_acc *a = (list_acc *)ud;
+  (void)c;
+  if (a->n < a->cap) {
+    a->i2[a->n] = (int64_t)((h->ii - 1) * a->nb + h->j);
+    a->i3[a->n] = (int64_t)((h->jj - 1) * a->nb + h->k);
+    a->i4[a->n] = (int64_t)((h->ss - 1) * a->nb + h->l);
+  }
+  a->n = a->n + 1;
+}
+
+int64_t orc_ind_only_mode(const orc_case *c, int ch, int mm, int64_t cap, int64_t *i2, int64_t *i3, int64_t *i4) {
+  list_acc a;
+  a.n = 0; a.cap = cap; a.i2 = i2; a.i3 = i3; a.i4 = i4; a.nb = c->nbands;
+  enumerate(c, ch, mm, 1, c->nptk, visit_list, &a);
+  return a.n;
+}
+
+/* ------------------------------------------------------ matrix elements */
+/* processes.f90:142-330.  Phase signs (phonons 2,3,4): ++ (+,+,-), +- (+,-,-),
+   -- (-,-,-); conj on phonon 4 (++), 3 and 4 (+-), 2, 3 and 4 (--). */
+void orc_vp(const orc_case *c, int ch, int i, int j, int k, int l, int q, int qp, int qdp, int qtp,
+            const double rqp[3], const double rqdp[3], const double rqtp[3], double out[2]) {
+  cplx V = {0.0, 0.0};
+  const int nt = c->ntri;
+  for (int t = 0; t < nt; t++) {
+    const int Ii = c->Index_i[t], Ij = c->Index_j[t], Ik = c->Index_k[t], Il = c->Index_l[t];
+    double mprod = c->masses[c->types[Ii - 1] - 1] * c->masses[c->types[Ij - 1] - 1];
+    mprod = mprod * c->masses[c->types[Ik - 1] - 1];
+    mprod = mprod * c->masses[c->types[Il - 1] - 1];
+    const double *rj = c->R_j + 3 * (size_t)t, *rk = c->R_k + 3 * (size_t)t, *rl = c->R_l + 3 * (size_t)t;
+    double dj = 0.0, dk = 0.0, dl = 0.0;
+    for (int a = 0; a < 3; a++) dj = dj + rqp[a] * rj[a];
+    for (int a = 0; a < 3; a++) dk = dk + rqdp[a] * rk[a];
+    for (int a = 0; a < 3; a++) dl = dl + rqtp[a] * rl[a];
+    cplx p2 = phexp(ch == 2 ? -dj : dj);
+    cplx p3 = phexp(ch == 0 ? dk : -dk);
+    cplx p4 = phexp(-dl);
+    cplx pre = rmul(1.0 / sqrt(mprod), p2);
+    pre = cmul(pre, p3);
+    pre = cmul(pre, p4);
+    cplx V0 = {0.0, 0.0};
+    const double *phi = c->Phi + 81 * (size_t)t;
+    for (int rr = 1; rr <= 3; rr++) {
+      for (int ss = 1; ss <= 3; ss++) {
+        for (int tt = 1; tt <= 3; tt++) {
+          for (int uu = 1; uu <= 3; uu++) {
+            const double f = phi[(uu - 1) + 3 * (tt - 1) + 9 * (ss - 1) + 27 * (rr - 1)];
+            cplx e1 = EV(c, q, i, uu + 3 * (Ii - 1));
+            cplx e2 = EV(c, qp, j, tt + 3 * (Ij - 1));
+            cplx e3 = EV(c, qdp, k, ss + 3 * (Ik - 1));
+            cplx e4 = EV(c, qtp, l, rr + 3 * (Il - 1));
+            if (ch == 2) e2 = conj_(e2);
+            if (ch != 0) e3 = conj_(e3);
+            e4 = conj_(e4);
+            cplx x = rmul(f, e1);
+            x = cmul(x, e2);
+            x = cmul(x, e3);
+            x = cmul(x, e4);
+            V0.re = V0.re + x.re;
+            V0.im = V0.im + x.im;
+          }
+        }
+      }
+    }
+    cplx y = cmul(pre, V0);
+    V.re = V.re + y.re;
+    V.im = V.im + y.im;
+  }
+  out[0] = V.re;
+  out[1] = V.im;
+}
+
+void orc_vp_idx(const orc_case *c, int ch, int i, int j, int k, int l, int q, int qp, int qdp, int qtp,
+                double out[2]) {
+  double a[3], b[3], d[3];
+  realq_of(c, qp, a);
+  realq_of(c, qdp, b);
+  realq_of(c, qtp, d);
+  orc_vp(c, ch, i, j, k, l, q, qp, qdp, qtp, a, b, d, out);
+}
+
+/* ------------------------------------------------------------------- RTA */
+typedef struct {
+  int i, qi, ch, nb;
+  double gamma, wp4;
+  /* iterative variant */
+  int store;
+  int64_t n, cap;
+  int64_t *i2, *i3, *i4;
+  double *g;
+} rta_acc;
+
+/* WP4 of one process, processes.f90:3895-3901 (++), 3988-3994 (+-), 4095-4101 (--) */
+static double wp4_of(const orc_case *c, int ch, const orc_hit *h) {
+  const double fp = bose(c, h->omegap), fdp = bose(c, h->omegadp), ftp = bose(c, h->omegatp);
+  double occ;
+  if (ch == 0)
+    occ = fp * fdp * (1 + ftp) - (1 + fp) * (1 + fdp) * ftp;
+  else if (ch == 1)
+    occ = fp * (1 + fdp) * (1 + ftp) - (1 + fp) * fdp * ftp;
+  else
+    occ = (1 + fp) * (1 + fdp) * (1 + ftp) - fp * fdp * ftp;
+  double w = occ * exp(-(h->delta * h->delta) / (h->sigma * h->sigma)) / h->sigma / sqrt(PI_D) /
+             (h->omega * h->omegap * h->omegadp * h->omegatp);
+  if (h->omegap <= 1.25 || h->omegadp <= 1.25 || h->omegatp <= 1.25) w = 0;
+  return w;
+}
+
+static void visit_rta(const orc_case *c, const orc_hit *h, void *ud) {
+  rta_acc *a = (rta_acc *)ud;
+  const double w = wp4_of(c, a->ch, h);
+  a->wp4 = a->wp4 + w;
+  const double hbarp = HBAR_D * (double)F_1E22;
+  if (a->store) {
+    double V[2];
+    orc_vp_idx(c, a->ch, a->i, h->j, h->k, h->l, a->qi, h->ii, h->jj, h->ss, V);
+    double av = hypot(V[0], V[1]);
+    double g = hbarp * hbarp * PI_D / 8.0 * w * (av * av);
+    g = g * (double)F_GAMMA_UNITS * 1.e9 / c->nptk / c->nptk;
+    if (a->n < a->cap) {
+      a->i2[a->n] = (int64_t)((h->ii - 1) * a->nb + h->j);
+      a->i3[a->n] = (int64_t)((h->jj - 1) * a->nb + h->k);
+      a->i4[a->n] = (int64_t)((h->ss - 1) * a->nb + h->l);
+      a->g[a->n] = g;
+    }
+    a->n = a->n + 1;
+  } else if (!c->onlyharmonic) {
+    double V[2];
+    orc_vp_idx(c, a->ch, a->i, h->j, h->k, h->l, a->qi, h->ii, h->jj, h->ss, V);
+    double av = hypot(V[0], V[1]);
+    a->gamma = a->gamma + hbarp * hbarp * PI_D / 8.0 * w * (av * av);
+  }
+}
+
+void orc_rta_mode(const orc_case *c, int ch, int mm, int32_t ii_lo, int32_t ii_hi, double *gamma, double *wp4) {
+  rta_acc a;
+  memset(&a, 0, sizeof(a));
+  a.i = fmodulo(mm - 1, c->nbands) + 1;
+  a.qi = c->List[(mm - 1) / c->nbands];
+  a.ch = ch;
+  a.nb = c->nbands;
+  const double omega = E(c, a.qi, a.i);
+  enumerate(c, ch, mm, ii_lo, ii_hi, visit_rta, &a);
+  if (omega != 0.0) a.wp4 = a.wp4 / c->nptk / c->nptk;
+  *gamma = a.gamma * (double)F_GAMMA_UNITS * 1.e9 / c->nptk / c->nptk;
+  *wp4 = a.wp4;
+}
+
+int64_t orc_ind_mode(const orc_case *c, int ch, int mm, int64_t cap, int64_t *i2, int64_t *i3, int64_t *i4,
+                     double *gamma, double *wp4) {
+  rta_acc a;
+  memset(&a, 0, sizeof(a));
+  a.i = fmodulo(mm - 1, c->nbands) + 1;
+  a.qi = c->List[(mm - 1) / c->nbands];
+  a.ch = ch;
+  a.nb = c->nbands;
+  a.store = 1;
+  a.cap = cap;
+  a.i2 = i2; a.i3 = i3; a.i4 = i4; a.g = gamma;
+  const double omega = E(c, a.qi, a.i);
+  enumerate(c, ch, mm, 1, c->nptk, visit_rta, &a);
+  if (omega != 0.0) a.wp4 = a.wp4 / c->nptk / c->nptk;
+  *wp4 = a.wp4;
+  return a.n;
+}
+
+/* --------------------------------------------------------------- drivers */
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+/* processes.f90:2749-2847 */
+void orc_np_driver(const orc_case *c, int nthreads, int64_t *N_pp, double *P_pp, int64_t *N_pm, double *P_pm,
+                   int64_t *N_mm, double *P_mm) {
+  const int nm = c->nbands * c->nlist;
+  set_threads(nthreads);
+  for (int m = 0; m < nm; m++) {
+    N_pp[m] = N_pm[m] = N_mm[m] = 0;
+    P_pp[m] = P_pm[m] = P_mm[m] = 0.0;
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (E(c, c->List[ll - 1], i) <= c->omega_max) {
+      orc_np_mode(c, 0, mm, &N_pp[mm - 1], &P_pp[mm - 1]);
+      orc_np_mode(c, 1, mm, &N_pm[mm - 1], &P_pm[mm - 1]);
+      orc_np_mode(c, 2, mm, &N_mm[mm - 1], &P_mm[mm - 1]);
+    }
+  }
+}
+
+/* processes.f90:3080-3246.  Returns 0, or -1 when a list is longer than its count. */
+int orc_ind_only_driver(const orc_case *c, int nthreads, const int64_t *N_pp, const int64_t *N_pm,
+                        const int64_t *N_mm, int64_t *i2_pp, int64_t *i3_pp, int64_t *i4_pp, int64_t *i2_pm,
+                        int64_t *i3_pm, int64_t *i4_pm, int64_t *i2_mm, int64_t *i3_mm, int64_t *i4_mm) {
+  const int nm = c->nbands * c->nlist;
+  const int64_t *Ns[3] = {N_pp, N_pm, N_mm};
+  int64_t *out[3][3] = {{i2_pp, i3_pp, i4_pp}, {i2_pm, i3_pm, i4_pm}, {i2_mm, i3_mm, i4_mm}};
+  int64_t *acc[3];
+  int bad = 0;
+  for (int ch = 0; ch < 3; ch++) {
+    acc[ch] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nm + 1));
+    acc[ch][0] = 0;
+    for (int m = 0; m < nm; m++) acc[ch][m + 1] = acc[ch][m] + Ns[ch][m];
+    for (int64_t t = 0; t < acc[ch][nm]; t++) out[ch][0][t] = out[ch][1][t] = out[ch][2][t] = 0;
+  }
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    for (int ch = 0; ch < 3; ch++) {
+      const int64_t n = Ns[ch][mm - 1];
+      if (n == 0) continue;
+      const int64_t off = acc[ch][mm - 1];
+      int64_t got = orc_ind_only_mode(c, ch, mm, n, out[ch][0] + off, out[ch][1] + off, out[ch][2] + off);
+      if (got != n) bad |= 1;
+    }
+  }
+  for (int ch = 0; ch < 3; ch++) free(acc[ch]);
+  return bad ? -1 : 0;
+}
+
+/* processes.f90:5174-5311, restricted to a list of modes and a q' window. */
+void orc_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi,
+                    double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm, double *wp4_pp,
+                    double *wp4_pm, double *wp4_mm) {
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < nmodes; t++) {
+    const int mm = modes[t];
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    double g, w;
+    orc_rta_mode(c, 0, mm, ii_lo, ii_hi, &g, &w);
+    rate_pp[mm - 1] = g; wp4_pp[mm - 1] = w;
+    orc_rta_mode(c, 1, mm, ii_lo, ii_hi, &g, &w);
+    rate_pm[mm - 1] = g; wp4_pm[mm - 1] = w;
+    orc_rta_mode(c, 2, mm, ii_lo, ii_hi, &g, &w);
+    rate_mm[mm - 1] = g; wp4_mm[mm - 1] = w;
+  }
+  for (int t = 0; t < nmodes; t++) {
+    const int m = modes[t] - 1;
+    rate_4ph[m] = rate_pp[m] + rate_pm[m] + rate_mm[m];
+  }
+}
+
+/* processes.f90:3551-3822 (the momentum test of Ind_* selects the same ss as
+   the direct Index_N lookup, so the enumeration is shared). */
+int orc_ind_driver(const orc_case *c, int nthreads, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
+                   int64_t *i2_pp, int64_t *i3_pp, int64_t *i4_pp, int64_t *i2_pm, int64_t *i3_pm, int64_t *i4_pm,
+                   int64_t *i2_mm, int64_t *i3_mm, int64_t *i4_mm, double *g_pp, double *g_pm, double *g_mm,
+                   double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm, double *wp4_pp,
+                   double *wp4_pm, double *wp4_mm) {
+  const int nm = c->nbands * c->nlist;
+  const int64_t *Ns[3] = {N_pp, N_pm, N_mm};
+  int64_t *out[3][3] = {{i2_pp, i3_pp, i4_pp}, {i2_pm, i3_pm, i4_pm}, {i2_mm, i3_mm, i4_mm}};
+  double *gs[3] = {g_pp, g_pm, g_mm};
+  double *rates[3] = {rate_pp, rate_pm, rate_mm};
+  double *wps[3] = {wp4_pp, wp4_pm, wp4_mm};
+  int64_t *acc[3];
+  int bad = 0;
+  for (int ch = 0; ch < 3; ch++) {
+    acc[ch] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nm + 1));
+    acc[ch][0] = 0;
+    for (int m = 0; m < nm; m++) acc[ch][m + 1] = acc[ch][m] + Ns[ch][m];
+    for (int m = 0; m < nm; m++) rates[ch][m] = wps[ch][m] = 0.0;
+  }
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int mm = 1; mm <= nm; mm++) {
+    for (int ch = 0; ch < 3; ch++) {
+      const int64_t n = Ns[ch][mm - 1];
+      if (n == 0) continue;
+      const int64_t off = acc[ch][mm - 1];
+      double w;
+      int64_t got = orc_ind_mode(c, ch, mm, n, out[ch][0] + off, out[ch][1] + off, out[ch][2] + off, gs[ch] + off, &w);
+      if (got != n) bad |= 1;
+      wps[ch][mm - 1] = w;
+      double s = 0.0;
+      for (int64_t t = 0; t < n; t++) s = s + gs[ch][off + t];
+      rates[ch][mm - 1] = s;
+    }
+  }
+  for (int m = 0; m < nm; m++) rate_4ph[m] = rate_pp[m] + rate_pm[m] + rate_mm[m];
+  for (int ch = 0; ch < 3; ch++) free(acc[ch]);
+  return bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------ iterations */
+#define FN(F, nb, nptk, b, q, a) (F)[(size_t)((b)-1) + (size_t)(nb) * ((size_t)((q)-1) + (size_t)(nptk) * ((a)-1))]
+
+/* iterations.f90:13-30 */
+void orc_iteration0(int nptk, int nbands, int nlist, int nsymm_rot, const int32_t *Nequi, const int32_t *ALLEquiList,
+                    const double *omega, const double *velocity, const double *tau_zero, double *F_n) {
+  for (int ll = 1; ll <= nlist; ll++)
+    for (int ii = 1; ii <= nbands; ii++)
+      for (int kk = 1; kk <= Nequi[ll - 1]; kk++) {
+        const int q = ALLEquiList[(kk - 1) + (size_t)nsymm_rot * (ll - 1)];
+        for (int a = 1; a <= 3; a++) {
+          const double v = velocity[(size_t)(q - 1) + (size_t)nptk * ((size_t)(ii - 1) + (size_t)nbands * (a - 1))];
+          const double w = omega[(size_t)(q - 1) + (size_t)nptk * (ii - 1)];
+          FN(F_n, nbands, nptk, ii, q, a) = tau_zero[(ii - 1) + (size_t)nbands * (ll - 1)] * v * w;
+        }
+      }
+}
+
+/* iterations.f90:95-215, four-phonon terms (the three-phonon lists are empty). */
+void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot, const int32_t *Nequi,
+                       const int32_t *ALLEquiList, const int32_t *TypeofSymmetry, const int32_t *ID_equi,
+                       const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm, const int64_t *i2_pp,
+                       const int64_t *i3_pp, const int64_t *i4_pp, const int64_t *i2_pm, const int64_t *i3_pm,
+                       const int64_t *i4_pm, const int64_t *i2_mm, const int64_t *i3_mm, const int64_t *i4_mm,
+                       const double *omega, const double *velocity, const double *g_pp, const double *g_pm,
+                       const double *g_mm, const double *tau_zero, const double *symmetrizers, double *F_n) {
+  const int nb = nbands;
+  double *dF = (double *)calloc((size_t)nb * nptk * 3, sizeof(double));
+  int64_t acc[3] = {0, 0, 0};
+  const int64_t *Ns[3] = {N_pp, N_pm, N_mm};
+  const int64_t *I2[3] = {i2_pp, i2_pm, i2_mm};
+  const int64_t *I3[3] = {i3_pp, i3_pm, i3_mm};
+  const int64_t *I4[3] = {i4_pp, i4_pm, i4_mm};
+  const double *G[3] = {g_pp, g_pm, g_mm};
+#define IDEQ(s, q) ID_equi[(size_t)((s)-1) + (size_t)nsymm_rot * ((q)-1)]
+  for (int ll = 1; ll <= nlist; ll++) {
+    for (int i = 1; i <= nb; i++) {
+      const int mm = (ll - 1) * nb + i;
+      if (mm != 1)
+        for (int ch = 0; ch < 3; ch++) acc[ch] = acc[ch] + Ns[ch][mm - 2];
+      for (int kk = 1; kk <= Nequi[ll - 1]; kk++) {
+        const int qk = ALLEquiList[(kk - 1) + (size_t)nsymm_rot * (ll - 1)];
+        const int sym = TypeofSymmetry[(kk - 1) + (size_t)nsymm_rot * (ll - 1)];
+        for (int ch = 0; ch < 3; ch++) {
+          const int64_t n = Ns[ch][mm - 1];
+          for (int64_t t = 0; t < n; t++) {
+            const int64_t p = acc[ch] + t;
+            const int j = (int)((I2[ch][p] - 1) % nb) + 1, qm = (int)((I2[ch][p] - 1) / nb) + 1;
+            const int k = (int)((I3[ch][p] - 1) % nb) + 1, qn = (int)((I3[ch][p] - 1) / nb) + 1;
+            const int l = (int)((I4[ch][p] - 1) % nb) + 1, qs = (int)((I4[ch][p] - 1) / nb) + 1;
+            const int m2 = IDEQ(sym, qm), n2 = IDEQ(sym, qn), s2 = IDEQ(sym, qs);
+            for (int a = 1; a <= 3; a++) {
+              const double Fj = FN(F_n, nb, nptk, j, m2, a);
+              const double Fk = FN(F_n, nb, nptk, k, n2, a);
+              const double Fl = FN(F_n, nb, nptk, l, s2, a);
+              double term;
+              if (ch == 0)
+                term = Fl - Fj - Fk;
+              else if (ch == 1)
+                term = Fl - Fj + Fk;
+              else
+                term = Fl + Fj + Fk;
+              FN(dF, nb, nptk, i, qk, a) = FN(dF, nb, nptk, i, qk, a) + G[ch][p] * term;
+            }
+          }
+        }
+        const double tz = tau_zero[(i - 1) + (size_t)nb * (ll - 1)];
+        for (int a = 1; a <= 3; a++) {
+          const double v = velocity[(size_t)(qk - 1) + (size_t)nptk * ((size_t)(i - 1) + (size_t)nb * (a - 1))];
+          const double w = omega[(size_t)(qk - 1) + (size_t)nptk * (i - 1)];
+          FN(F_n, nb, nptk, i, qk, a) = tz * v * w + tz * FN(dF, nb, nptk, i, qk, a);
+        }
+        const double *S = symmetrizers + 9 * (size_t)(qk - 1);
+        for (int b = 1; b <= nb; b++) {
+          double x[3], y[3];
+          for (int a = 0; a < 3; a++) x[a] = FN(F_n, nb, nptk, b, qk, a + 1);
+          for (int a = 0; a < 3; a++) {
+            double s = 0.0;
+            for (int c2 = 0; c2 < 3; c2++) s = s + S[a + 3 * c2] * x[c2];
+            y[a] = s;
+          }
+          for (int a = 0; a < 3; a++) FN(F_n, nb, nptk, b, qk, a + 1) = y[a];
+        }
+      }
+    }
+  }
+#undef IDEQ
+  free(dF);
+}
+
+/* conductivity.f90:11-37; kappa is (Nbands,3,3) */
+void orc_tconduct(int nptk, int nbands, double T, double V, const double *omega, const double *velocity,
+                  const double *F_n, double *kappa) {
+  for (int t = 0; t < nbands * 9; t++) kappa[t] = 0.0;
+  for (int jj = 1; jj <= nbands; jj++) {
+    for (int ii = 2; ii <= nptk; ii++) {
+      double tmp[3][3];
+      for (int d1 = 1; d1 <= 3; d1++)
+        for (int d2 = 1; d2 <= 3; d2++)
+          tmp[d1 - 1][d2 - 1] = velocity[(size_t)(ii - 1) + (size_t)nptk * ((size_t)(jj - 1) + (size_t)nbands * (d1 - 1))] *
+                                FN(F_n, nbands, nptk, jj, ii, d2);
+      const double w = omega[(size_t)(ii - 1) + (size_t)nptk * (jj - 1)];
+      const double f = 1.0 / (exp(HBAR_D * w / KB_D / T) - 1.0);
+      for (int d1 = 0; d1 < 3; d1++)
+        for (int d2 = 0; d2 < 3; d2++) {
+          const double mode = f * (f + 1) * w * tmp[d1][d2];
+          kappa[(jj - 1) + nbands * (d1 + 3 * d2)] = kappa[(jj - 1) + nbands * (d1 + 3 * d2)] + mode;
+        }
+    }
+  }
+  const double pre = (double)F_1E21 * (HBAR_D * HBAR_D);
+  const double den = KB_D * T * T * V * nptk;
+  for (int t = 0; t < nbands * 9; t++) kappa[t] = pre * kappa[t] / den;
+}
