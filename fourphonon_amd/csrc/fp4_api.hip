@@ -1,0 +1,878 @@
+// Host runtime and C ABI of the fp4 library (declared in include/fp4.h).
+//
+// The context owns one HIP stream (default priority), the device copies of
+// the mode data and the IFC clusters, and a few grow-only scratch buffers.
+// Every entry point runs the same pipeline:
+//   count pass  -> per-item survivor counts (kernel a)
+//   host scan   -> exclusive offsets, batch plan under a device-memory cap
+//   per batch   -> emit pass (kernel a) -> rate kernel -> segment reduction
+// and accumulates per-(channel, mode) results in batch order, so results do
+// not depend on the batch size.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fp4.h"
+#include "fp4_dense.hpp"
+#include "fp4_kernels.hpp"
+
+using namespace fp4;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(x)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(FP4_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
+      p = nullptr;
+      return fail(FP4_ERR_NOMEM, "hipMalloc(%zu) failed", need);
+    }
+    bytes = need;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_N = 4 };
+
+}  // namespace
+
+struct fp4_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int ngrid[3] = {0, 0, 0};
+  int nptk = 0, nb = 0, natoms = 0, nelem = 0, nlist = 0;
+  double rlattvec[9];
+  double lattvec[9];
+  double T = 300.0, scalebroad = 1.0, omega_max = 1e100;
+  int onlyharmonic = 0;
+  std::vector<double> masses;
+  std::vector<int> types;
+  bool have_modes = false, have_ifc = false;
+  // mode data
+  DevBuf energy, vel, vbound, eig, ijk, list;
+  // clusters
+  int ntri = 0, L = 1;
+  DevBuf clusters, phi, roots;
+  std::vector<ClusterRec> h_clusters;
+  std::vector<double> h_phi;
+  // scratch
+  DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g;
+  DevBuf dense_scratch;
+  long long list_cap = 1LL << 26;   // processes per batch
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ks_ms[KS_N] = {0, 0, 0, 0};
+  long long ks_n[KS_N] = {0, 0, 0, 0};
+  DenseTuning dense_tuning;
+};
+
+namespace {
+
+ModeView mode_view(const fp4_ctx *c) {
+  ModeView m;
+  m.energy = c->energy.as<double>();
+  m.vel = c->vel.as<double>();
+  m.vbound = c->vbound.as<float>();
+  m.eig = c->eig.as<double2>();
+  m.ijk = c->ijk.as<int>();
+  m.list = c->list.as<int>();
+  m.nptk = c->nptk;
+  m.nb = c->nb;
+  m.nlist = c->nlist;
+  m.n1 = c->ngrid[0];
+  m.n2 = c->ngrid[1];
+  m.n3 = c->ngrid[2];
+  for (int t = 0; t < 9; t++) m.b[t] = c->rlattvec[t];
+  m.scalebroad = c->scalebroad;
+  m.omega_max = c->omega_max;
+  return m;
+}
+
+IfcView ifc_view(const fp4_ctx *c) {
+  IfcView f;
+  f.cl = c->clusters.as<ClusterRec>();
+  f.phi = c->phi.as<double>();
+  f.roots = c->roots.as<double2>();
+  f.ntri = c->ntri;
+  f.L = c->L;
+  return f;
+}
+
+bool nb_supported(int nb) { return nb == 3 || nb == 6 || nb == 9 || nb == 12 || nb == 18 || nb == 24; }
+
+// Timed launch helper: HIP events on the context stream around one launch.
+template <typename F>
+int timed(fp4_ctx *c, int kind, F &&launch) {
+  HIPCHECK(hipEventRecord(c->ev0, c->stream));
+  int rc = launch();
+  if (rc) return rc;
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->ks_ms[kind] += ms;
+  c->ks_n[kind] += 1;
+  return 0;
+}
+
+// ---------------------------------------------------------------- pipeline
+struct Plan {
+  std::vector<int> modes;          // mm (1-based) of the subset
+  int ii_lo0 = 0, nii = 0;
+  long items = 0;
+  std::vector<long long> off;      // exclusive offsets, size items+1
+  std::vector<long long> segN;     // [3*nsub] processes per (ch, mode)
+  std::vector<double> segP;        // [3*nsub] phase space partial sums
+};
+
+int count_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
+  const int nsub = (int)pl.modes.size();
+  pl.items = 3L * nsub * pl.nii;
+  const int nseg = 3 * nsub;
+  int rc;
+  if ((rc = c->modes_sub.ensure(sizeof(int) * std::max(nsub, 1)))) return rc;
+  if ((rc = c->cnt.ensure(sizeof(int) * std::max(pl.items, 1L)))) return rc;
+  if ((rc = c->psum.ensure(sizeof(double) * std::max(pl.items, 1L)))) return rc;
+  if ((rc = c->segN.ensure(sizeof(long long) * std::max(nseg, 1)))) return rc;
+  if ((rc = c->segP.ensure(sizeof(double) * std::max(nseg, 1)))) return rc;
+  HIPCHECK(hipMemcpyAsync(c->modes_sub.p, pl.modes.data(), sizeof(int) * nsub, hipMemcpyHostToDevice, c->stream));
+  PrefilterArgs a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = nsub;
+  a.nii = pl.nii;
+  a.ii_lo0 = pl.ii_lo0;
+  a.item_lo = 0;
+  a.item_hi = pl.items;
+  a.cnt = c->cnt.as<int>();
+  a.psum = c->psum.as<double>();
+  if ((rc = timed(c, KS_PREFILTER, [&] {
+         return launch_prefilter(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+       })))
+    return rc;
+  launch_reduce_counts(c->cnt.as<int>(), c->psum.as<double>(), nseg, pl.nii, c->segN.as<long long>(),
+                       c->segP.as<double>(), c->stream);
+  HIPCHECK(hipGetLastError());
+  pl.segN.assign(nseg, 0);
+  pl.segP.assign(nseg, 0.0);
+  HIPCHECK(hipMemcpyAsync(pl.segN.data(), c->segN.p, sizeof(long long) * nseg, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(pl.segP.data(), c->segP.p, sizeof(double) * nseg, hipMemcpyDeviceToHost, c->stream));
+  if (want_offsets) {
+    std::vector<int> h(pl.items);
+    HIPCHECK(hipMemcpyAsync(h.data(), c->cnt.p, sizeof(int) * pl.items, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    pl.off.resize(pl.items + 1);
+    long long s = 0;
+    for (long t = 0; t < pl.items; t++) {
+      pl.off[t] = s;
+      s += h[t];
+    }
+    pl.off[pl.items] = s;
+    if ((rc = c->offsets.ensure(sizeof(long long) * (pl.items + 1)))) return rc;
+    HIPCHECK(hipMemcpyAsync(c->offsets.p, pl.off.data(), sizeof(long long) * (pl.items + 1), hipMemcpyHostToDevice,
+                            c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+struct Batch {
+  long item_lo, item_hi;
+  long long p0, p1;
+};
+
+std::vector<Batch> plan_batches(const Plan &pl, long long cap) {
+  std::vector<Batch> out;
+  long lo = 0;
+  while (lo < pl.items) {
+    // largest hi with off[hi]-off[lo] <= cap
+    const long long lim = pl.off[lo] + cap;
+    long hi = (long)(std::upper_bound(pl.off.begin() + lo + 1, pl.off.end(), lim) - pl.off.begin()) - 1;
+    if (hi <= lo) hi = lo + 1;
+    if (pl.off[hi] > pl.off[lo]) out.push_back({lo, hi, pl.off[lo], pl.off[hi]});
+    lo = hi;
+  }
+  return out;
+}
+
+// Block descriptors for one batch; returns the per-piece (ch, mode-in-subset) map.
+void plan_blocks(const Plan &pl, const Batch &b, std::vector<BlockDesc> &blocks, std::vector<int> &seg_first,
+                 std::vector<int> &piece_seg) {
+  blocks.clear();
+  seg_first.clear();
+  piece_seg.clear();
+  const int nsub = (int)pl.modes.size();
+  const long s_lo = b.item_lo / pl.nii, s_hi = (b.item_hi - 1) / pl.nii;
+  for (long s = s_lo; s <= s_hi; s++) {
+    const long i0 = std::max(b.item_lo, s * pl.nii), i1 = std::min(b.item_hi, (s + 1) * pl.nii);
+    const long long a = pl.off[i0], e = pl.off[i1];
+    if (e <= a) continue;
+    const int ch = (int)(s / nsub), mi = (int)(s % nsub);
+    const int piece = (int)piece_seg.size();
+    piece_seg.push_back((int)s);
+    seg_first.push_back((int)blocks.size());
+    for (long long p = a; p < e; p += 256) {
+      BlockDesc d;
+      d.start = p - b.p0;
+      d.count = (int)std::min<long long>(256, e - p);
+      d.ch = ch;
+      d.mm = pl.modes[mi];
+      d.seg = piece;
+      blocks.push_back(d);
+    }
+  }
+  seg_first.push_back((int)blocks.size());
+}
+
+int emit_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide) {
+  int rc;
+  const long long n = b.p1 - b.p0;
+  for (int t = 0; t < 3; t++)
+    if ((rc = c->lists[t].ensure((wide ? 8 : 4) * (size_t)std::max(n, 1LL)))) return rc;
+  PrefilterArgs a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = (int)pl.modes.size();
+  a.nii = pl.nii;
+  a.ii_lo0 = pl.ii_lo0;
+  a.item_lo = b.item_lo;
+  a.item_hi = b.item_hi;
+  a.offsets = c->offsets.as<long long>();
+  a.out_base = b.p0;
+  for (int t = 0; t < 3; t++) {
+    a.out32[t] = c->lists[t].as<int>();
+    a.out64[t] = c->lists[t].as<long long>();
+  }
+  return timed(c, KS_EMIT, [&] { return launch_prefilter(a, wide ? 2 : 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+}
+
+// Rate kernel over a batch whose lists are in c->lists (int32, or int64 when wide).
+int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store_gamma, std::vector<double> &acc_g,
+               std::vector<double> &acc_w) {
+  std::vector<BlockDesc> blocks;
+  std::vector<int> seg_first, piece_seg;
+  plan_blocks(pl, b, blocks, seg_first, piece_seg);
+  if (blocks.empty()) return 0;
+  int rc;
+  const int nblk = (int)blocks.size(), npc = (int)piece_seg.size();
+  if ((rc = c->blk.ensure(sizeof(BlockDesc) * nblk))) return rc;
+  if ((rc = c->blk_g.ensure(sizeof(double) * nblk))) return rc;
+  if ((rc = c->blk_w.ensure(sizeof(double) * nblk))) return rc;
+  if ((rc = c->seg_first.ensure(sizeof(int) * (npc + 1)))) return rc;
+  if ((rc = c->seg_g.ensure(sizeof(double) * npc))) return rc;
+  if ((rc = c->seg_w.ensure(sizeof(double) * npc))) return rc;
+  if (store_gamma && (rc = c->proc_g.ensure(sizeof(double) * (size_t)std::max(b.p1 - b.p0, 1LL)))) return rc;
+  HIPCHECK(hipMemcpyAsync(c->blk.p, blocks.data(), sizeof(BlockDesc) * nblk, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->seg_first.p, seg_first.data(), sizeof(int) * (npc + 1), hipMemcpyHostToDevice, c->stream));
+  RateArgs a{};
+  a.m = mode_view(c);
+  a.f = ifc_view(c);
+  for (int t = 0; t < 3; t++) {
+    a.lst[t] = wide ? nullptr : c->lists[t].as<int>();
+    a.lst64[t] = wide ? c->lists[t].as<long long>() : nullptr;
+  }
+  a.blocks = c->blk.as<BlockDesc>();
+  a.nblocks = nblk;
+  a.T = c->T;
+  a.onlyharmonic = c->onlyharmonic;
+  a.store_gamma = store_gamma;
+  a.proc_gamma = c->proc_g.as<double>();
+  a.blk_gamma = c->blk_g.as<double>();
+  a.blk_wp4 = c->blk_w.as<double>();
+  if ((rc = timed(c, KS_RATE64, [&] { return launch_rate64(a, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; })))
+    return rc;
+  launch_seg_reduce(c->blk_g.as<double>(), c->blk_w.as<double>(), c->seg_first.as<int>(), npc,
+                    c->seg_g.as<double>(), c->seg_w.as<double>(), c->stream);
+  HIPCHECK(hipGetLastError());
+  std::vector<double> g(npc), w(npc);
+  HIPCHECK(hipMemcpyAsync(g.data(), c->seg_g.p, sizeof(double) * npc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(w.data(), c->seg_w.p, sizeof(double) * npc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int t = 0; t < npc; t++) {
+    acc_g[piece_seg[t]] = acc_g[piece_seg[t]] + g[t];
+    acc_w[piece_seg[t]] = acc_w[piece_seg[t]] + w[t];
+  }
+  return 0;
+}
+
+int check_ready(const fp4_ctx *c, bool need_ifc) {
+  if (!c) return fail(FP4_ERR_ARG, "null context");
+  if (!c->have_modes) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes has not been called");
+  if (need_ifc && !c->have_ifc) return fail(FP4_ERR_STATE, "fp4_ctx_set_ifc4 has not been called");
+  HIPCHECK(hipSetDevice(c->device));
+  return 0;
+}
+
+void write_rates(const fp4_ctx *c, const Plan &pl, const std::vector<double> &g, const std::vector<double> &w,
+                 double *rate_4ph, double *r_pp, double *r_pm, double *r_mm, double *w_pp, double *w_pm, double *w_mm,
+                 bool scale_gamma) {
+  const int nsub = (int)pl.modes.size();
+  double *R[3] = {r_pp, r_pm, r_mm};
+  double *W[3] = {w_pp, w_pm, w_mm};
+  std::vector<double> host_e(1);
+  for (int mi = 0; mi < nsub; mi++) {
+    const int m = pl.modes[mi] - 1;
+    for (int ch = 0; ch < 3; ch++) {
+      const double gs = g[ch * nsub + mi], ws = w[ch * nsub + mi];
+      R[ch][m] = scale_gamma ? gs * (double)kGammaUnits * 1.e9 / c->nptk / c->nptk : gs;
+      W[ch][m] = ws / c->nptk / c->nptk;
+    }
+    rate_4ph[m] = r_pp[m] + r_pm[m] + r_mm[m];
+  }
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+const char *fp4_last_error(void) { return g_err.c_str(); }
+
+int fp4_ctx_create(const fp4_config *cfg, fp4_ctx **out) {
+  if (!cfg || !out) return fail(FP4_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->natoms < 1 || cfg->nbands != 3 * cfg->natoms) return fail(FP4_ERR_ARG, "nbands must be 3*natoms");
+  if (!nb_supported(cfg->nbands)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d not built", cfg->nbands);
+  for (int t = 0; t < 3; t++)
+    if (cfg->ngrid[t] < 1) return fail(FP4_ERR_ARG, "ngrid must be positive");
+  if (cfg->nelements < 1 || !cfg->masses || !cfg->types) return fail(FP4_ERR_ARG, "masses/types missing");
+  for (int a = 0; a < cfg->natoms; a++)
+    if (cfg->types[a] < 1 || cfg->types[a] > cfg->nelements) return fail(FP4_ERR_ARG, "types out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FP4_ERR_DEVICE, "no HIP device");
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(FP4_ERR_DEVICE, "device %d out of range", cfg->device);
+  fp4_ctx *c = new fp4_ctx();
+  c->device = cfg->device;
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(FP4_ERR_DEVICE, "stream/event creation failed");
+  }
+  for (int t = 0; t < 3; t++) c->ngrid[t] = cfg->ngrid[t];
+  c->nptk = cfg->ngrid[0] * cfg->ngrid[1] * cfg->ngrid[2];
+  c->nb = cfg->nbands;
+  c->natoms = cfg->natoms;
+  c->nelem = cfg->nelements;
+  std::memcpy(c->rlattvec, cfg->rlattvec, sizeof(c->rlattvec));
+  // lattvec = 2 pi inverse(rlattvec)^T
+  const double *b = cfg->rlattvec;
+  double det = b[0] * (b[4] * b[8] - b[7] * b[5]) - b[3] * (b[1] * b[8] - b[7] * b[2]) + b[6] * (b[1] * b[5] - b[4] * b[2]);
+  if (!(std::fabs(det) > 0)) {
+    fp4_ctx_destroy(c);
+    return fail(FP4_ERR_ARG, "singular rlattvec");
+  }
+  c->T = cfg->T;
+  c->scalebroad = cfg->scalebroad;
+  c->omega_max = cfg->omega_max;
+  c->onlyharmonic = cfg->onlyharmonic;
+  c->masses.assign(cfg->masses, cfg->masses + cfg->nelements);
+  c->types.assign(cfg->types, cfg->types + cfg->natoms);
+  *out = c;
+  return FP4_OK;
+}
+
+int fp4_ctx_destroy(fp4_ctx *c) {
+  if (!c) return FP4_OK;
+  (void)hipSetDevice(c->device);
+  DevBuf *bufs[] = {&c->energy, &c->vel, &c->vbound, &c->eig, &c->ijk, &c->list, &c->clusters, &c->phi, &c->roots,
+                    &c->modes_sub, &c->cnt, &c->psum, &c->offsets, &c->segN, &c->segP, &c->lists[0], &c->lists[1],
+                    &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g,
+                    &c->dense_scratch};
+  for (DevBuf *b : bufs) b->release();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FP4_OK;
+}
+
+int fp4_ctx_set_temperature(fp4_ctx *c, double T) {
+  if (!c) return fail(FP4_ERR_ARG, "null context");
+  if (!(T > 0)) return fail(FP4_ERR_ARG, "T must be positive");
+  c->T = T;
+  return FP4_OK;
+}
+
+int fp4_ctx_set_modes(fp4_ctx *c, const double *energy, const double *velocity, const double *eigenvect,
+                      const int32_t *IJK, int32_t nlist, const int32_t *List) {
+  if (!c || !energy || !velocity || !eigenvect || !IJK || !List) return fail(FP4_ERR_ARG, "null argument");
+  if (nlist < 1 || nlist > c->nptk) return fail(FP4_ERR_ARG, "Nlist out of range");
+  for (int t = 0; t < nlist; t++)
+    if (List[t] < 1 || List[t] > c->nptk) return fail(FP4_ERR_ARG, "List(%d)=%d out of range", t + 1, List[t]);
+  for (int q = 0; q < c->nptk; q++)
+    for (int t = 0; t < 3; t++)
+      if (IJK[3 * q + t] < 0 || IJK[3 * q + t] >= c->ngrid[t]) return fail(FP4_ERR_ARG, "IJK out of range");
+  HIPCHECK(hipSetDevice(c->device));
+  const size_t np = c->nptk, nb = c->nb;
+  int rc;
+  if ((rc = c->energy.ensure(sizeof(double) * np * nb)) || (rc = c->vel.ensure(sizeof(double) * np * nb * 3)) ||
+      (rc = c->vbound.ensure(sizeof(float) * np * nb)) || (rc = c->eig.ensure(sizeof(double) * 2 * np * nb * nb)) ||
+      (rc = c->ijk.ensure(sizeof(int) * 3 * np)) || (rc = c->list.ensure(sizeof(int) * nlist)))
+    return rc;
+  // eigenvectors: reference (q, b, comp) with q fastest -> [q][b][comp]
+  std::vector<double> eig(2 * np * nb * nb);
+  for (size_t comp = 0; comp < nb; comp++)
+    for (size_t b = 0; b < nb; b++)
+      for (size_t q = 0; q < np; q++) {
+        const size_t src = q + np * (b + nb * comp), dst = (q * nb + b) * nb + comp;
+        eig[2 * dst] = eigenvect[2 * src];
+        eig[2 * dst + 1] = eigenvect[2 * src + 1];
+      }
+  // conservative broadening bound per mode (see fp4_prefilter.hip)
+  double bm = 0.0;
+  for (int nu = 0; nu < 3; nu++) {
+    const double *bv = c->rlattvec + 3 * nu;
+    bm = std::max(bm, std::sqrt(bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2]) / c->ngrid[nu]);
+  }
+  std::vector<float> vb(np * nb);
+  for (size_t b = 0; b < nb; b++)
+    for (size_t q = 0; q < np; q++) {
+      const double *v = velocity;
+      const double vx = v[q + np * (b + nb * 0)], vy = v[q + np * (b + nb * 1)], vz = v[q + np * (b + nb * 2)];
+      const double val = std::fabs(c->scalebroad) * bm * std::sqrt(vx * vx + vy * vy + vz * vz) / std::sqrt(2.0);
+      float f = (float)(val * (1.0 + 1e-6));
+      vb[b * np + q] = std::nextafter(f, INFINITY);
+    }
+  HIPCHECK(hipMemcpy(c->energy.p, energy, sizeof(double) * np * nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->vel.p, velocity, sizeof(double) * np * nb * 3, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->vbound.p, vb.data(), sizeof(float) * np * nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->eig.p, eig.data(), sizeof(double) * 2 * np * nb * nb, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ijk.p, IJK, sizeof(int) * 3 * np, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->list.p, List, sizeof(int) * nlist, hipMemcpyHostToDevice));
+  c->nlist = nlist;
+  c->have_modes = true;
+  return FP4_OK;
+}
+
+int fp4_ctx_set_ifc4(fp4_ctx *c, int32_t ntri, const double *Phi, const double *R_j, const double *R_k,
+                     const double *R_l, const int32_t *Ii, const int32_t *Ij, const int32_t *Ik, const int32_t *Il) {
+  if (!c || !Phi || !R_j || !R_k || !R_l || !Ii || !Ij || !Ik || !Il) return fail(FP4_ERR_ARG, "null argument");
+  if (ntri < 1) return fail(FP4_ERR_ARG, "Ntri must be positive");
+  HIPCHECK(hipSetDevice(c->device));
+  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
+  const int L = std::lcm(std::lcm(n1, n2), n3);
+  const int ng[3] = {n1, n2, n3};
+  c->h_clusters.assign(ntri, ClusterRec{});
+  for (int t = 0; t < ntri; t++) {
+    ClusterRec &r = c->h_clusters[t];
+    const int32_t *ix[4] = {Ii, Ij, Ik, Il};
+    for (int a = 0; a < 4; a++) {
+      if (ix[a][t] < 1 || ix[a][t] > c->natoms) return fail(FP4_ERR_ARG, "Index out of range at cluster %d", t + 1);
+      r.atom[a] = ix[a][t] - 1;
+    }
+    const double *Rs[3] = {R_j + 3 * (size_t)t, R_k + 3 * (size_t)t, R_l + 3 * (size_t)t};
+    int *outs[3] = {r.pj, r.pk, r.pl};
+    for (int w = 0; w < 3; w++) {
+      for (int nu = 0; nu < 3; nu++) {
+        // b_nu . R = 2 pi m_nu for a lattice vector R
+        const double *bv = c->rlattvec + 3 * nu;
+        const double x = (bv[0] * Rs[w][0] + bv[1] * Rs[w][1] + bv[2] * Rs[w][2]) / (2.0 * M_PI);
+        const double xr = std::nearbyint(x);
+        if (std::fabs(x - xr) > 1e-6) return fail(FP4_ERR_ARG, "R of cluster %d is not a lattice vector", t + 1);
+        outs[w][nu] = (int)xr * (L / ng[nu]);
+      }
+    }
+    double mp = c->masses[c->types[r.atom[0]] - 1] * c->masses[c->types[r.atom[1]] - 1];
+    mp = mp * c->masses[c->types[r.atom[2]] - 1];
+    mp = mp * c->masses[c->types[r.atom[3]] - 1];
+    r.minv = 1.0 / std::sqrt(mp);
+  }
+  std::vector<double> roots(2 * L);
+  for (int k = 0; k < L; k++) {
+    const double x = 2.0 * M_PI * (double)k / (double)L;
+    roots[2 * k] = std::cos(x);
+    roots[2 * k + 1] = std::sin(x);
+  }
+  int rc;
+  if ((rc = c->clusters.ensure(sizeof(ClusterRec) * ntri)) || (rc = c->phi.ensure(sizeof(double) * 81 * ntri)) ||
+      (rc = c->roots.ensure(sizeof(double) * 2 * L)))
+    return rc;
+  c->h_phi.assign(Phi, Phi + 81 * (size_t)ntri);
+  HIPCHECK(hipMemcpy(c->clusters.p, c->h_clusters.data(), sizeof(ClusterRec) * ntri, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->phi.p, Phi, sizeof(double) * 81 * ntri, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->roots.p, roots.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
+  c->ntri = ntri;
+  c->L = L;
+  c->have_ifc = true;
+  int drc = dense_prepare(c->h_clusters, c->h_phi, c->natoms, c->nb);
+  (void)drc;
+  return FP4_OK;
+}
+
+static void all_modes(const fp4_ctx *c, Plan &pl) {
+  pl.modes.resize((size_t)c->nb * c->nlist);
+  std::iota(pl.modes.begin(), pl.modes.end(), 1);
+  pl.ii_lo0 = 0;
+  pl.nii = c->nptk;
+}
+
+int fp4_np_4ph(fp4_ctx *c, int64_t *N_pp, double *P_pp, int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!N_pp || !P_pp || !N_pm || !P_pm || !N_mm || !P_mm) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  all_modes(c, pl);
+  if ((rc = count_pass(c, pl, false))) return rc;
+  const int nm = (int)pl.modes.size();
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  double *P[3] = {P_pp, P_pm, P_mm};
+  for (int ch = 0; ch < 3; ch++)
+    for (int m = 0; m < nm; m++) {
+      N[ch][m] = pl.segN[ch * nm + m];
+      P[ch][m] = pl.segP[ch * nm + m];
+    }
+  return FP4_OK;
+}
+
+static int verify_counts(const Plan &pl, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm) {
+  const int nm = (int)pl.modes.size();
+  const int64_t *N[3] = {N_pp, N_pm, N_mm};
+  for (int ch = 0; ch < 3; ch++)
+    for (int m = 0; m < nm; m++)
+      if (N[ch][m] != pl.segN[ch * nm + m])
+        return fail(FP4_ERR_OVERFLOW, "count mismatch: channel %d mode %d has %lld processes, caller passed %lld", ch,
+                    m + 1, (long long)pl.segN[ch * nm + m], (long long)N[ch][m]);
+  return 0;
+}
+
+int fp4_ind_only_4ph(fp4_ctx *c, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm, int64_t *a2,
+                     int64_t *a3, int64_t *a4, int64_t *b2, int64_t *b3, int64_t *b4, int64_t *c2, int64_t *c3,
+                     int64_t *c4) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!N_pp || !N_pm || !N_mm) return fail(FP4_ERR_ARG, "null counts");
+  Plan pl;
+  all_modes(c, pl);
+  if ((rc = count_pass(c, pl, true))) return rc;
+  if ((rc = verify_counts(pl, N_pp, N_pm, N_mm))) return rc;
+  int64_t *outs[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
+  const long per_ch = pl.items / 3;
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit_batch(c, pl, b, true))) return rc;
+    // copy each channel's part of the batch to its host lists
+    for (int ch = 0; ch < 3; ch++) {
+      const long lo = std::max(b.item_lo, ch * per_ch), hi = std::min(b.item_hi, (ch + 1) * per_ch);
+      if (hi <= lo) continue;
+      const long long p0 = pl.off[lo], p1 = pl.off[hi], base = pl.off[ch * per_ch];
+      if (p1 <= p0) continue;
+      for (int t = 0; t < 3; t++) {
+        if (!outs[ch][t]) return fail(FP4_ERR_ARG, "null list output");
+        HIPCHECK(hipMemcpyAsync(outs[ch][t] + (p0 - base), c->lists[t].as<long long>() + (p0 - b.p0),
+                                sizeof(long long) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+      }
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  return FP4_OK;
+}
+
+int fp4_rta_4ph_modes(fp4_ctx *c, int32_t precision, int32_t nmodes, const int32_t *modes, int32_t ii_lo,
+                      int32_t ii_hi, double *rate_4ph, double *r_pp, double *r_pm, double *r_mm, double *w_pp,
+                      double *w_pm, double *w_mm, int64_t *nproc) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  if (!rate_4ph || !r_pp || !r_pm || !r_mm || !w_pp || !w_pm || !w_mm) return fail(FP4_ERR_ARG, "null output");
+  if (nmodes < 1 || !modes) return fail(FP4_ERR_ARG, "empty mode subset");
+  if (ii_lo < 1 || ii_hi > c->nptk || ii_hi < ii_lo) return fail(FP4_ERR_ARG, "bad ii window %d..%d", ii_lo, ii_hi);
+  Plan pl;
+  pl.modes.assign(modes, modes + nmodes);
+  for (size_t t = 0; t < pl.modes.size(); t++) {
+    if (pl.modes[t] < 1 || pl.modes[t] > c->nb * c->nlist) return fail(FP4_ERR_ARG, "mode %d out of range", pl.modes[t]);
+    if (t && pl.modes[t] <= pl.modes[t - 1]) return fail(FP4_ERR_ARG, "modes must be strictly increasing");
+  }
+  pl.ii_lo0 = ii_lo - 1;
+  pl.nii = ii_hi - ii_lo + 1;
+  const int nseg = 3 * nmodes;
+  std::vector<double> g(nseg, 0.0), w(nseg, 0.0);
+  if (precision == FP4_PREC_FP64) {
+    if ((rc = count_pass(c, pl, true))) return rc;
+    for (const Batch &b : plan_batches(pl, c->list_cap)) {
+      if ((rc = emit_batch(c, pl, b, false))) return rc;
+      if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
+    }
+  } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3) {
+    if ((rc = count_pass(c, pl, false))) return rc;
+    DenseRun dr;
+    dr.m = mode_view(c);
+    dr.T = c->T;
+    dr.onlyharmonic = c->onlyharmonic;
+    dr.natoms = c->natoms;
+    dr.precision = precision;
+    dr.stream = c->stream;
+    dr.scratch = &c->dense_scratch.p;
+    dr.scratch_bytes = &c->dense_scratch.bytes;
+    dr.tuning = &c->dense_tuning;
+    double ms = 0.0;
+    rc = dense_rates(dr, pl.modes, pl.ii_lo0, pl.nii, g, w, &ms);
+    if (rc) return fail(rc, "dense path: %s", dense_error());
+    c->ks_ms[KS_DENSE] += ms;
+    c->ks_n[KS_DENSE] += 1;
+  } else {
+    return fail(FP4_ERR_ARG, "unknown precision %d", precision);
+  }
+  write_rates(c, pl, g, w, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm, true);
+  if (nproc) *nproc = std::accumulate(pl.segN.begin(), pl.segN.end(), 0LL);
+  return FP4_OK;
+}
+
+int fp4_rta_4ph(fp4_ctx *c, int32_t precision, double *rate_4ph, double *r_pp, double *r_pm, double *r_mm,
+                double *w_pp, double *w_pm, double *w_mm) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  std::vector<int> modes((size_t)c->nb * c->nlist);
+  std::iota(modes.begin(), modes.end(), 1);
+  const size_t nm = modes.size();
+  double *outs[7] = {rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  return fp4_rta_4ph_modes(c, precision, (int)nm, modes.data(), 1, c->nptk, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm,
+                           w_mm, nullptr);
+}
+
+int fp4_rta_4ph_using_ind(fp4_ctx *c, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
+                          const int64_t *a2, const int64_t *a3, const int64_t *a4, const int64_t *b2,
+                          const int64_t *b3, const int64_t *b4, const int64_t *c2, const int64_t *c3,
+                          const int64_t *c4, double *rate_4ph, double *r_pp, double *r_pm, double *r_mm,
+                          double *w_pp, double *w_pm, double *w_mm) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  const int nm = c->nb * c->nlist;
+  const int64_t *N[3] = {N_pp, N_pm, N_mm};
+  const int64_t *L3[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
+  double *outs[7] = {rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  // Items are (channel, mode) with one "ii" each; offsets from the given counts.
+  Plan pl;
+  pl.modes.resize(nm);
+  std::iota(pl.modes.begin(), pl.modes.end(), 1);
+  pl.nii = 1;
+  pl.items = 3L * nm;
+  pl.off.assign(pl.items + 1, 0);
+  for (int ch = 0; ch < 3; ch++)
+    for (int m = 0; m < nm; m++) {
+      if (!N[ch] || N[ch][m] < 0) return fail(FP4_ERR_ARG, "bad counts");
+      pl.off[(long)ch * nm + m + 1] = pl.off[(long)ch * nm + m] + N[ch][m];
+    }
+  const long long tot[3] = {pl.off[nm] - pl.off[0], pl.off[2L * nm] - pl.off[nm], pl.off[3L * nm] - pl.off[2L * nm]};
+  for (int ch = 0; ch < 3; ch++)
+    for (int t = 0; t < 3; t++)
+      if (tot[ch] > 0 && !L3[ch][t]) return fail(FP4_ERR_ARG, "null list");
+  // range check of the caller's entries happens on the host before upload
+  const long long maxv = (long long)c->nptk * c->nb;
+  std::vector<double> g(3 * nm, 0.0), w(3 * nm, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    const long long n = b.p1 - b.p0;
+    for (int t = 0; t < 3; t++)
+      if ((rc = c->lists[t].ensure(8 * (size_t)n))) return rc;
+    for (long it = b.item_lo; it < b.item_hi; it++) {
+      const int ch = (int)(it / nm);
+      const long long p0 = pl.off[it], p1 = pl.off[it + 1];
+      if (p1 <= p0) continue;
+      const long long base = pl.off[(long)ch * nm];
+      for (int t = 0; t < 3; t++) {
+        const int64_t *src = L3[ch][t] + (p0 - base);
+        for (long long x = 0; x < p1 - p0; x++)
+          if (src[x] < 1 || src[x] > maxv) return fail(FP4_ERR_ARG, "list entry out of range");
+        HIPCHECK(hipMemcpyAsync(c->lists[t].as<long long>() + (p0 - b.p0), src, sizeof(long long) * (p1 - p0),
+                                hipMemcpyHostToDevice, c->stream));
+      }
+    }
+    if ((rc = rate_batch(c, pl, b, true, false, g, w))) return rc;
+  }
+  write_rates(c, pl, g, w, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm, true);
+  return FP4_OK;
+}
+
+int fp4_ind_4ph(fp4_ctx *c, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm, int64_t *a2, int64_t *a3,
+                int64_t *a4, int64_t *b2, int64_t *b3, int64_t *b4, int64_t *c2, int64_t *c3, int64_t *c4,
+                double *G_pp, double *G_pm, double *G_mm, double *rate_4ph, double *r_pp, double *r_pm,
+                double *r_mm, double *w_pp, double *w_pm, double *w_mm) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  Plan pl;
+  all_modes(c, pl);
+  const int nm = (int)pl.modes.size();
+  double *outs[7] = {rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  if ((rc = count_pass(c, pl, true))) return rc;
+  if ((rc = verify_counts(pl, N_pp, N_pm, N_mm))) return rc;
+  int64_t *outl[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
+  double *outg[3] = {G_pp, G_pm, G_mm};
+  const long per_ch = pl.items / 3;
+  std::vector<double> g(3 * nm, 0.0), w(3 * nm, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit_batch(c, pl, b, true))) return rc;
+    if ((rc = rate_batch(c, pl, b, true, true, g, w))) return rc;
+    for (int ch = 0; ch < 3; ch++) {
+      const long lo = std::max(b.item_lo, ch * per_ch), hi = std::min(b.item_hi, (ch + 1) * per_ch);
+      if (hi <= lo) continue;
+      const long long p0 = pl.off[lo], p1 = pl.off[hi], base = pl.off[ch * per_ch];
+      if (p1 <= p0) continue;
+      for (int t = 0; t < 3; t++) {
+        if (!outl[ch][t]) return fail(FP4_ERR_ARG, "null list output");
+        HIPCHECK(hipMemcpyAsync(outl[ch][t] + (p0 - base), c->lists[t].as<long long>() + (p0 - b.p0),
+                                sizeof(long long) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+      }
+      if (!outg[ch]) return fail(FP4_ERR_ARG, "null Gamma output");
+      HIPCHECK(hipMemcpyAsync(outg[ch] + (p0 - base), c->proc_g.as<double>() + (p0 - b.p0),
+                              sizeof(double) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  // per-process Gamma are already in THz; Ind_driver_4ph sums them per mode
+  write_rates(c, pl, g, w, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm, false);
+  return FP4_OK;
+}
+
+int fp4_vp_4ph(fp4_ctx *c, int32_t channel, int64_t n, const int32_t *idx, double *out) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  if (channel < 0 || channel > 2) return fail(FP4_ERR_ARG, "channel must be 0, 1 or 2");
+  if (n < 0 || (n > 0 && (!idx || !out))) return fail(FP4_ERR_ARG, "bad arguments");
+  if (n == 0) return FP4_OK;
+  for (int64_t t = 0; t < n; t++) {
+    const int32_t *r = idx + 8 * t;
+    for (int a = 0; a < 4; a++)
+      if (r[a] < 1 || r[a] > c->nb) return fail(FP4_ERR_ARG, "branch index out of range in row %lld", (long long)t);
+    for (int a = 4; a < 8; a++)
+      if (r[a] < 1 || r[a] > c->nptk) return fail(FP4_ERR_ARG, "q index out of range in row %lld", (long long)t);
+  }
+  DevBuf di, dout;
+  if ((rc = di.ensure(sizeof(int) * 8 * n)) || (rc = dout.ensure(sizeof(double) * 2 * n))) {
+    di.release();
+    return rc;
+  }
+  VpArgs a{};
+  a.m = mode_view(c);
+  a.f = ifc_view(c);
+  a.ch = channel;
+  a.n = n;
+  a.idx = di.as<int>();
+  a.out = dout.as<double2>();
+  hipError_t e = hipMemcpyAsync(di.p, idx, sizeof(int) * 8 * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_vp64(a, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  di.release();
+  dout.release();
+  if (e != hipSuccess) return fail(FP4_ERR_DEVICE, "fp4_vp_4ph: %s", hipGetErrorString(e));
+  return FP4_OK;
+}
+
+int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t cap, int64_t *N_pp,
+                           int64_t *N_pm, int64_t *N_mm, int64_t *a2, int64_t *a3, int64_t *a4, int64_t *b2,
+                           int64_t *b3, int64_t *b4, int64_t *c2, int64_t *c3, int64_t *c4) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (nmodes < 1 || !modes || !N_pp || !N_pm || !N_mm) return fail(FP4_ERR_ARG, "bad arguments");
+  Plan pl;
+  pl.modes.assign(modes, modes + nmodes);
+  for (size_t t = 0; t < pl.modes.size(); t++) {
+    if (pl.modes[t] < 1 || pl.modes[t] > c->nb * c->nlist) return fail(FP4_ERR_ARG, "mode %d out of range", pl.modes[t]);
+    if (t && pl.modes[t] <= pl.modes[t - 1]) return fail(FP4_ERR_ARG, "modes must be strictly increasing");
+  }
+  pl.ii_lo0 = 0;
+  pl.nii = c->nptk;
+  if ((rc = count_pass(c, pl, true))) return rc;
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  int64_t *outs[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
+  const long per_ch = pl.items / 3;
+  for (int ch = 0; ch < 3; ch++) {
+    for (int m = 0; m < nmodes; m++) N[ch][m] = pl.segN[ch * nmodes + m];
+    if (pl.off[(ch + 1) * per_ch] - pl.off[ch * per_ch] > cap)
+      return fail(FP4_ERR_OVERFLOW, "channel %d needs %lld entries, cap is %lld", ch,
+                  (long long)(pl.off[(ch + 1) * per_ch] - pl.off[ch * per_ch]), (long long)cap);
+  }
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit_batch(c, pl, b, true))) return rc;
+    for (int ch = 0; ch < 3; ch++) {
+      const long lo = std::max(b.item_lo, ch * per_ch), hi = std::min(b.item_hi, (ch + 1) * per_ch);
+      if (hi <= lo) continue;
+      const long long p0 = pl.off[lo], p1 = pl.off[hi], base = pl.off[ch * per_ch];
+      if (p1 <= p0) continue;
+      for (int t = 0; t < 3; t++) {
+        if (!outs[ch][t]) return fail(FP4_ERR_ARG, "null list output");
+        HIPCHECK(hipMemcpyAsync(outs[ch][t] + (p0 - base), c->lists[t].as<long long>() + (p0 - b.p0),
+                                sizeof(long long) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+      }
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  return FP4_OK;
+}
+
+// ---- diagnostics (no counterpart in the reference) -----------------------
+int fp4_ctx_kernel_stats(fp4_ctx *c, int32_t kind, double *total_ms, int64_t *launches) {
+  if (!c || kind < 0 || kind >= KS_N) return fail(FP4_ERR_ARG, "bad arguments");
+  if (total_ms) *total_ms = c->ks_ms[kind];
+  if (launches) *launches = c->ks_n[kind];
+  return FP4_OK;
+}
+
+int fp4_ctx_reset_stats(fp4_ctx *c) {
+  if (!c) return fail(FP4_ERR_ARG, "null context");
+  for (int k = 0; k < KS_N; k++) {
+    c->ks_ms[k] = 0;
+    c->ks_n[k] = 0;
+  }
+  return FP4_OK;
+}
+
+int fp4_ctx_set_batch_capacity(fp4_ctx *c, int64_t processes) {
+  if (!c || processes < 1024) return fail(FP4_ERR_ARG, "capacity must be at least 1024 processes");
+  c->list_cap = processes;
+  return FP4_OK;
+}
+
+}  // extern "C"

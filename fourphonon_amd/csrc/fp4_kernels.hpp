@@ -1,0 +1,73 @@
+// Launch interfaces between the host runtime (fp4_api.hip) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fp4_device.hpp"
+
+namespace fp4 {
+
+// Items are (channel, mode-in-subset, ii) with ii fastest; item_lo..item_hi
+// is the slice this launch covers.
+struct PrefilterArgs {
+  ModeView m;
+  const int *modes;      // mm (1-based) of the subset, ascending
+  int nsub;              // number of modes in the subset
+  int nii;               // ii window length
+  int ii_lo0;            // first ii (0-based)
+  long item_lo, item_hi;
+  // count pass
+  int *cnt;              // [items]
+  double *psum;          // [items]
+  // emit pass
+  const long long *offsets;  // exclusive prefix of cnt over all items
+  long long out_base;        // offsets[item_lo]
+  int *out32[3];
+  long long *out64[3];
+};
+
+int launch_prefilter(const PrefilterArgs &a, int mode, hipStream_t st);
+void launch_reduce_counts(const int *cnt, const double *psum, int nseg, int nii, long long *N, double *P,
+                          hipStream_t st);
+void launch_widen(const int *cnt, long long *out, long n, hipStream_t st);
+
+// A block of at most 256 consecutive processes of one (channel, mode).
+struct BlockDesc {
+  long long start;   // first process, index into the batch's list arrays
+  int count;
+  int ch;
+  int mm;            // 1-based mode
+  int seg;           // output slot of the (channel, mode) segment
+};
+
+struct RateArgs {
+  ModeView m;
+  IfcView f;
+  const int *lst[3];         // batch lists (reference encoding), int32
+  const long long *lst64[3]; // or int64 (caller-provided lists); one of the two is set
+  const BlockDesc *blocks;
+  int nblocks;
+  double T;
+  int onlyharmonic;
+  int store_gamma;           // Ind_driver_4ph: write per-process Gamma
+  double *proc_gamma;        // [batch processes], when store_gamma
+  double *blk_gamma;         // [nblocks]
+  double *blk_wp4;           // [nblocks]
+};
+
+int launch_rate64(const RateArgs &a, hipStream_t st);
+
+// Per-segment sums of block partials, in block order.
+void launch_seg_reduce(const double *blk_gamma, const double *blk_wp4, const int *seg_first, int nseg,
+                       double *seg_gamma, double *seg_wp4, hipStream_t st);
+
+struct VpArgs {
+  ModeView m;
+  IfcView f;
+  int ch;
+  long n;
+  const int *idx;   // n rows of (i, j, k, l, q, q', q'', q''') 1-based
+  double2 *out;
+};
+int launch_vp64(const VpArgs &a, hipStream_t st);
+
+}  // namespace fp4

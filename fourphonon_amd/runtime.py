@@ -1,0 +1,228 @@
+"""Python mirror of the reference's four-phonon call surface over the fp4 C ABI.
+
+The names, argument meaning and output arrays follow ``module processes``
+(``Src/processes.f90``): ``NP_driver_4ph``, ``Ind_only_driver_4ph``,
+``RTA_driver_4ph``, ``RTA_driver_4ph_GPU_using_Ind``, ``Ind_driver_4ph`` and
+``Vp_pp``/``Vp_pm``/``Vp_mm``.  The module globals the reference reads
+implicitly (nptk, Nbands, ngrid, rlattvec, masses, types, T, scalebroad,
+omega_max, onlyharmonic) are given once to :class:`Context`.
+
+Everything here runs on the GPU through ``libfp4.so``; there is no CPU
+fallback.  Importing works without a GPU, but creating a context raises if
+the library or a HIP device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfp4.so")
+
+PREC_FP64 = 0
+PREC_FP64_DENSE = 1
+PREC_FP32 = 2
+PREC_BF16X3 = 3
+
+ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR_OVERFLOW", 5: "FP4_ERR_STATE",
+          6: "FP4_ERR_UNSUPPORTED"}
+
+# every symbol include/fp4.h declares
+EXPORTS = [
+    "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
+    "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
+    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_ind_only_4ph_modes", "fp4_ctx_kernel_stats",
+    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity",
+]
+
+
+class FP4Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("ngrid", C.c_int32 * 3), ("nbands", C.c_int32), ("natoms", C.c_int32), ("nelements", C.c_int32),
+                ("rlattvec", C.c_double * 9), ("masses", C.c_void_p), ("types", C.c_void_p), ("T", C.c_double),
+                ("scalebroad", C.c_double), ("omega_max", C.c_double), ("onlyharmonic", C.c_int32),
+                ("device", C.c_int32)]
+
+
+_lib = None
+
+
+def load():
+    """Loads libfp4.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FP4Error(2, f"{LIB_PATH} is missing: build it with fourphonon_amd/build.py")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.fp4_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise FP4Error(rc, load().fp4_last_error().decode())
+
+
+class Context:
+    """One device context holding a crystal's modes and fourth-order IFCs."""
+
+    def __init__(self, case, device: int = 0):
+        L = load()
+        st, m, f = case.structure, case.modes, case.ifc4
+        self.case = case
+        self.nptk, self.nb, self.nlist = m.nptk, m.nbands, m.Nlist
+        self._masses = np.ascontiguousarray(st.masses, np.float64)
+        self._types = np.ascontiguousarray(st.types, np.int32)
+        cfg = Config()
+        cfg.ngrid[:] = [int(x) for x in m.ngrid]
+        cfg.nbands, cfg.natoms, cfg.nelements = m.nbands, st.natoms, len(st.masses)
+        cfg.rlattvec[:] = list(np.asarray(st.rlattvec).ravel(order="F"))
+        cfg.masses, cfg.types = _p(self._masses), _p(self._types)
+        cfg.T, cfg.scalebroad, cfg.omega_max = case.T, case.scalebroad, case.omega_max
+        cfg.onlyharmonic = int(case.onlyharmonic)
+        cfg.device = device
+        h = C.c_void_p()
+        _check(L.fp4_ctx_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        _check(L.fp4_ctx_set_modes(h, _p(np.asfortranarray(m.energy, np.float64)),
+                                   _p(np.asfortranarray(m.velocity, np.float64)),
+                                   _p(np.asfortranarray(m.eigenvect, np.complex128)),
+                                   _p(np.asfortranarray(m.IJK, np.int32)), C.c_int32(m.Nlist),
+                                   _p(np.ascontiguousarray(m.List, np.int32))))
+        _check(L.fp4_ctx_set_ifc4(h, C.c_int32(f.Ntri), _p(np.asfortranarray(f.Phi, np.float64)),
+                                  _p(np.asfortranarray(f.R_j, np.float64)), _p(np.asfortranarray(f.R_k, np.float64)),
+                                  _p(np.asfortranarray(f.R_l, np.float64)),
+                                  *[_p(np.ascontiguousarray(x, np.int32))
+                                    for x in (f.Index_i, f.Index_j, f.Index_k, f.Index_l)]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fp4_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nmodes(self):
+        return self.nb * self.nlist
+
+    def set_temperature(self, T):
+        _check(load().fp4_ctx_set_temperature(self.h, C.c_double(T)))
+
+    # ------------------------------------------------------------ drivers
+    def NP_driver_4ph(self):
+        """-> (N_pp, P_pp, N_pm, P_pm, N_mm, P_mm); N int64 (Nlist*Nb), P (Nb,Nlist)."""
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int64) for _ in range(3)]
+        P = [np.zeros((self.nb, self.nlist), order="F") for _ in range(3)]
+        _check(load().fp4_np_4ph(self.h, _p(N[0]), _p(P[0]), _p(N[1]), _p(P[1]), _p(N[2]), _p(P[2])))
+        return N[0], P[0], N[1], P[1], N[2], P[2]
+
+    def Ind_only_driver_4ph(self, N_pp, N_pm, N_mm):
+        """-> [[Indof2nd, Indof3rd, Indof4th] for ++, +-, --] (int64, reference encoding)."""
+        Ns = [np.ascontiguousarray(n, np.int64) for n in (N_pp, N_pm, N_mm)]
+        outs = [[np.zeros(max(int(n.sum()), 1), np.int64) for _ in range(3)] for n in Ns]
+        _check(load().fp4_ind_only_4ph(self.h, *[_p(n) for n in Ns], *[_p(o) for ch in outs for o in ch]))
+        return [[o[:int(n.sum())] for o in ch] for ch, n in zip(outs, Ns)]
+
+    def Ind_only_driver_4ph_modes(self, modes, cap=None):
+        """Lists of a mode subset -> (N_pp, N_pm, N_mm, lists)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        if cap is None:
+            N = self.NP_driver_4ph()
+            cap = max(int(N[2 * ch][modes - 1].sum()) for ch in range(3))
+        cap = max(int(cap), 1)
+        Ns = [np.zeros(len(modes), np.int64) for _ in range(3)]
+        outs = [[np.zeros(cap, np.int64) for _ in range(3)] for _ in range(3)]
+        _check(load().fp4_ind_only_4ph_modes(self.h, C.c_int32(len(modes)), _p(modes), C.c_int64(cap),
+                                             *[_p(n) for n in Ns], *[_p(o) for ch in outs for o in ch]))
+        return Ns[0], Ns[1], Ns[2], [[o[:int(n.sum())] for o in ch] for ch, n in zip(outs, Ns)]
+
+    def _rate_outputs(self):
+        return [np.zeros((self.nb, self.nlist), order="F") for _ in range(7)]
+
+    @staticmethod
+    def _named(out):
+        return dict(zip(["rate_scatt_4ph", "rate_scatt_plusplus", "rate_scatt_plusminus", "rate_scatt_minusminus",
+                         "WP4_plusplus", "WP4_plusminus", "WP4_minusminus"], out))
+
+    def RTA_driver_4ph(self, precision=PREC_FP64):
+        out = self._rate_outputs()
+        _check(load().fp4_rta_4ph(self.h, C.c_int32(precision), *[_p(o) for o in out]))
+        return self._named(out)
+
+    def RTA_driver_4ph_modes(self, modes, precision=PREC_FP64, ii_lo=1, ii_hi=None):
+        """RTA_driver_4ph over a mode subset (mm, 1-based) and a q' window; returns (dict, nproc)."""
+        out = self._rate_outputs()
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        nproc = C.c_int64()
+        _check(load().fp4_rta_4ph_modes(self.h, C.c_int32(precision), C.c_int32(len(modes)), _p(modes),
+                                        C.c_int32(ii_lo), C.c_int32(self.nptk if ii_hi is None else ii_hi),
+                                        *[_p(o) for o in out], C.byref(nproc)))
+        return self._named(out), nproc.value
+
+    def RTA_driver_4ph_GPU_using_Ind(self, N_pp, N_pm, N_mm, lists):
+        Ns = [np.ascontiguousarray(n, np.int64) for n in (N_pp, N_pm, N_mm)]
+        L3 = [[np.ascontiguousarray(x, np.int64) if len(x) else np.zeros(1, np.int64) for x in ch] for ch in lists]
+        out = self._rate_outputs()
+        _check(load().fp4_rta_4ph_using_ind(self.h, *[_p(n) for n in Ns], *[_p(x) for ch in L3 for x in ch],
+                                            *[_p(o) for o in out]))
+        return self._named(out)
+
+    def Ind_driver_4ph(self, N_pp, N_pm, N_mm):
+        """-> (lists, [Gamma_pp, Gamma_pm, Gamma_mm], rates dict)."""
+        Ns = [np.ascontiguousarray(n, np.int64) for n in (N_pp, N_pm, N_mm)]
+        tot = [int(n.sum()) for n in Ns]
+        lists = [[np.zeros(max(t, 1), np.int64) for _ in range(3)] for t in tot]
+        gam = [np.zeros(max(t, 1)) for t in tot]
+        out = self._rate_outputs()
+        _check(load().fp4_ind_4ph(self.h, *[_p(n) for n in Ns], *[_p(x) for ch in lists for x in ch],
+                                  *[_p(g) for g in gam], *[_p(o) for o in out]))
+        return ([[x[:t] for x in ch] for ch, t in zip(lists, tot)], [g[:t] for g, t in zip(gam, tot)],
+                self._named(out))
+
+    def Vp(self, channel, rows):
+        """Vp_pp (0) / Vp_pm (1) / Vp_mm (2) for rows of (i,j,k,l,q,q',q'',q''') (1-based)."""
+        rows = np.ascontiguousarray(np.atleast_2d(rows), np.int32)
+        out = np.zeros(len(rows), np.complex128)
+        _check(load().fp4_vp_4ph(self.h, C.c_int32(channel), C.c_int64(len(rows)), _p(rows), _p(out)))
+        return out
+
+    def Vp_pp(self, rows):
+        return self.Vp(0, rows)
+
+    def Vp_pm(self, rows):
+        return self.Vp(1, rows)
+
+    def Vp_mm(self, rows):
+        return self.Vp(2, rows)
+
+    # ------------------------------------------------------------ diagnostics
+    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE = 0, 1, 2, 3
+
+    def kernel_stats(self, kind):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(load().fp4_ctx_kernel_stats(self.h, C.c_int32(kind), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_stats(self):
+        _check(load().fp4_ctx_reset_stats(self.h))
+
+    def set_batch_capacity(self, processes):
+        _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))

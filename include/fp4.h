@@ -1,0 +1,162 @@
+/*
+ * fp4 -- MI355X (gfx950) implementation of FourPhonon's four-phonon
+ * scattering hot path, exposed as a plain C ABI.
+ *
+ * Every entry point replaces one procedure of the reference's
+ * `module processes` (Src/processes.f90) or `module iterations`
+ * (Src/iterations.f90); the comment above each declaration names it.
+ * Arrays are the reference's own Fortran arrays: column-major, 1-based
+ * indices stored in the data, complex(8) as interleaved (re, im) doubles.
+ * The caller owns every host array; the library keeps no host pointer after
+ * a call returns.  Device copies of omega, v, e and the IFCs live in the
+ * context and are reused across calls and temperatures.
+ *
+ * The module globals the reference reads implicitly (nptk, Nbands, ngrid,
+ * rlattvec, masses, types, T, scalebroad, omega_max, onlyharmonic;
+ * Src/config.f90:9-51) are passed once in fp4_config.
+ *
+ * Calls are synchronous, not re-entrant per context, and made from one host
+ * thread.  All functions return FP4_OK (0) or an FP4_ERR_* code; the text of
+ * the last error on the calling thread is available from fp4_last_error().
+ */
+#ifndef FP4_H
+#define FP4_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FP4_OK = 0,
+  FP4_ERR_ARG = 1,         /* bad argument or inconsistent sizes */
+  FP4_ERR_DEVICE = 2,      /* HIP runtime error or no device */
+  FP4_ERR_NOMEM = 3,       /* device allocation failed */
+  FP4_ERR_OVERFLOW = 4,    /* a list longer than the count the caller passed */
+  FP4_ERR_STATE = 5,       /* modes or IFCs not set yet */
+  FP4_ERR_UNSUPPORTED = 6  /* e.g. Nbands outside 3..24, sampling requested */
+};
+
+/* Arithmetic used for the matrix elements of fp4_rta_4ph. */
+enum {
+  FP4_PREC_FP64 = 0,       /* per-process FP64, reference operation order */
+  FP4_PREC_FP64_DENSE = 1, /* dense fold + contraction, FP64 */
+  FP4_PREC_FP32 = 2,       /* dense fold + contraction on FP32 MFMA */
+  FP4_PREC_BF16X3 = 3      /* dense, BF16 hi/lo split (3 products) on MFMA, FP32 accumulate */
+};
+
+typedef struct fp4_ctx fp4_ctx;
+
+typedef struct fp4_config {
+  int32_t ngrid[3];
+  int32_t nbands;          /* 3*natoms */
+  int32_t natoms;
+  int32_t nelements;
+  double rlattvec[9];      /* rlattvec(3,3), nm^-1, column b_i = rlattvec(:,i) */
+  const double *masses;    /* masses(nelements), amu */
+  const int32_t *types;    /* types(natoms), 1-based */
+  double T;                /* K */
+  double scalebroad;
+  double omega_max;        /* rad/ps */
+  int32_t onlyharmonic;    /* logical */
+  int32_t device;          /* HIP device ordinal */
+} fp4_config;
+
+const char *fp4_last_error(void);
+
+int fp4_ctx_create(const fp4_config *cfg, fp4_ctx **out);
+int fp4_ctx_destroy(fp4_ctx *ctx);
+/* T and scalebroad may change between calls (temperature loop of
+   Src/ShengBTE.f90:787-1469) without re-uploading anything. */
+int fp4_ctx_set_temperature(fp4_ctx *ctx, double T);
+
+/* energy(nptk,Nb), velocity(nptk,Nb,3), eigenvect(nptk,Nb,Nb) complex,
+   IJK(3,nptk), List(Nlist) -- the arrays of Src/ShengBTE.f90:142-149. */
+int fp4_ctx_set_modes(fp4_ctx *ctx, const double *energy, const double *velocity, const double *eigenvect,
+                      const int32_t *IJK, int32_t nlist, const int32_t *List);
+
+/* Phi(3,3,3,3,Ntri), R_j/R_k/R_l(3,Ntri) in nm on lattice vectors,
+   Index_i..l(Ntri) -- the output of read4fc, Src/input.f90:141-182. */
+int fp4_ctx_set_ifc4(fp4_ctx *ctx, int32_t ntri, const double *Phi, const double *R_j, const double *R_k,
+                     const double *R_l, const int32_t *Index_i, const int32_t *Index_j, const int32_t *Index_k,
+                     const int32_t *Index_l);
+
+/* NP_driver_4ph, Src/processes.f90:2749-2847.  N_*(Nlist*Nb) int64 counts,
+   P_*(Nb,Nlist) phase space. */
+int fp4_np_4ph(fp4_ctx *ctx, int64_t *N_pp, double *P_pp, int64_t *N_pm, double *P_pm, int64_t *N_mm,
+               double *P_mm);
+
+/* Ind_only_driver_4ph, Src/processes.f90:3080-3246.  N_* are the counts
+   from fp4_np_4ph; the nine output lists have sum(N_*) entries each, in the
+   reference's enumeration order, encoded (q-1)*Nb+branch. */
+int fp4_ind_only_4ph(fp4_ctx *ctx, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
+                     int64_t *Indof2ndPhonon_pp, int64_t *Indof3rdPhonon_pp, int64_t *Indof4thPhonon_pp,
+                     int64_t *Indof2ndPhonon_pm, int64_t *Indof3rdPhonon_pm, int64_t *Indof4thPhonon_pm,
+                     int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm, int64_t *Indof4thPhonon_mm);
+
+/* RTA_driver_4ph, Src/processes.f90:5174-5311 (and, with precision
+   FP4_PREC_FP64, the same numbers RTA_driver_4ph_GPU_using_Ind of
+   Src/processes.f90:5315-5387 would give with the CPU path's constants).
+   Outputs are (Nb,Nlist) arrays, overwritten. */
+int fp4_rta_4ph(fp4_ctx *ctx, int32_t precision, double *rate_4ph, double *rate_pp, double *rate_pm,
+                double *rate_mm, double *WP4_pp, double *WP4_pm, double *WP4_mm);
+
+/* RTA_driver_4ph restricted to a subset of modes (mm, 1-based) and to the
+   second-phonon window ii_lo..ii_hi (1-based, inclusive; 1..nptk is the
+   full rate).  Entries of the outputs outside the subset are left as they
+   are.  *nproc receives the number of processes evaluated (may be NULL). */
+int fp4_rta_4ph_modes(fp4_ctx *ctx, int32_t precision, int32_t nmodes, const int32_t *modes, int32_t ii_lo,
+                      int32_t ii_hi, double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm,
+                      double *WP4_pp, double *WP4_pm, double *WP4_mm, int64_t *nproc);
+
+/* RTA_driver_4ph_GPU_using_Ind, Src/processes.f90:5315-5387: rates from
+   caller-provided lists (counts N_*, lists as from fp4_ind_only_4ph). */
+int fp4_rta_4ph_using_ind(fp4_ctx *ctx, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
+                          const int64_t *Indof2ndPhonon_pp, const int64_t *Indof3rdPhonon_pp,
+                          const int64_t *Indof4thPhonon_pp, const int64_t *Indof2ndPhonon_pm,
+                          const int64_t *Indof3rdPhonon_pm, const int64_t *Indof4thPhonon_pm,
+                          const int64_t *Indof2ndPhonon_mm, const int64_t *Indof3rdPhonon_mm,
+                          const int64_t *Indof4thPhonon_mm, double *rate_4ph, double *rate_pp, double *rate_pm,
+                          double *rate_mm, double *WP4_pp, double *WP4_pm, double *WP4_mm);
+
+/* Ind_driver_4ph, Src/processes.f90:3551-3822: lists plus the per-process
+   Gamma_* (same order as the lists), per-mode rates and WP4. */
+int fp4_ind_4ph(fp4_ctx *ctx, const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
+                int64_t *Indof2ndPhonon_pp, int64_t *Indof3rdPhonon_pp, int64_t *Indof4thPhonon_pp,
+                int64_t *Indof2ndPhonon_pm, int64_t *Indof3rdPhonon_pm, int64_t *Indof4thPhonon_pm,
+                int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm, int64_t *Indof4thPhonon_mm,
+                double *Gamma_pp, double *Gamma_pm, double *Gamma_mm, double *rate_4ph, double *rate_pp,
+                double *rate_pm, double *rate_mm, double *WP4_pp, double *WP4_pm, double *WP4_mm);
+
+/* Vp_pp / Vp_pm / Vp_mm, Src/processes.f90:142-330, for n processes at
+   once.  channel 0 = ++, 1 = +-, 2 = --.  idx holds n rows of
+   (i, j, k, l, q, q', q'', q''') (1-based branches and grid indices), out
+   receives n complex values. */
+int fp4_vp_4ph(fp4_ctx *ctx, int32_t channel, int64_t n, const int32_t *idx, double *out);
+
+/* Ind_only_driver_4ph restricted to a strictly increasing subset of modes
+   (mm, 1-based), for callers that split the mode range the way the
+   reference's MPI chunks do (Src/processes.f90:3160-3203).  N_* receive the
+   subset's counts (nmodes each); the lists receive the subset's entries,
+   modes in the given order, each list sized by the caller to cap entries
+   (FP4_ERR_OVERFLOW if a channel needs more). */
+int fp4_ind_only_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t cap, int64_t *N_pp,
+                           int64_t *N_pm, int64_t *N_mm, int64_t *Indof2ndPhonon_pp, int64_t *Indof3rdPhonon_pp,
+                           int64_t *Indof4thPhonon_pp, int64_t *Indof2ndPhonon_pm, int64_t *Indof3rdPhonon_pm,
+                           int64_t *Indof4thPhonon_pm, int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm,
+                           int64_t *Indof4thPhonon_mm);
+
+/* Diagnostics; no counterpart in the reference.  Accumulated device time of
+   each kernel family, measured with HIP events on the context's stream:
+   kind 0 = prefilter count pass, 1 = list emission, 2 = FP64 per-process
+   rate kernel, 3 = dense rate kernels. */
+int fp4_ctx_kernel_stats(fp4_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
+int fp4_ctx_reset_stats(fp4_ctx *ctx);
+/* Upper bound on the processes materialised per batch (default 2^26); no
+   counterpart in the reference, which materialises every list at once. */
+int fp4_ctx_set_batch_capacity(fp4_ctx *ctx, int64_t processes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

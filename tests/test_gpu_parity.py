@@ -1,0 +1,249 @@
+"""Parity of the HIP path against the CPU oracle and the golden fixtures.
+
+Bars: counts and lists bit-exact; FP64 rates, WP4, phase space and matrix
+elements within 1e-10 relative (the FP64 form keeps the reference's operation
+order; remaining differences come from summation order and from exact
+root-of-unity phases instead of cos/sin of a rounded dot product).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fourphonon_amd import crystal
+from fourphonon_amd import runtime as rt
+from tests.golden.listhash import list_hash
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RTOL = 1e-10
+
+
+def banner(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+
+
+def assert_close(got, want, rtol=RTOL, what=""):
+    got, want = np.asarray(got, float).ravel(), np.asarray(want, float).ravel()
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want)
+    bad = err > rtol * np.maximum(np.abs(want), 1e-3 * scale)
+    assert not bad.any(), f"{what}: max rel err {np.max(err / np.maximum(np.abs(want), 1e-300)):.3e}"
+
+
+@pytest.fixture(scope="module")
+def case4():
+    return crystal.config_case(1, ngrid=(4, 4, 4))
+
+
+@pytest.fixture(scope="module")
+def ctx4(case4):
+    banner("context diamond 4x4x4 shell 1")
+    c = rt.Context(case4)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def orc4(case4):
+    from oracle.oracle import Oracle
+    return Oracle(case4)
+
+
+@pytest.fixture(scope="module")
+def ref_counts4(orc4):
+    return orc4.np_driver()
+
+
+def test_counts_bitexact(ctx4, ref_counts4):
+    banner("NP_driver_4ph 4x4x4")
+    N = ctx4.NP_driver_4ph()
+    No, Po = ref_counts4
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch], No[ch]), f"channel {ch}"
+        assert_close(N[2 * ch + 1].ravel(order="F"), Po[ch], 1e-12, f"P channel {ch}")
+
+
+def test_lists_bitexact(ctx4, orc4, ref_counts4):
+    banner("Ind_only_driver_4ph 4x4x4")
+    No, _ = ref_counts4
+    want = orc4.ind_only_driver(No)
+    got = ctx4.Ind_only_driver_4ph(*No)
+    for ch in range(3):
+        for t in range(3):
+            assert np.array_equal(got[ch][t], want[ch][t]), (ch, t)
+    # the same lists when emitted in many small batches
+    ctx4.set_batch_capacity(2048)
+    try:
+        got2 = ctx4.Ind_only_driver_4ph(*No)
+    finally:
+        ctx4.set_batch_capacity(1 << 26)
+    for ch in range(3):
+        for t in range(3):
+            assert np.array_equal(got2[ch][t], want[ch][t]), (ch, t)
+
+
+def test_rta_fp64(ctx4, orc4):
+    banner("RTA_driver_4ph FP64 4x4x4")
+    got = ctx4.RTA_driver_4ph(rt.PREC_FP64)
+    want = orc4.rta_driver(nthreads=8)
+    pairs = [("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+             ("rate_scatt_minusminus", "rate_mm"), ("rate_scatt_4ph", "rate_4ph"),
+             ("WP4_plusplus", "wp4_pp"), ("WP4_plusminus", "wp4_pm"), ("WP4_minusminus", "wp4_mm")]
+    for g, w in pairs:
+        assert_close(got[g].ravel(order="F"), want[w], RTOL, g)
+    assert np.abs(want["rate_4ph"]).max() > 0
+
+
+def test_rta_batch_invariant_and_using_ind(ctx4, ref_counts4):
+    banner("batch invariance + RTA_driver_4ph_GPU_using_Ind")
+    base = ctx4.RTA_driver_4ph(rt.PREC_FP64)
+    ctx4.set_batch_capacity(4096)
+    try:
+        small = ctx4.RTA_driver_4ph(rt.PREC_FP64)
+    finally:
+        ctx4.set_batch_capacity(1 << 26)
+    for k in base:
+        assert np.array_equal(base[k], small[k]), k
+    No, _ = ref_counts4
+    lists = ctx4.Ind_only_driver_4ph(*No)
+    via = ctx4.RTA_driver_4ph_GPU_using_Ind(*No, lists)
+    for k in base:
+        assert np.array_equal(base[k], via[k]), k
+
+
+def test_ind_driver(ctx4, orc4, ref_counts4):
+    banner("Ind_driver_4ph 4x4x4")
+    No, _ = ref_counts4
+    lists, gam, rates = ctx4.Ind_driver_4ph(*No)
+    wl, wg, wr = orc4.ind_driver(No, nthreads=8)
+    for ch in range(3):
+        for t in range(3):
+            assert np.array_equal(lists[ch][t], wl[ch][t])
+        assert_close(gam[ch], wg[ch], RTOL, f"Gamma channel {ch}")
+    assert_close(rates["rate_scatt_4ph"].ravel(order="F"), wr["rate_4ph"], RTOL, "rate_4ph")
+
+
+def test_vp_all_channels():
+    banner("Vp_pp/pm/mm diamond 4x4x4 shell 2")
+    case = crystal.config_case(1, ngrid=(4, 4, 4), shell=2)
+    from oracle.oracle import Oracle
+    o = Oracle(case)
+    c = rt.Context(case)
+    m = case.modes
+    rng = np.random.default_rng(1)
+    for ch, (s2, s3) in enumerate([(1, 1), (1, -1), (-1, -1)]):
+        rows = []
+        for _ in range(40):
+            q = int(m.List[rng.integers(m.Nlist)])
+            qp, qdp = (int(x) for x in rng.integers(1, m.nptk + 1, size=2))
+            g = (m.IJK[:, q - 1] + s2 * m.IJK[:, qp - 1] + s3 * m.IJK[:, qdp - 1]) % m.ngrid
+            qt = int(g[0] + m.ngrid[0] * (g[1] + m.ngrid[1] * g[2])) + 1
+            rows.append([*(int(x) for x in rng.integers(1, m.nbands + 1, size=4)), q, qp, qdp, qt])
+        rows = np.array(rows, np.int32)
+        got = c.Vp(ch, rows)
+        want = np.array([o.vp(ch, *r) for r in rows])
+        scale = np.abs(want).max()
+        assert scale > 0
+        assert np.abs(got - want).max() <= 1e-12 * scale, ch
+    c.close()
+
+
+def test_onlyharmonic_and_omega_max():
+    banner("onlyharmonic / omega_max")
+    from oracle.oracle import Oracle
+    case = crystal.config_case(1, ngrid=(4, 4, 4))
+    case.onlyharmonic = True
+    case.omega_max = 70.0
+    c = rt.Context(case)
+    o = Oracle(case)
+    N = c.NP_driver_4ph()
+    No, _ = o.np_driver()
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch], No[ch])
+    got = c.RTA_driver_4ph(rt.PREC_FP64)
+    want = o.rta_driver()
+    assert np.all(got["rate_scatt_4ph"] == 0)
+    assert_close(got["WP4_plusminus"].ravel(order="F"), want["wp4_pm"], RTOL, "WP4 +-")
+    c.close()
+
+
+def test_wurtzite_nb12():
+    banner("wurtzite 3x3x2 shell 1, Nb=12")
+    from oracle.oracle import Oracle
+    case = crystal.config_case(3, ngrid=(3, 3, 2), shell=1)
+    c = rt.Context(case)
+    o = Oracle(case)
+    N = c.NP_driver_4ph()
+    No, Po = o.np_driver()
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch], No[ch])
+    got = c.Ind_only_driver_4ph(*No)
+    want = o.ind_only_driver(No)
+    for ch in range(3):
+        for t in range(3):
+            assert np.array_equal(got[ch][t], want[ch][t])
+    modes = np.arange(1, c.nmodes + 1)[::4]
+    r, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
+    w = o.rta_driver(modes=modes, nthreads=8)
+    assert_close(r["rate_scatt_4ph"].ravel(order="F")[modes - 1], w["rate_4ph"][modes - 1], RTOL, "rate")
+    c.close()
+
+
+# --------------------------------------------------------------- golden sizes
+
+def _golden(name):
+    path = os.path.join(GOLDEN, name + ".npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{name}.npz not generated")
+    return np.load(path)
+
+
+def _check_golden_lists(c, g, modes, chunk):
+    """List hashes of the given modes, emitted a few modes at a time."""
+    idx = {int(m): t for t, m in enumerate(g["modes"])}
+    for s in range(0, len(modes), chunk):
+        sub = modes[s:s + chunk]
+        Np, Npm, Nmm, lists = c.Ind_only_driver_4ph_modes(sub)
+        for ch, Nch in enumerate((Np, Npm, Nmm)):
+            acc = np.concatenate([[0], np.cumsum(Nch)])
+            for t, mm in enumerate(sub):
+                h = list_hash(*(x[acc[t]:acc[t + 1]] for x in lists[ch]))
+                assert h == int(g["H"][ch, idx[int(mm)]]), (ch, int(mm))
+
+
+def test_golden_diamond_8cube():
+    g = _golden("cfg1")
+    banner("golden diamond 8x8x8 (workload 1)")
+    case = crystal.config_case(1)
+    c = rt.Context(case)
+    N = c.NP_driver_4ph()
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch], g["N"][ch]), ch
+        assert_close(N[2 * ch + 1].ravel(order="F"), g["P"][ch], 1e-12, f"P {ch}")
+    _check_golden_lists(c, g, g["modes"], 24)
+    rm = g["rate_modes"]
+    r, _ = c.RTA_driver_4ph_modes(rm, rt.PREC_FP64)
+    for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+                    ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusminus", "wp4_pm")):
+        assert_close(r[key].ravel(order="F")[rm - 1], g[wk], RTOL, key)
+    c.close()
+
+
+def test_golden_diamond_16cube_shell2():
+    g = _golden("cfg2")
+    banner("golden diamond 16x16x16 shell 2 (workload 2)")
+    case = crystal.config_case(2)
+    c = rt.Context(case)
+    N = c.NP_driver_4ph()
+    modes = g["modes"]
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch][modes - 1], g["N"][ch]), ch
+        assert_close(N[2 * ch + 1].ravel(order="F")[modes - 1], g["P"][ch], 1e-12, f"P {ch}")
+    _check_golden_lists(c, g, modes[::3], 1)
+    rm = g["rate_modes"]
+    r, _ = c.RTA_driver_4ph_modes(rm, rt.PREC_FP64, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
+    for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+                    ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusplus", "wp4_pp")):
+        assert_close(r[key].ravel(order="F")[rm - 1], g[wk], RTOL, key)
+    c.close()
