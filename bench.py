@@ -1,0 +1,205 @@
+"""Benchmark: four-phonon processes/s on one MI355X (dense form vs FP64 form).
+
+Workload: the synthetic diamond crystal (a = 5.43 A, m = 28.085 amu, spring
+FC2, seeded second-shell fourth-order IFCs) on a 16x16x16 Gamma-centred mesh
+(4096 q, 6 branches, 145 irreducible q, 870 modes), RTA at 300 K,
+scalebroad 1.  A step is one RTA_driver_4ph pass over all 870 modes
+(all three channels) in the dense FP32 form; value = processes/s of that
+pass, with every input resident in HBM before the timed region.
+
+Also reported: the FP64 per-process form (the reference-faithful kernel) and
+the dense form on the same stratified sample (every 29th mode plus the modes
+with the fewest and most processes, second-phonon window q' = 1..64), the
+per-mode rate and RTA kappa errors of the FP32 form against the dense FP64
+form on the full workload, the dominant kernel's roofline numbers (HIP events
+on the library's stream), and the CPU oracle timed on the host cores on a
+bounded sample.
+
+Library calls are synchronous (they return after their stream has drained),
+so a wall clock around the steps brackets exactly the device work.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix peak
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kappa_rta(case, rate):
+    """RTA kappa (sum over branches, W/m/K) from per-mode rates, FP64 on the host
+    (TConduct, conductivity.f90:11-37, with F = tau v omega)."""
+    m = case.modes
+    r = np.asarray(rate).reshape(m.nbands, m.Nlist, order="F")
+    tau = np.where(r != 0, 1.0 / np.where(r != 0, r, 1.0), 0.0)
+    # expand irreducible tau to the full grid through the stars
+    tq = np.zeros((m.nptk, m.nbands))
+    for ll in range(m.Nlist):
+        for kk in range(m.Nequi[ll]):
+            tq[m.ALLEquiList[kk, ll] - 1, :] = tau[:, ll]
+    hbar, kb = 1.05457172647e-22, 1.380648813e-23
+    T, V = case.T, case.structure.volume
+    w = m.energy[1:]
+    v = m.velocity[1:]
+    f = 1.0 / np.expm1(hbar * w / kb / T)
+    k = np.einsum("qb,qba,qbc->ac", f * (f + 1) * w * w * tq[1:], v, v)
+    return k * float(np.float32(1e21)) * hbar ** 2 / (kb * T * T * V * m.nptk)
+
+
+def dense_flops(case, ngroups, nfine, ntriples, nproc, nitems):
+    """Algorithmic flops of one dense-form pass (Nb = number of branches):
+    per visited (q, q', q'') triple the skinny GEMM, 8 flops per complex MAC
+    over ngroups * Nb^2 * 9 entries; per process the contraction,
+    8 * (Nb^2 + Nb); per work item (channel, q, q') the A' build,
+    5 * 81 * Ntri + nfine * 8 * 27 * Nb * (3 + Nb)."""
+    nb = case.nbands
+    per_triple = 8.0 * ngroups * nb * nb * 9
+    per_proc = 8.0 * (nb * nb + nb)
+    per_item = 5.0 * 81 * case.ifc4.Ntri + nfine * 8.0 * 27 * nb * (3 + nb)
+    return per_triple * ntriples + per_proc * nproc + per_item * nitems
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    from fourphonon_amd import crystal
+    from fourphonon_amd import runtime as rt
+
+    t0 = time.time()
+    case = crystal.config_case(2)
+    m = case.modes
+    log(f"case built in {time.time() - t0:.1f}s: nptk={m.nptk} Nb={m.nbands} Nlist={m.Nlist} Ntri={case.ifc4.Ntri}")
+    ctx = rt.Context(case)
+    nm = ctx.nmodes
+    all_modes = np.arange(1, nm + 1, dtype=np.int32)
+
+    # ---- counts (prefilter), reported separately
+    t1 = time.perf_counter()
+    Npp, _, Npm, _, Nmm, _ = ctx.NP_driver_4ph()
+    t_pref = time.perf_counter() - t1
+    nproc_total = int(Npp.sum() + Npm.sum() + Nmm.sum())
+    log(f"prefilter: {nproc_total} processes in {t_pref:.2f}s")
+
+    # ---- timed dense FP32 steps over the whole workload
+    for w in range(args.warmup):
+        ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP32)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    ctx.reset_stats()
+    t1 = time.perf_counter()
+    for s in range(args.steps):
+        r32, nproc = ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP32)
+        log(f"step {s + 1}/{args.steps} done")
+    wall = time.perf_counter() - t1
+    assert nproc == nproc_total, (nproc, nproc_total)
+    k_ms, k_n = ctx.kernel_stats(rt.Context.KS_DENSE)
+    ms_step = 1e3 * wall / args.steps
+    value = nproc_total * args.steps / wall
+
+    # ---- roofline of the dense kernel
+    info = ctx.dense_info()
+    ntriples = info["triples"]
+    flops = dense_flops(case, info["ngroups"], info["nfine"], ntriples, nproc_total, info["items"])
+    avg_ms = k_ms / max(k_n, 1)
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_dense<float,6,6,4>", "kernel_ms": round(avg_ms, 3),
+                "flops_per_launch": flops}
+
+    out = {
+        "metric": "4ph processes/s on 1 MI355X, MFMA form vs FP64 form, kappa within 1e-4 of FP64",
+        "value": value, "unit": "processes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "diamond 16x16x16, 2nd-shell 4th-order IFCs (Ntri=1522), 4ph RTA at 300 K, "
+                               "all 870 modes", "ngrid": [16, 16, 16], "nbands": 6, "nlist": int(m.Nlist),
+                   "ntri4": int(case.ifc4.Ntri), "processes": nproc_total, "triples": ntriples},
+        "prefilter_seconds": t_pref,
+        "roofline": roofline,
+    }
+
+    # ---- FP64 form and dense form on the same stratified sample
+    if not args.no_fp64:
+        tot = Npp + Npm + Nmm
+        act = np.nonzero(tot > 0)[0] + 1
+        sample = np.unique(np.concatenate([act[::29], [act[np.argmin(tot[act - 1])], act[np.argmax(tot[act - 1])]]]))
+        lo, hi = 1, 64
+        ctx.reset_stats()
+        t1 = time.perf_counter()
+        r64s, n64s = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP64, ii_lo=lo, ii_hi=hi)
+        t64 = time.perf_counter() - t1
+        ms_r64, _ = ctx.kernel_stats(rt.Context.KS_RATE64)
+        ms_emit, _ = ctx.kernel_stats(rt.Context.KS_EMIT)
+        ms_cnt, _ = ctx.kernel_stats(rt.Context.KS_PREFILTER)
+        t1 = time.perf_counter()
+        r32s, n32s = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP32, ii_lo=lo, ii_hi=hi)
+        t32 = time.perf_counter() - t1
+        assert n32s == n64s
+        fp64_pp = n64s / t64
+        fp64_flops = 1822.0 * case.ifc4.Ntri * n64s
+        out["fp64_form"] = {"value": fp64_pp, "unit": "processes/s", "processes": n64s,
+                            "sample": f"{len(sample)} modes (every 29th active mode + min/max count), q'={lo}..{hi}",
+                            "rate_kernel_ms": ms_r64, "emit_ms": ms_emit, "count_ms": ms_cnt,
+                            "rate_kernel_tflops": fp64_flops / (ms_r64 * 1e-3) / 1e12,
+                            "rate_kernel_frac_fp64_peak": fp64_flops / (ms_r64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        out["dense_on_fp64_sample"] = {"value": n32s / t32, "unit": "processes/s"}
+        out["speedup_vs_fp64_form"] = (n32s / t32) / fp64_pp
+        e_s = np.abs(r32s["rate_scatt_4ph"].ravel(order="F")[sample - 1] - r64s["rate_scatt_4ph"].ravel(order="F")[sample - 1])
+        out["sample_rate_max_rel_err"] = float(np.max(e_s / np.abs(r64s["rate_scatt_4ph"].ravel(order="F")[sample - 1])))
+        log(f"FP64 form {fp64_pp:.3e} proc/s on sample; dense {n32s / t32:.3e}")
+
+        # accuracy of the FP32 form on the whole workload, against dense FP64
+        t1 = time.perf_counter()
+        r64, _ = ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP64_DENSE)
+        out["dense_fp64_seconds"] = time.perf_counter() - t1
+        a = r32["rate_scatt_4ph"].ravel(order="F")
+        b = r64["rate_scatt_4ph"].ravel(order="F")
+        nz = b != 0
+        out["rate_max_rel_err"] = float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])))
+        k32, k64 = kappa_rta(case, a), kappa_rta(case, b)
+        out["kappa_rta_trace_fp64"] = float(np.trace(k64) / 3)
+        out["kappa_rel_err"] = float(abs(np.trace(k32) - np.trace(k64)) / abs(np.trace(k64)))
+        log(f"rate max rel err {out['rate_max_rel_err']:.2e}, kappa rel err {out['kappa_rel_err']:.2e}")
+
+    # ---- CPU baseline: the oracle on the host cores, bounded sample
+    if not args.no_cpu:
+        try:
+            from oracle.oracle import Oracle
+            cores = min(16, len(os.sched_getaffinity(0)))
+            o = Oracle(case)
+            cs = np.array([5, 118, 333, 500, 611, 777, 850, nm], np.int32)
+            t1 = time.perf_counter()
+            o.rta_driver(modes=cs, ii_lo=1, ii_hi=2, nthreads=cores)
+            tc = time.perf_counter() - t1
+            Ncs = ctx.RTA_driver_4ph_modes(cs, rt.PREC_FP64, ii_lo=1, ii_hi=2)[1]
+            out["cpu_baseline"] = {"value": Ncs / tc, "unit": "processes/s", "cores": cores, "kind": "port",
+                                   "sample": f"FP64 C oracle (OpenMP dynamic,1), {len(cs)} modes, q'=1..2, "
+                                             f"{Ncs} processes in {tc:.1f}s"}
+        except Exception as e:  # the baseline must not hide the GPU result
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

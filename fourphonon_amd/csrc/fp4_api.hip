@@ -96,7 +96,7 @@ struct fp4_ctx {
   std::vector<double> h_phi;
   // scratch
   DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g;
-  DevBuf dense_scratch;
+  DenseState *dense = nullptr;
   long long list_cap = 1LL << 26;   // processes per batch
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -416,9 +416,10 @@ int fp4_ctx_destroy(fp4_ctx *c) {
   (void)hipSetDevice(c->device);
   DevBuf *bufs[] = {&c->energy, &c->vel, &c->vbound, &c->eig, &c->ijk, &c->list, &c->clusters, &c->phi, &c->roots,
                     &c->modes_sub, &c->cnt, &c->psum, &c->offsets, &c->segN, &c->segP, &c->lists[0], &c->lists[1],
-                    &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g,
-                    &c->dense_scratch};
+                    &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g};
   for (DevBuf *b : bufs) b->release();
+  dense_destroy(c->dense);
+  c->dense = nullptr;
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -533,9 +534,11 @@ int fp4_ctx_set_ifc4(fp4_ctx *c, int32_t ntri, const double *Phi, const double *
   HIPCHECK(hipMemcpy(c->roots.p, roots.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
   c->ntri = ntri;
   c->L = L;
+  dense_destroy(c->dense);
+  int drc = 0;
+  c->dense = dense_create(c->h_clusters, c->h_phi, c->natoms, c->nb, L, &drc);
+  if (!c->dense) return fail(drc, "%s", dense_error());
   c->have_ifc = true;
-  int drc = dense_prepare(c->h_clusters, c->h_phi, c->natoms, c->nb);
-  (void)drc;
   return FP4_OK;
 }
 
@@ -631,22 +634,22 @@ int fp4_rta_4ph_modes(fp4_ctx *c, int32_t precision, int32_t nmodes, const int32
       if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
     }
   } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3) {
-    if ((rc = count_pass(c, pl, false))) return rc;
     DenseRun dr;
     dr.m = mode_view(c);
+    dr.roots = c->roots.as<double2>();
     dr.T = c->T;
     dr.onlyharmonic = c->onlyharmonic;
     dr.natoms = c->natoms;
     dr.precision = precision;
     dr.stream = c->stream;
-    dr.scratch = &c->dense_scratch.p;
-    dr.scratch_bytes = &c->dense_scratch.bytes;
     dr.tuning = &c->dense_tuning;
     double ms = 0.0;
-    rc = dense_rates(dr, pl.modes, pl.ii_lo0, pl.nii, g, w, &ms);
+    std::vector<long long> n;
+    rc = dense_rates(c->dense, dr, pl.modes, pl.ii_lo0, pl.nii, g, w, n, &ms);
     if (rc) return fail(rc, "dense path: %s", dense_error());
     c->ks_ms[KS_DENSE] += ms;
     c->ks_n[KS_DENSE] += 1;
+    pl.segN = n;
   } else {
     return fail(FP4_ERR_ARG, "unknown precision %d", precision);
   }
@@ -866,6 +869,18 @@ int fp4_ctx_reset_stats(fp4_ctx *c) {
     c->ks_ms[k] = 0;
     c->ks_n[k] = 0;
   }
+  return FP4_OK;
+}
+
+int fp4_ctx_dense_info(fp4_ctx *c, int64_t *triples, int64_t *items, int32_t *ngroups, int32_t *nfine) {
+  if (!c || !triples || !items || !ngroups || !nfine) return fail(FP4_ERR_ARG, "null argument");
+  long long t, it;
+  int g, f;
+  dense_info(c->dense, &t, &it, &g, &f);
+  *triples = t;
+  *items = it;
+  *ngroups = g;
+  *nfine = f;
   return FP4_OK;
 }
 

@@ -8,25 +8,55 @@
 
 namespace fp4 {
 
-struct DenseTuning {
-  int qblock = 0;   // q'' per tile; 0 = choose automatically
+// Coarse phase group: all clusters whose third and fourth atoms are (K, L)
+// and whose R_k - R_l is the same lattice vector d (in phase units).
+struct DGroup {
+  int K, L;
+  int d[3];
+  int f_first, f_count;   // fine groups of this coarse group
 };
+// Fine group: clusters of one coarse group that also share atoms (I, J).
+struct FGroup {
+  int I, J;
+  int c_first, c_count;   // clusters (sorted) of this fine group
+};
+// Cluster in dense order: q'-dependent phase multipliers and mass factor.
+struct DCluster {
+  int pjl[3];             // pj - pl
+  int pl[3];
+  double minv;
+};
+
+struct DenseTuning {
+  int slots = 0;          // resident blocks of the persistent kernel; 0 = 4 per CU
+};
+
+struct DenseState;        // device copies of the grouped clusters (opaque)
+
+DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<double> &phi, int natoms, int nb,
+                         int L, int *rc);
+void dense_destroy(DenseState *s);
+int dense_ngroups(const DenseState *s);
+// Work of the last dense_rates call: visited triples and work items, and the
+// coarse / fine group counts of the clusters.
+void dense_info(const DenseState *s, long long *triples, long long *items, int *ngroups, int *nfine);
 
 struct DenseRun {
   ModeView m;
+  const double2 *roots;
   double T;
   int onlyharmonic;
   int natoms;
-  int precision;
+  int precision;          // FP4_PREC_FP32 / FP4_PREC_FP64_DENSE
   hipStream_t stream;
-  void **scratch;
-  size_t *scratch_bytes;
   DenseTuning *tuning;
 };
 
-int dense_prepare(const std::vector<ClusterRec> &cl, const std::vector<double> &phi, int natoms, int nb);
-int dense_rates(const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii, std::vector<double> &g,
-                std::vector<double> &w, double *ms);
+// Rates of the given modes over the q' window [ii_lo0, ii_lo0+nii): g, w and
+// n are per-(channel, mode-in-subset) sums of Gamma (unscaled), WP4 and the
+// number of processes.
+int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
+                std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms);
 const char *dense_error();
 
 }  // namespace fp4

@@ -34,7 +34,7 @@ EXPORTS = [
     "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
     "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_ind_only_4ph_modes", "fp4_ctx_kernel_stats",
-    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity",
+    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info",
 ]
 
 
@@ -223,6 +223,13 @@ class Context:
 
     def reset_stats(self):
         _check(load().fp4_ctx_reset_stats(self.h))
+
+    def dense_info(self):
+        """-> dict(triples, items, ngroups, nfine) of the last dense-form call."""
+        t, it = C.c_int64(), C.c_int64()
+        g, f = C.c_int32(), C.c_int32()
+        _check(load().fp4_ctx_dense_info(self.h, C.byref(t), C.byref(it), C.byref(g), C.byref(f)))
+        return {"triples": t.value, "items": it.value, "ngroups": g.value, "nfine": f.value}
 
     def set_batch_capacity(self, processes):
         _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))

@@ -152,6 +152,10 @@ int fp4_ind_only_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, i
    rate kernel, 3 = dense rate kernels. */
 int fp4_ctx_kernel_stats(fp4_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 int fp4_ctx_reset_stats(fp4_ctx *ctx);
+/* Diagnostics; no counterpart in the reference.  Work of the last dense-form
+   call: visited (q, q', q'') triples, work items, and the number of coarse
+   (K, L, R_k-R_l) and fine (I, J, K, L, R_k-R_l) cluster groups. */
+int fp4_ctx_dense_info(fp4_ctx *ctx, int64_t *triples, int64_t *items, int32_t *ngroups, int32_t *nfine);
 /* Upper bound on the processes materialised per batch (default 2^26); no
    counterpart in the reference, which materialises every list at once. */
 int fp4_ctx_set_batch_capacity(fp4_ctx *ctx, int64_t processes);

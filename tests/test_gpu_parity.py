@@ -103,8 +103,9 @@ def test_rta_batch_invariant_and_using_ind(ctx4, ref_counts4):
         small = ctx4.RTA_driver_4ph(rt.PREC_FP64)
     finally:
         ctx4.set_batch_capacity(1 << 26)
+    # splitting a mode over batches changes only the association of its partial sums
     for k in base:
-        assert np.array_equal(base[k], small[k]), k
+        assert_close(small[k], base[k], 1e-13, k)
     No, _ = ref_counts4
     lists = ctx4.Ind_only_driver_4ph(*No)
     via = ctx4.RTA_driver_4ph_GPU_using_Ind(*No, lists)
@@ -239,7 +240,8 @@ def test_golden_diamond_16cube_shell2():
     modes = g["modes"]
     for ch in range(3):
         assert np.array_equal(N[2 * ch][modes - 1], g["N"][ch]), ch
-        assert_close(N[2 * ch + 1].ravel(order="F")[modes - 1], g["P"][ch], 1e-12, f"P {ch}")
+        # ~1e8 terms summed in a different order than the CPU loop
+        assert_close(N[2 * ch + 1].ravel(order="F")[modes - 1], g["P"][ch], RTOL, f"P {ch}")
     _check_golden_lists(c, g, modes[::3], 1)
     rm = g["rate_modes"]
     r, _ = c.RTA_driver_4ph_modes(rm, rt.PREC_FP64, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
