@@ -124,6 +124,9 @@ ModeView mode_view(const fp4_ctx *c) {
   for (int t = 0; t < 9; t++) m.b[t] = c->rlattvec[t];
   m.scalebroad = c->scalebroad;
   m.omega_max = c->omega_max;
+  for (int nu = 0; nu < 3; nu++)
+    for (int k = 0; k < 3; k++) m.bf[3 * nu + k] = (float)(c->rlattvec[3 * nu + k] / c->ngrid[nu]);
+  m.sbf = (float)std::fabs(c->scalebroad);
   return m;
 }
 
@@ -882,6 +885,25 @@ int fp4_ctx_dense_info(fp4_ctx *c, int64_t *triples, int64_t *items, int32_t *ng
   *ngroups = g;
   *nfine = f;
   return FP4_OK;
+}
+
+int fp4_ctx_dense_shape(fp4_ctx *c, int32_t *mfma, int32_t *tq, int32_t *ib) {
+  if (!c || !mfma || !tq || !ib) return fail(FP4_ERR_ARG, "null argument");
+  int a, b, d;
+  dense_shape(c->dense, &a, &b, &d);
+  *mfma = a;
+  *tq = b;
+  *ib = d;
+  return FP4_OK;
+}
+
+int fp4_selftest_mfma(int32_t device, double *max_err) {
+  if (!max_err) return fail(FP4_ERR_ARG, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return fail(FP4_ERR_DEVICE, "no device");
+  HIPCHECK(hipSetDevice(device));
+  const int rc = dense_selftest_mfma(max_err);
+  return rc ? fail(rc, "%s", dense_error()) : FP4_OK;
 }
 
 int fp4_ctx_set_batch_capacity(fp4_ctx *c, int64_t processes) {

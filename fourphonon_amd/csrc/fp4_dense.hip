@@ -23,12 +23,15 @@
 // exactly the reference's; Bose-Einstein weights, WP4 and the Gamma sums
 // stay in FP64.  T (float or double) is the arithmetic of A', Psi and V.
 //
-// Scheduling: one persistent block per slot pulls work items from an atomic
-// counter; each block owns a global scratch slot for A'.  Results are
-// written per item and reduced over q' in a fixed order, and each block's
-// partial sums are reduced in a fixed tree order, so the output does not
-// depend on which block ran which item.
+// Scheduling: one persistent 512-thread block per CU pulls work items from
+// an atomic counter.  A', the per-cluster coefficients, the phase table and
+// a tile of Psi live in LDS (up to ~150 KiB of gfx950's 160 KiB); the kernel
+// shape (modes per block, q'' per tile) is the first one in kShapes whose
+// LDS fits.  Results are written per item and reduced over q' in a fixed
+// order, and each block's partial sums are reduced in a fixed tree order, so
+// the output does not depend on which block ran which item.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -39,6 +42,8 @@
 #include "fp4_dense.hpp"
 
 namespace fp4 {
+
+constexpr int NT_DENSE = 512;
 
 namespace {
 thread_local std::string d_err;
@@ -64,6 +69,7 @@ struct DenseState {
   int *counter = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   long long last_triples = 0, last_items = 0;
+  int last_mfma = 0, last_tq = 0, last_ib = 0;
 };
 
 template <class T>
@@ -90,12 +96,14 @@ struct DenseArgs {
   int onlyharmonic;
   long nitems;
   int *counter;
-  void *scratch;
-  long slot_elems;       // complex elements of A' per slot
   double *out_g;
   double *out_w;
   int *out_n;
   int *out_t;            // [nitems] visited (q, q', q'') triples
+  int ntri;
+  int smem_static;       // bytes of DenseSmem at the start of dynamic LDS
+  float bf[9];           // rlattvec(:,nu)/ngrid(nu) as floats
+  float sbf;             // scalebroad
 };
 
 template <class T>
@@ -106,25 +114,8 @@ __device__ __forceinline__ double2 zm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// block-wide exclusive scan of 0/1 flags in thread order; returns the total
-__device__ __forceinline__ int block_flag_scan(int flag, int *s_wsum, int &pos) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const unsigned long long bal = __ballot(flag);
-  const int before = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-  if (lane == 0) s_wsum[wv] = __popcll(bal);
-  __syncthreads();
-  int base = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < 4; w++) {
-    if (w < wv) base += s_wsum[w];
-    tot += s_wsum[w];
-  }
-  pos = base + before;
-  __syncthreads();
-  return tot;
-}
-
-// block-wide exclusive scan of per-thread counts in thread order
+// block-wide exclusive scan of per-thread counts in thread order (NW waves)
+template <int NW>
 __device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int incl_w = wave_excl_scan(v, lane) + v;
@@ -133,7 +124,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
   int base = 0;
   tot = 0;
 #pragma unroll
-  for (int w = 0; w < 4; w++) {
+  for (int w = 0; w < NW; w++) {
     if (w < wv) base += s_wsum[w];
     tot += s_wsum[w];
   }
@@ -141,44 +132,114 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
   return base + incl_w - v;
 }
 
+// Velocities of the tile's third and fourth phonons live in LDS, so sigma is
+// recomputed here with the same operations as pair_sigma().
+__device__ __forceinline__ double tile_sigma(const ModeView &m, int ch, const double *v3, const double *v4) {
+  double d[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) d[a] = (ch == 0) ? (-v3[a] + v4[a]) : (v3[a] - v4[a]);
+  return m.scalebroad * base_sigma(m, d[0], d[1], d[2]);
+}
+
+// Float estimate of sigma for the cull: b_nu / N_nu and scalebroad as floats,
+// velocity differences formed in FP64 and rounded once.  Its relative error
+// is below 1e-6, far inside the 1e-4 band that is sent to the exact test.
+__device__ __forceinline__ float sigma_estimate(const float *bf, int ch, const double *v3, const double *v4,
+                                                float sbf) {
+  float d[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) d[a] = (float)((ch == 0) ? (-v3[a] + v4[a]) : (v3[a] - v4[a]));
+  float mx = 0.f;
+#pragma unroll
+  for (int nu = 0; nu < 3; nu++) {
+    const float x = bf[3 * nu] * d[0] + bf[3 * nu + 1] * d[1] + bf[3 * nu + 2] * d[2];
+    mx = fmaxf(mx, x * x);
+  }
+  return sbf * sqrtf(0.5f * mx);
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Lane layout of v_mfma_f32_16x16x4f32 (one 16x16 block, K = 4):
+//   A[i][k] is held by lane i + 16 k, B[k][j] by lane j + 16 k,
+//   D[i][j] by lane j + 16 (i / 4), accumulator register i % 4.
+// k_mfma_selftest checks exactly this mapping on the device.
+__global__ void k_mfma_selftest(double *err) {
+  const int l = threadIdx.x;
+  const int i = l & 15, kk = l >> 4;
+  const float av = (float)(1 + i + 3 * kk) * 0.25f;          // A[i][kk]
+  const float bv = (float)(2 + (l & 15) * 5 - kk) * 0.125f;  // B[kk][j], j = l & 15
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+  double e = 0.0;
+  for (int r = 0; r < 4; r++) {
+    const int ii = 4 * (l >> 4) + r, jj = l & 15;
+    double want = 0.0;
+    for (int k = 0; k < 4; k++) want += (double)((1 + ii + 3 * k) * 0.25f) * (double)((2 + jj * 5 - k) * 0.125f);
+    e = fmax(e, fabs(want - (double)acc[r]));
+  }
+  for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+  if (l == 0) *err = e;
+}
+
 template <class T, int NB, int IB, int TQ>
-__global__ void __launch_bounds__(256) k_dense(DenseArgs a) {
+struct DenseSmem {
+  static constexpr int NOUT2 = IB * NB * 9;   // A' entries per group = Psi entries per (K,L) and triple
+  static constexpr int FMAX = 8;              // fine groups per setup chunk
+  static constexpr int QMAX = NT_DENSE * NB;  // survivors contracted together (>= one round)
+  union {
+    C2<T> psi[TQ * NOUT2];                    // Psi of one (K,L) pair for the tile
+    struct {
+      double2 af[FMAX][81];
+      double2 b1[FMAX][IB][27];
+    } setup;
+  } u;
+  C2<T> e3[TQ][NB][NB];   // [t][comp][k]
+  C2<T> e4[TQ][NB][NB];   // [t][comp][l]
+  double Ejj[TQ][NB], Ess[TQ][NB];
+  double vjj[TQ][NB][3], vss[TQ][NB][3];
+  double sig[TQ][NB][NB];  // exact sigma of (q'',k) / (q''',l), -1 when excluded
+  int jj[TQ], ss[TQ];
+  double2 e1[NB][IB];
+  double2 e2[NB][NB];
+  int vj[NT_DENSE], vs[NT_DENSE];
+  unsigned short q[QMAX];  // candidate codes (t, k, l, i, j)
+  int wsum[NT_DENSE / 64];
+  double om[IB];
+  int act[IB];
+  double Eii[NB];
+  double x1[IB][NB];       // w_i +- w'_j, the first partial sum of delta
+  long item;
+  double red[NT_DENSE / 64][3 * IB];
+};
+
+template <class T, int NB, int IB, int TQ, bool MF>
+__global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
+  using S = DenseSmem<T, NB, IB, TQ>;
   constexpr int NA = NB / 3;
-  constexpr int NCOL = IB * NB * NB * NB;
-  constexpr int QMAX = 1280;
-  constexpr int NOUT2 = IB * NB * 9;            // A' entries per group
-  constexpr int OPT = (NOUT2 + 255) / 256;      // A' entries owned per thread
-  extern __shared__ unsigned char dyn[];
-  C2<T> *s_ph = reinterpret_cast<C2<T> *>(dyn);  // [TQ][ng]
-  __shared__ double2 s_e1[NB][IB];
-  __shared__ double2 s_e2[NB][NB];
-  __shared__ double2 s_af[81];
-  __shared__ double2 s_b1[IB][27];
-  __shared__ C2<T> s_psi[TQ][NCOL];
-  __shared__ C2<T> s_e3[TQ][NB][NB];
-  __shared__ C2<T> s_e4[TQ][NB][NB];
-  __shared__ int s_jj[TQ], s_ss[TQ], s_n3[TQ][3];
-  __shared__ double s_Ejj[TQ][NB], s_Ess[TQ][NB];
-  __shared__ int s_vjj[256];
-  __shared__ int s_q[QMAX];
-  __shared__ double s_qw[QMAX];
-  __shared__ int s_wsum[4];
-  __shared__ double s_om[IB];
-  __shared__ int s_act[IB];
-  __shared__ double s_Eii[NB];
-  __shared__ long s_item;
-  __shared__ double s_red[4][3 * IB];
+  constexpr int QMAX = S::QMAX;
+  constexpr int FMAX = S::FMAX;
+  constexpr int NOUT2 = S::NOUT2;
+  constexpr int NW = NT_DENSE / 64;
+  constexpr int QPT = QMAX / NT_DENSE;        // survivors per thread in a contraction
+  static_assert(!MF || (sizeof(T) == 4 && 2 * TQ == 16), "MFMA path: float, 8 q'' per tile");
+  static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
+  extern __shared__ __align__(16) unsigned char dyn[];
+  S &s = *reinterpret_cast<S *>(dyn);
+  C2<T> *s_A = reinterpret_cast<C2<T> *>(dyn + a.smem_static);   // [ng][NOUT2]
+  C2<T> *s_c = s_A + (size_t)a.ng * NOUT2;                        // [ntri]
+  C2<T> *s_ph = s_c + a.ntri;                                     // [TQ][ng]
 
   const ModeView &m = a.m;
   const int tid = threadIdx.x;
-  C2<T> *Ap = reinterpret_cast<C2<T> *>(a.scratch) + (size_t)blockIdx.x * a.slot_elems;
+  const int lane = tid & 63, wave = tid >> 6;
   const double hbarp = kHbar * (double)k1e22;
   const double gpre = hbarp * hbarp * kPi / 8.0;
 
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(a.counter, 1);
+    if (tid == 0) s.item = atomicAdd(a.counter, 1);
     __syncthreads();
-    const long item = s_item;
+    const long item = s.item;
     __syncthreads();
     if (item >= a.nitems) break;
     const int ii0 = (int)(item % a.nii) + a.ii_lo0;
@@ -200,15 +261,18 @@ __global__ void __launch_bounds__(256) k_dense(DenseArgs a) {
     if (tid < IB) {
       const int i0 = ib * IB + tid;
       const double w = E(m, qi0, i0);
-      const int mm0 = ll0 * NB + i0;
-      s_om[tid] = w;
-      s_act[tid] = (a.sel[mm0] != 0) && (w != 0.0) && (w <= m.omega_max);
+      s.om[tid] = w;
+      s.act[tid] = (a.sel[ll0 * NB + i0] != 0) && (w != 0.0) && (w <= m.omega_max);
     }
-    if (tid < NB) s_Eii[tid] = E(m, ii0, tid);
+    if (tid < NB) s.Eii[tid] = E(m, ii0, tid);
     __syncthreads();
+    if (tid < IB * NB) {
+      const int il = tid / NB, j = tid % NB;
+      s.x1[il][j] = (ch == 2) ? (s.om[il] - s.Eii[j]) : (s.om[il] + s.Eii[j]);
+    }
     int any = 0;
 #pragma unroll
-    for (int il = 0; il < IB; il++) any |= s_act[il];
+    for (int il = 0; il < IB; il++) any |= s.act[il];
 
     double acc_g[IB], acc_w[IB];
     int acc_n[IB];
@@ -221,254 +285,324 @@ __global__ void __launch_bounds__(256) k_dense(DenseArgs a) {
     }
 
     if (any) {
-      // ---------------- setup: eigenvectors of phonons 1 and 2
-      for (int x = tid; x < NB * IB; x += 256) {
+      // ------------- setup 1: eigenvectors of phonons 1, 2 and cluster coefficients
+      for (int x = tid; x < NB * IB; x += NT_DENSE) {
         const int comp = x / IB, il = x % IB;
-        s_e1[comp][il] = m.eig[((size_t)qi0 * NB + ib * IB + il) * NB + comp];
+        s.e1[comp][il] = m.eig[((size_t)qi0 * NB + ib * IB + il) * NB + comp];
       }
-      for (int x = tid; x < NB * NB; x += 256) {
+      for (int x = tid; x < NB * NB; x += NT_DENSE) {
         const int comp = x / NB, j = x % NB;
         double2 e = m.eig[((size_t)ii0 * NB + j) * NB + comp];
         if (ch == 2) e.y = -e.y;
-        s_e2[comp][j] = e;
+        s.e2[comp][j] = e;
+      }
+      for (int ci = tid; ci < a.ntri; ci += NT_DENSE) {
+        const DCluster c = a.c[ci];
+        const int k = s2 * (n2[0] * c.pjl[0] + n2[1] * c.pjl[1] + n2[2] * c.pjl[2]) -
+                      (n1[0] * c.pl[0] + n1[1] * c.pl[1] + n1[2] * c.pl[2]);
+        const double2 rt = a.roots[fmod_pos(k, a.L)];
+        s_c[ci] = C2<T>{(T)(c.minv * rt.x), (T)(c.minv * rt.y)};
       }
       __syncthreads();
-      // ---------------- setup: A'[gamma] for every coarse group
+      // ------------- setup 2: A'[gamma] for every coarse group, into LDS
       for (int gi = 0; gi < a.ng; gi++) {
         const DGroup G = a.g[gi];
-        double2 acc[OPT];
-#pragma unroll
-        for (int o = 0; o < OPT; o++) acc[o] = make_double2(0.0, 0.0);
-        for (int fi = G.f_first; fi < G.f_first + G.f_count; fi++) {
-          const FGroup F = a.f[fi];
-          if (tid < 81) {
-            double2 s = make_double2(0.0, 0.0);
-            for (int ci = F.c_first; ci < F.c_first + F.c_count; ci++) {
-              const DCluster c = a.c[ci];
-              const int k = s2 * (n2[0] * c.pjl[0] + n2[1] * c.pjl[1] + n2[2] * c.pjl[2]) -
-                            (n1[0] * c.pl[0] + n1[1] * c.pl[1] + n1[2] * c.pl[2]);
-              const double2 rt = a.roots[fmod_pos(k, a.L)];
-              const double p = a.phi[(size_t)ci * 81 + tid] * c.minv;
-              s.x += p * rt.x;
-              s.y += p * rt.y;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int f0 = 0; f0 < G.f_count; f0 += FMAX) {
+          const int nf = min(FMAX, G.f_count - f0);
+          for (int x = tid; x < nf * 81; x += NT_DENSE) {
+            const int fl = x / 81, e = x % 81;
+            const FGroup F = a.f[G.f_first + f0 + fl];
+            double sr = 0.0, si = 0.0;
+            int ci = F.c_first;
+            const int ce = F.c_first + F.c_count;
+            for (; ci + 1 < ce; ci += 2) {
+              const double p0 = a.phi[(size_t)ci * 81 + e], p1 = a.phi[(size_t)(ci + 1) * 81 + e];
+              const C2<T> c0 = s_c[ci], c1 = s_c[ci + 1];
+              sr += p0 * (double)c0.re;
+              si += p0 * (double)c0.im;
+              sr += p1 * (double)c1.re;
+              si += p1 * (double)c1.im;
             }
-            s_af[tid] = s;
+            if (ci < ce) {
+              const double p0 = a.phi[(size_t)ci * 81 + e];
+              const C2<T> c0 = s_c[ci];
+              sr += p0 * (double)c0.re;
+              si += p0 * (double)c0.im;
+            }
+            s.u.setup.af[fl][e] = make_double2(sr, si);
           }
           __syncthreads();
-          for (int x = tid; x < IB * 27; x += 256) {
-            const int il = x / 27, tsr = x % 27;
-            double2 s = make_double2(0.0, 0.0);
+          for (int x = tid; x < nf * IB * 27; x += NT_DENSE) {
+            const int fl = x / (IB * 27), il = (x / 27) % IB, tsr = x % 27;
+            const int I = a.f[G.f_first + f0 + fl].I;
+            double2 sacc = make_double2(0.0, 0.0);
 #pragma unroll
-            for (int u = 0; u < 3; u++) {
-              const double2 v = zm(s_af[u + 3 * tsr], s_e1[3 * F.I + u][il]);
-              s.x += v.x;
-              s.y += v.y;
+            for (int uu = 0; uu < 3; uu++) {
+              const double2 v = zm(s.u.setup.af[fl][uu + 3 * tsr], s.e1[3 * I + uu][il]);
+              sacc.x += v.x;
+              sacc.y += v.y;
             }
-            s_b1[il][tsr] = s;
+            s.u.setup.b1[fl][il][tsr] = sacc;
           }
           __syncthreads();
-#pragma unroll
-          for (int o = 0; o < OPT; o++) {
-            const int x = tid + 256 * o;
-            if (x < NOUT2) {
-              const int il = x / (NB * 9), rem = x % (NB * 9), j = rem / 9, sr = rem % 9;
-              double2 s = acc[o];
+          if (tid < NOUT2) {
+            const int il = tid / (NB * 9), rem = tid % (NB * 9), j = rem / 9, sr = rem % 9;
+            for (int fl = 0; fl < nf; fl++) {
+              const int J = a.f[G.f_first + f0 + fl].J;
 #pragma unroll
               for (int t = 0; t < 3; t++) {
-                const double2 v = zm(s_b1[il][t + 3 * sr], s_e2[3 * F.J + t][j]);
-                s.x += v.x;
-                s.y += v.y;
+                const double2 v = zm(s.u.setup.b1[fl][il][t + 3 * sr], s.e2[3 * J + t][j]);
+                acc.x += v.x;
+                acc.y += v.y;
               }
-              acc[o] = s;
             }
           }
           __syncthreads();
         }
-#pragma unroll
-        for (int o = 0; o < OPT; o++) {
-          const int x = tid + 256 * o;
-          if (x < NOUT2) Ap[(size_t)gi * NOUT2 + x] = C2<T>{(T)acc[o].x, (T)acc[o].y};
-        }
+        if (tid < NOUT2) s_A[(size_t)gi * NOUT2 + tid] = C2<T>{(T)acc.x, (T)acc.y};
       }
       __syncthreads();
 
-      // ---------------- q'' loop
+      // ------------- q'' loop
       const int jj_first = (ch == 1) ? 0 : ii0;
-      for (int base = jj_first; base < m.nptk; base += 256) {
-        // valid q'' of this chunk, in ascending order
+      for (int base = jj_first; base < m.nptk; base += NT_DENSE) {
         const int jj = base + tid;
-        int ok = 0;
+        int ok = 0, ssv = 0;
         if (jj < m.nptk) {
           int n3[3];
 #pragma unroll
           for (int t = 0; t < 3; t++) n3[t] = m.ijk[3 * jj + t];
-          const int ss = partner(m, ch, n1, n2, n3);
-          ok = (ch == 0) || (ss >= jj);
+          ssv = partner(m, ch, n1, n2, n3);
+          ok = (ch == 0) || (ssv >= jj);
         }
-        int pos;
-        const int nv = block_flag_scan(ok, s_wsum, pos);
-        ntrip += nv;
-        if (ok) s_vjj[pos] = jj;
+        int tot;
+        const int pos = block_excl_scan<NW>(ok, s.wsum, tot);
+        ntrip += tot;
+        if (ok) {
+          s.vj[pos] = jj;
+          s.vs[pos] = ssv;
+        }
         __syncthreads();
-        for (int t0 = 0; t0 < nv; t0 += TQ) {
-          const int tq = min(TQ, nv - t0);
-          // tile data
+        for (int t0 = 0; t0 < tot; t0 += TQ) {
+          const int tq = min(TQ, tot - t0);
           if (tid < TQ) {
-            if (tid < tq) {
-              const int jv = s_vjj[t0 + tid];
-              int n3[3];
-#pragma unroll
-              for (int t = 0; t < 3; t++) {
-                n3[t] = m.ijk[3 * jv + t];
-                s_n3[tid][t] = n3[t];
-              }
-              s_jj[tid] = jv;
-              s_ss[tid] = partner(m, ch, n1, n2, n3);
-            }
+            s.jj[tid] = (tid < tq) ? s.vj[t0 + tid] : 0;
+            s.ss[tid] = (tid < tq) ? s.vs[t0 + tid] : 0;
           }
           __syncthreads();
-          for (int x = tid; x < TQ * NB * NB; x += 256) {
+          // tile data: eigenvectors of phonons 3 and 4, frequencies, velocities, q'' phases
+          for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
             const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
+            C2<T> v3 = {(T)0, (T)0}, v4 = {(T)0, (T)0};
             if (t < tq) {
-              double2 e3 = m.eig[((size_t)s_jj[t] * NB + b) * NB + comp];
-              double2 e4 = m.eig[((size_t)s_ss[t] * NB + b) * NB + comp];
-              if (ch != 0) e3.y = -e3.y;
-              e4.y = -e4.y;
-              s_e3[t][comp][b] = C2<T>{(T)e3.x, (T)e3.y};
-              s_e4[t][comp][b] = C2<T>{(T)e4.x, (T)e4.y};
+              const double2 e3 = m.eig[((size_t)s.jj[t] * NB + b) * NB + comp];
+              const double2 e4 = m.eig[((size_t)s.ss[t] * NB + b) * NB + comp];
+              v3 = C2<T>{(T)e3.x, (T)((ch != 0) ? -e3.y : e3.y)};
+              v4 = C2<T>{(T)e4.x, (T)(-e4.y)};
             }
+            s.e3[t][comp][b] = v3;
+            s.e4[t][comp][b] = v4;
           }
-          for (int x = tid; x < TQ * NB; x += 256) {
+          for (int x = tid; x < TQ * NB; x += NT_DENSE) {
             const int t = x / NB, b = x % NB;
             if (t < tq) {
-              s_Ejj[t][b] = E(m, s_jj[t], b);
-              s_Ess[t][b] = E(m, s_ss[t], b);
-            }
-          }
-          for (int x = tid; x < TQ * a.ng; x += 256) {
-            const int t = x / a.ng, gi = x % a.ng;
-            if (t < tq) {
-              const DGroup G = a.g[gi];
-              const int k = s3 * (s_n3[t][0] * G.d[0] + s_n3[t][1] * G.d[1] + s_n3[t][2] * G.d[2]);
-              const double2 rt = a.roots[fmod_pos(k, a.L)];
-              s_ph[t * a.ng + gi] = C2<T>{(T)rt.x, (T)rt.y};
-            }
-          }
-          __syncthreads();
-          // skinny GEMM: Psi[t][col] = sum_{gamma in (K,L)} ph[t][gamma] A'[gamma][i j s r]
-          for (int col = tid; col < NCOL; col += 256) {
-            const int D = col % NB, Cc = (col / NB) % NB, ij = col / (NB * NB);
-            const int K = Cc / 3, s = Cc % 3, L = D / 3, r = D % 3;
-            const int g0 = a.kl_first[K * NA + L], g1 = a.kl_first[K * NA + L + 1];
-            C2<T> acc[TQ];
+              const int jv = s.jj[t], sv = s.ss[t];
+              s.Ejj[t][b] = E(m, jv, b);
+              s.Ess[t][b] = E(m, sv, b);
 #pragma unroll
-            for (int t = 0; t < TQ; t++) acc[t] = C2<T>{(T)0, (T)0};
-            for (int gi = g0; gi < g1; gi++) {
-              const C2<T> av = Ap[(size_t)gi * NOUT2 + ij * 9 + s + 3 * r];
-#pragma unroll
-              for (int t = 0; t < TQ; t++) {
-                const C2<T> p = s_ph[t * a.ng + gi];
-                acc[t].re += p.re * av.re - p.im * av.im;
-                acc[t].im += p.re * av.im + p.im * av.re;
+              for (int d = 0; d < 3; d++) {
+                s.vjj[t][b][d] = Vel(m, jv, b, d);
+                s.vss[t][b][d] = Vel(m, sv, b, d);
               }
             }
-#pragma unroll
-            for (int t = 0; t < TQ; t++) s_psi[t][col] = acc[t];
+          }
+          for (int x = tid; x < TQ * a.ng; x += NT_DENSE) {
+            const int t = x / a.ng, gi = x % a.ng;
+            C2<T> p = {(T)0, (T)0};
+            if (t < tq) {
+              const int vjt = s.jj[t];
+              const DGroup G = a.g[gi];
+              const int k = s3 * (m.ijk[3 * vjt] * G.d[0] + m.ijk[3 * vjt + 1] * G.d[1] + m.ijk[3 * vjt + 2] * G.d[2]);
+              const double2 rt = a.roots[fmod_pos(k, a.L)];
+              p = C2<T>{(T)rt.x, (T)rt.y};
+            }
+            s_ph[t * a.ng + gi] = p;
           }
           __syncthreads();
-          // exact filter over (t, i, j, k, l): each thread tests R consecutive
-          // candidates, a block scan of the per-thread counts keeps the
-          // survivors in candidate order, and the queue is contracted when
-          // it is about to overflow and at the end of the tile.
-          constexpr int NCAND = TQ * IB * NB * NB * NB;
-          constexpr int R = 4, CH = 256 * R;
-          int qlen = 0;
-          for (int cb = 0; cb < NCAND; cb += CH) {
-            double wps[R];
-            int mask = 0;
+          // exact sigma of every (t, k, l) of the tile
+          for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
+            const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
+            double sg = -1.0;
+            if (t < tq) {
+              const bool okkl = (s.Ejj[t][k] != 0.0) && (s.Ess[t][l] != 0.0) &&
+                                !(ch != 0 && s.ss[t] == s.jj[t] && l < k);
+              if (okkl) sg = tile_sigma(m, ch, s.vjj[t][k], s.vss[t][l]);
+            }
+            s.sig[t][k][l] = sg;
+          }
+          __syncthreads();
+
+          // contraction of the queued survivors: for each (K,L) pair, Psi of
+          // the tile (skinny GEMM over that pair's phase groups), then the
+          // survivors' partial sums; finally weights and rates in FP64.
+          auto contract = [&](int qlen) {
+            C2<T> V[QPT];
 #pragma unroll
-            for (int rr = 0; rr < R; rr++) {
-              const int cidx = cb + tid * R + rr;
-              wps[rr] = 0.0;
-              if (cidx < NCAND) {
-                const int l = cidx % NB, k = (cidx / NB) % NB, j = (cidx / (NB * NB)) % NB;
-                const int il = (cidx / (NB * NB * NB)) % IB, t = cidx / (NB * NB * NB * IB);
-                if (t < tq && s_act[il]) {
-                  const int jj0 = s_jj[t], ss0 = s_ss[t];
-                  const int i0 = ib * IB + il;
-                  const double w = s_om[il], op = s_Eii[j], odp = s_Ejj[t][k], otp = s_Ess[t][l];
-                  bool c1 = (op != 0.0) && (odp != 0.0) && (otp != 0.0);
-                  if (ch != 1 && jj0 == ii0 && k < j) c1 = false;
-                  if (ch != 0 && ss0 == jj0 && l < k) c1 = false;
-                  if (ch == 1 && c1)
-                    c1 = (qi0 != jj0 || i0 != k) && (qi0 != ss0 || i0 != l) && (ii0 != jj0 || j != k) &&
-                         (ii0 != ss0 || j != l);
-                  if (c1) {
-                    const double delta = mismatch(ch, w, op, odp, otp);
-                    const float bound =
-                        (m.vbound[(size_t)k * m.nptk + jj0] + m.vbound[(size_t)l * m.nptk + ss0]) * 1.0001f + 1e-30f;
-                    if (!((float)fabs(delta) > bound)) {
-                      double sigma = pair_sigma(m, ch, jj0, k, ss0, l);
-                      if (fabs(delta) <= sigma) {
-                        if (sigma <= (double)kClamp) sigma = 1.0 / sqrt(kPi);
-                        wps[rr] = wp4_of(ch, w, op, odp, otp, delta, sigma, a.T);
-                        mask |= 1 << rr;
+            for (int u = 0; u < QPT; u++) V[u] = C2<T>{(T)0, (T)0};
+            for (int kl = 0; kl < NA * NA; kl++) {
+              const int g0 = a.kl_first[kl], g1 = a.kl_first[kl + 1];
+              if (g1 == g0) continue;
+              if constexpr (MF) {
+                constexpr int NTILE = (NOUT2 + 15) / 16;
+                const int ksteps = (2 * (g1 - g0) + 3) / 4;
+                const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
+                for (int nt = wave; nt < NTILE; nt += NW) {
+                  const int n = nt * 16 + (lane & 15);
+                  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+                  for (int ks = 0; ks < ksteps; ks++) {
+                    const int kp = 4 * ks + kg;
+                    const int g = g0 + (kp >> 1), qq = kp & 1;
+                    float av = 0.f, bv = 0.f;
+                    if (g < g1) {
+                      const C2<T> p = s_ph[t * a.ng + g];
+                      av = part == 0 ? (qq == 0 ? (float)p.re : -(float)p.im) : (qq == 0 ? (float)p.im : (float)p.re);
+                      if (n < NOUT2) {
+                        const C2<T> A = s_A[(size_t)g * NOUT2 + n];
+                        bv = qq == 0 ? (float)A.re : (float)A.im;
                       }
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                  }
+                  if (n < NOUT2) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                      const int row = 4 * (lane >> 4) + r;
+                      if ((row & 1) == 0)
+                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)acc[r];
+                      else
+                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)acc[r];
                     }
                   }
                 }
-              }
-            }
-            int tot;
-            int o = qlen + block_excl_scan(__popc(mask), s_wsum, tot);
+              } else {
+                for (int n = tid; n < NOUT2; n += NT_DENSE) {
+                  C2<T> acc[TQ];
 #pragma unroll
-            for (int rr = 0; rr < R; rr++)
-              if ((mask >> rr) & 1) {
-                s_q[o] = cb + tid * R + rr;
-                s_qw[o] = wps[rr];
-                o++;
+                  for (int t = 0; t < TQ; t++) acc[t] = C2<T>{(T)0, (T)0};
+                  for (int g = g0; g < g1; g++) {
+                    const C2<T> av = s_A[(size_t)g * NOUT2 + n];
+#pragma unroll
+                    for (int t = 0; t < TQ; t++) {
+                      const C2<T> p = s_ph[t * a.ng + g];
+                      acc[t].re += p.re * av.re - p.im * av.im;
+                      acc[t].im += p.re * av.im + p.im * av.re;
+                    }
+                  }
+#pragma unroll
+                  for (int t = 0; t < TQ; t++) s.u.psi[t * NOUT2 + n] = acc[t];
+                }
               }
-            qlen += tot;
-            const bool last = (cb + CH >= NCAND);
-            if (qlen > QMAX - CH || (last && qlen > 0)) {
               __syncthreads();
-              for (int qi = tid; qi < qlen; qi += 256) {
-                const int cc = s_q[qi];
-                const int l = cc % NB, k = (cc / NB) % NB, j = (cc / (NB * NB)) % NB;
-                const int il = (cc / (NB * NB * NB)) % IB, t = cc / (NB * NB * NB * IB);
-                const double wpv = s_qw[qi];
-                double g = 0.0;
-                if (!a.onlyharmonic) {
-                  const C2<T> *psi = &s_psi[t][(il * NB + j) * NB * NB];
-                  C2<T> V = {(T)0, (T)0};
+              const int K = kl / NA, L = kl % NA;
 #pragma unroll
-                  for (int Cc = 0; Cc < NB; Cc++) {
+              for (int u = 0; u < QPT; u++) {
+                const int qi = tid + u * NT_DENSE;
+                if (qi < qlen) {
+                  const int cc = s.q[qi];
+                  const int j = cc % NB, il = (cc / NB) % IB, l = (cc / (NB * IB)) % NB;
+                  const int k = (cc / (NB * IB * NB)) % NB, t = cc / (NB * IB * NB * NB);
+                  const C2<T> *psi = &s.u.psi[t * NOUT2 + (il * NB + j) * 9];
+#pragma unroll
+                  for (int sx = 0; sx < 3; sx++) {
                     C2<T> inner = {(T)0, (T)0};
 #pragma unroll
-                    for (int D = 0; D < NB; D++) {
-                      const C2<T> x = cm(psi[Cc * NB + D], s_e4[t][D][l]);
-                      inner.re += x.re;
-                      inner.im += x.im;
+                    for (int rx = 0; rx < 3; rx++) {
+                      const C2<T> xx = cm(psi[sx + 3 * rx], s.e4[t][3 * L + rx][l]);
+                      inner.re += xx.re;
+                      inner.im += xx.im;
                     }
-                    const C2<T> y = cm(inner, s_e3[t][Cc][k]);
-                    V.re += y.re;
-                    V.im += y.im;
+                    const C2<T> y = cm(inner, s.e3[t][3 * K + sx][k]);
+                    V[u].re += y.re;
+                    V[u].im += y.im;
                   }
-                  const double vr = (double)V.re, vi = (double)V.im;
+                }
+              }
+              __syncthreads();
+            }
+#pragma unroll
+            for (int u = 0; u < QPT; u++) {
+              const int qi = tid + u * NT_DENSE;
+              if (qi < qlen) {
+                const int cc = s.q[qi];
+                const int j = cc % NB, il = (cc / NB) % IB, l = (cc / (NB * IB)) % NB;
+                const int k = (cc / (NB * IB * NB)) % NB, t = cc / (NB * IB * NB * NB);
+                const double w = s.om[il], op = s.Eii[j], odp = s.Ejj[t][k], otp = s.Ess[t][l];
+                const double delta = mismatch(ch, w, op, odp, otp);
+                double sigma = s.sig[t][k][l];
+                if (sigma <= (double)kClamp) sigma = 1.0 / sqrt(kPi);
+                const double wpv = wp4_of(ch, w, op, odp, otp, delta, sigma, a.T);
+                double g = 0.0;
+                if (!a.onlyharmonic) {
+                  const double vr = (double)V[u].re, vi = (double)V[u].im;
                   g = gpre * wpv * (vr * vr + vi * vi);
                 }
 #pragma unroll
-                for (int x = 0; x < IB; x++)
-                  if (x == il) {
-                    acc_g[x] += g;
-                    acc_w[x] += wpv;
-                    acc_n[x] += 1;
+                for (int xi = 0; xi < IB; xi++)
+                  if (xi == il) {
+                    acc_g[xi] += g;
+                    acc_w[xi] += wpv;
+                    acc_n[xi] += 1;
                   }
               }
+            }
+          };
+
+          // exact filter.  Lane task = (t, k, l, i), looping over j; each
+          // candidate is two FP64 additions in the reference's order,
+          // ((w +- w') +- w'') - w''', and one comparison with the exact sigma.
+          constexpr int NTASK = TQ * NB * NB * IB;
+          int qlen = 0;
+          for (int tb = 0; tb < NTASK; tb += NT_DENSE) {
+            const int task = tb + tid;
+            int mask = 0;
+            if (task < NTASK) {
+              const int il = task % IB, l = (task / IB) % NB, k = (task / (IB * NB)) % NB, t = task / (IB * NB * NB);
+              const double sg = s.sig[t][k][l];
+              if (sg >= 0.0 && s.act[il]) {
+                const int jj0 = s.jj[t], ss0 = s.ss[t];
+                const int i0 = ib * IB + il;
+                const double odp = s.Ejj[t][k], otp = s.Ess[t][l];
+                const bool exk = (ch == 1) && ((qi0 == jj0 && i0 == k) || (qi0 == ss0 && i0 == l));
+                if (!exk) {
+#pragma unroll
+                  for (int j = 0; j < NB; j++) {
+                    if (!(s.Eii[j] != 0.0)) continue;
+                    if (ch != 1 && jj0 == ii0 && k < j) continue;
+                    if (ch == 1 && ((ii0 == jj0 && j == k) || (ii0 == ss0 && j == l))) continue;
+                    const double x = s.x1[il][j];
+                    const double delta = ((ch == 0) ? (x + odp) : (x - odp)) - otp;
+                    if (fabs(delta) <= sg) mask |= 1 << j;
+                  }
+                }
+              }
+            }
+            int tot2;
+            const int pos2 = block_excl_scan<NW>(__popc(mask), s.wsum, tot2);
+            if (qlen + tot2 > QMAX) {
+              // queue full: contract what is queued, then continue
+              contract(qlen);
               qlen = 0;
               __syncthreads();
             }
+            int o = qlen + pos2;
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+              if ((mask >> j) & 1) s.q[o++] = (unsigned short)(task * NB + j);
+            qlen += tot2;
+            __syncthreads();
           }
+          if (qlen > 0) contract(qlen);
           __syncthreads();
         }
       }
@@ -476,20 +610,24 @@ __global__ void __launch_bounds__(256) k_dense(DenseArgs a) {
     // fixed-order block reduction of the per-thread partial sums
 #pragma unroll
     for (int il = 0; il < IB; il++) {
-      double g = wave_sum(acc_g[il]);
-      double w = wave_sum(acc_w[il]);
-      double n = wave_sum((double)acc_n[il]);
-      if ((tid & 63) == 0) {
-        s_red[tid >> 6][3 * il] = g;
-        s_red[tid >> 6][3 * il + 1] = w;
-        s_red[tid >> 6][3 * il + 2] = n;
+      const double g = wave_sum(acc_g[il]);
+      const double w = wave_sum(acc_w[il]);
+      const double n = wave_sum((double)acc_n[il]);
+      if (lane == 0) {
+        s.red[wave][3 * il] = g;
+        s.red[wave][3 * il + 1] = w;
+        s.red[wave][3 * il + 2] = n;
       }
     }
     __syncthreads();
     if (tid < IB) {
-      double g = ((s_red[0][3 * tid] + s_red[1][3 * tid]) + s_red[2][3 * tid]) + s_red[3][3 * tid];
-      double w = ((s_red[0][3 * tid + 1] + s_red[1][3 * tid + 1]) + s_red[2][3 * tid + 1]) + s_red[3][3 * tid + 1];
-      double n = ((s_red[0][3 * tid + 2] + s_red[1][3 * tid + 2]) + s_red[2][3 * tid + 2]) + s_red[3][3 * tid + 2];
+      double g = 0.0, w = 0.0, n = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < NW; wv++) {
+        g = g + s.red[wv][3 * tid];
+        w = w + s.red[wv][3 * tid + 1];
+        n = n + s.red[wv][3 * tid + 2];
+      }
       a.out_g[item * IB + tid] = g;
       a.out_w[item * IB + tid] = w;
       a.out_n[item * IB + tid] = (int)n;
@@ -616,6 +754,22 @@ void dense_destroy(DenseState *s) {
 
 int dense_ngroups(const DenseState *s) { return s ? s->ng : 0; }
 
+int dense_selftest_mfma(double *err) {
+  double *d = nullptr;
+  if (hipMalloc(&d, sizeof(double)) != hipSuccess) return derr(FP4_ERR_NOMEM, "selftest: hipMalloc failed");
+  hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, 0, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(err, d, sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? 0 : derr(FP4_ERR_DEVICE, "selftest: kernel failed");
+}
+
+void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib) {
+  *mfma = s ? s->last_mfma : 0;
+  *tq = s ? s->last_tq : 0;
+  *ib = s ? s->last_ib : 0;
+}
+
 void dense_info(const DenseState *s, long long *triples, long long *items, int *ngroups, int *nfine) {
   *triples = s ? s->last_triples : 0;
   *items = s ? s->last_items : 0;
@@ -625,46 +779,50 @@ void dense_info(const DenseState *s, long long *triples, long long *items, int *
 
 namespace {
 
-struct Shape {
-  int IB, TQ;
-};
+template <class T, int NB, int IB, int TQ>
+size_t static_bytes() {
+  return (sizeof(DenseSmem<T, NB, IB, TQ>) + 15) & ~size_t(15);
+}
 
 template <class T, int NB, int IB, int TQ>
-int launch_dense(DenseArgs &a, int slots, hipStream_t st) {
-  const size_t dyn = sizeof(C2<T>) * TQ * a.ng;
-  hipLaunchKernelGGL((k_dense<T, NB, IB, TQ>), dim3(slots), dim3(256), dyn, st, a);
-  return hipGetLastError() == hipSuccess ? 0 : derr(FP4_ERR_DEVICE, "dense kernel launch failed");
+size_t lds_bytes(int ng, int ntri) {
+  return static_bytes<T, NB, IB, TQ>() + sizeof(C2<T>) * ((size_t)ng * IB * NB * 9 + ntri + (size_t)TQ * ng);
 }
 
-// (precision, Nb) -> (modes per block, q'' per tile) under 64 KiB of LDS
-bool pick_shape(int prec, int nb, Shape &s) {
-  const bool f32 = prec == FP4_PREC_FP32;
-  switch (nb) {
-    case 3: s = {3, f32 ? 16 : 8}; return true;
-    case 6: s = {6, f32 ? 4 : 2}; return true;
-    case 9: s = {3, f32 ? 2 : 1}; return true;
-    case 12: s = {1, f32 ? 2 : 1}; return true;
-    default: return false;
-  }
+// One kernel shape: modes per block (IB) and q'' per tile (TQ).
+struct Shape {
+  int prec, nb, IB, TQ;
+  bool mfma;
+  size_t (*bytes)(int, int);
+  size_t (*stat)();
+  void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
+};
+
+template <class T, int NB, int IB, int TQ, bool MF>
+void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  hipLaunchKernelGGL((k_dense<T, NB, IB, TQ, MF>), dim3(slots), dim3(NT_DENSE), dyn, st, a);
 }
 
-int dispatch(int prec, int nb, DenseArgs &a, int slots, hipStream_t st) {
-  if (prec == FP4_PREC_FP32) {
-    switch (nb) {
-      case 3: return launch_dense<float, 3, 3, 16>(a, slots, st);
-      case 6: return launch_dense<float, 6, 6, 4>(a, slots, st);
-      case 9: return launch_dense<float, 9, 3, 2>(a, slots, st);
-      case 12: return launch_dense<float, 12, 1, 2>(a, slots, st);
-    }
-  } else {
-    switch (nb) {
-      case 3: return launch_dense<double, 3, 3, 8>(a, slots, st);
-      case 6: return launch_dense<double, 6, 6, 2>(a, slots, st);
-      case 9: return launch_dense<double, 9, 3, 1>(a, slots, st);
-      case 12: return launch_dense<double, 12, 1, 1>(a, slots, st);
-    }
-  }
-  return derr(FP4_ERR_UNSUPPORTED, "dense form: Nbands not supported");
+#define FP4_SHAPE(T, P, NB, IB, TQ, MF) \
+  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>}
+
+// In order of preference; the first shape whose LDS fits is used.  The FP32
+// shapes with 8 q'' per tile run the Psi GEMM on MFMA.
+const Shape kShapes[] = {
+    FP4_SHAPE(float, FP4_PREC_FP32, 3, 3, 8, true),          FP4_SHAPE(float, FP4_PREC_FP32, 6, 6, 8, true),
+    FP4_SHAPE(float, FP4_PREC_FP32, 6, 2, 8, true),          FP4_SHAPE(float, FP4_PREC_FP32, 9, 3, 8, true),
+    FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 8, true),         FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 4, false),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 3, 3, 4, false),  FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 3, 4, false),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 1, 4, false),  FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 9, 1, 2, false),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, false),
+};
+
+const Shape *pick_shape(int prec, int nb, int ng, int ntri, size_t lds_max) {
+  for (const Shape &s : kShapes)
+    if (s.prec == prec && s.nb == nb && s.bytes(ng, ntri) <= lds_max) return &s;
+  return nullptr;
 }
 
 int ensure(void *&p, size_t &bytes, size_t need) {
@@ -685,8 +843,15 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE)
     return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
   const int nb = r.m.nb;
-  Shape sh;
-  if (!pick_shape(r.precision, nb, sh)) return derr(FP4_ERR_UNSUPPORTED, "dense form: Nbands not supported");
+  int dev = 0, ldsmax = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const Shape *shp = pick_shape(r.precision, nb, s->ng, s->nc, (size_t)ldsmax);
+  if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
+  const Shape &sh = *shp;
+  s->last_mfma = sh.mfma;
+  s->last_tq = sh.TQ;
+  s->last_ib = sh.IB;
   const int nmodes_all = nb * r.m.nlist;
   std::vector<unsigned char> sel(nmodes_all, 0);
   std::vector<int> lls;
@@ -697,15 +862,11 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   }
   const int nls = (int)lls.size(), nib = nb / sh.IB;
   const long nitems = 3L * nls * nib * nii;
-  int dev = 0, ncu = 0;
-  (void)hipGetDevice(&dev);
+  int ncu = 0;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  int slots = (r.tuning && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu) * 2;
+  int slots = (r.tuning && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu);
   if (slots > nitems) slots = (int)std::max(1L, nitems);
-  const size_t tsz = (r.precision == FP4_PREC_FP32) ? sizeof(float) : sizeof(double);
-  const long slot_elems = (long)s->ng * sh.IB * nb * 9;
   int rc;
-  if ((rc = ensure(s->scratch, s->scratch_bytes, (size_t)slots * slot_elems * 2 * tsz))) return rc;
   const size_t per_item = sizeof(double) * 2 + sizeof(int);
   const long nseg = 3L * nls * nib * sh.IB;
   // outputs: og, ow, on per item*IB, then sg, sw, sn per segment, then lls and sel
@@ -755,14 +916,21 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.onlyharmonic = r.onlyharmonic;
   a.nitems = nitems;
   a.counter = s->counter;
-  a.scratch = s->scratch;
-  a.slot_elems = slot_elems;
+  a.ntri = s->nc;
+  a.smem_static = (int)sh.stat();
+  {
+    const int ng3[3] = {r.m.n1, r.m.n2, r.m.n3};
+    for (int nu = 0; nu < 3; nu++)
+      for (int c = 0; c < 3; c++) a.bf[3 * nu + c] = (float)(r.m.b[3 * nu + c] / ng3[nu]);
+    a.sbf = (float)std::fabs(r.m.scalebroad);
+  }
   a.out_g = og;
   a.out_w = ow;
   a.out_n = on;
   a.out_t = ot;
   if (hipEventRecord(s->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  if ((rc = dispatch(r.precision, nb, a, slots, st))) return rc;
+  sh.launch(a, slots, sh.bytes(s->ng, s->nc), st);
+  if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense kernel launch failed");
   if (hipEventRecord(s->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
   hipLaunchKernelGGL(k_dense_reduce, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, st, og, ow, on, nseg, sh.IB,
                      nii, sg, sw, sn);

@@ -40,6 +40,10 @@ int dense_ngroups(const DenseState *s);
 // Work of the last dense_rates call: visited triples and work items, and the
 // coarse / fine group counts of the clusters.
 void dense_info(const DenseState *s, long long *triples, long long *items, int *ngroups, int *nfine);
+// Kernel shape of the last dense_rates call: MFMA path, q'' per tile, modes per block.
+void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib);
+// Checks the MFMA lane layout the dense kernel assumes; *err = max abs error.
+int dense_selftest_mfma(double *err);
 
 struct DenseRun {
   ModeView m;

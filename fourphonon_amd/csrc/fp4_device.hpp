@@ -53,6 +53,8 @@ struct ModeView {
   double b[9];     // rlattvec(3,3) column-major
   double scalebroad;
   double omega_max;
+  float bf[9];     // rlattvec(:,nu)/ngrid(nu) in float, for the sigma estimate
+  float sbf;       // |scalebroad| in float
 };
 
 struct IfcView {
@@ -108,6 +110,24 @@ __device__ __forceinline__ double pair_sigma(const ModeView &m, int ch, int jj0,
     d[a] = (ch == 0) ? (-v3 + v4) : (v3 - v4);
   }
   return m.scalebroad * base_sigma(m, d[0], d[1], d[2]);
+}
+
+// Float estimate of sigma used to cull candidates: velocity differences
+// formed in FP64 and rounded once, then float dot products.  Its relative
+// error is below 1e-6, so any candidate whose |delta| lies outside
+// [0.9999, 1.0001] * estimate is decided by the estimate; the others take the
+// exact FP64 test.
+__device__ __forceinline__ float sigma_estimate(const ModeView &m, int ch, const double *v3, const double *v4) {
+  float d[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) d[a] = (float)((ch == 0) ? (-v3[a] + v4[a]) : (v3[a] - v4[a]));
+  float mx = 0.f;
+#pragma unroll
+  for (int nu = 0; nu < 3; nu++) {
+    const float x = m.bf[3 * nu] * d[0] + m.bf[3 * nu + 1] * d[1] + m.bf[3 * nu + 2] * d[2];
+    mx = fmaxf(mx, x * x);
+  }
+  return m.sbf * sqrtf(0.5f * mx);
 }
 
 __device__ __forceinline__ double mismatch(int ch, double w, double wp, double wdp, double wtp) {

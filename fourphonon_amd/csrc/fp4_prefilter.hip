@@ -11,9 +11,12 @@
 //   2. a conservative float cull: sigma <= vb(jj,k) + vb(ss,l) where
 //      vb = scalebroad * max_nu(|b_nu|/N_nu) * |v| / sqrt(2), padded so that
 //      float rounding can only keep a candidate, never drop one;
-//   3. the exact FP64 window test |delta| <= sigma with base_sigma computed
-//      in the reference's operation order.
-// Only step 3 decides, so counts and lists are bit-exact.
+//   3. a float estimate of sigma (relative error < 1e-6) that settles every
+//      candidate whose |delta| is not within 1e-4 of it;
+//   4. for the rest, the exact FP64 window test |delta| <= sigma with
+//      base_sigma computed in the reference's operation order.
+// Steps 2 and 3 only decide where the answer cannot differ from step 4, so
+// counts and lists are bit-exact.
 //
 // Count pass: per-item survivor count (int32) and phase-space partial sum.
 // Emit pass: per-lane counts, a wavefront exclusive scan in lane (= jj)
@@ -72,8 +75,21 @@ __device__ __forceinline__ void scan_candidates(const ModeView &m, int ch, int i
                 (ii0 != ss0 || j != l)))
             continue;
         }
-        const double sigma = pair_sigma(m, ch, jj0, k, ss0, l);
-        if (fabs(delta) <= sigma) f(j, k, l, delta, sigma);
+        double v3[3], v4[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          v3[a] = Vel(m, jj0, k, a);
+          v4[a] = Vel(m, ss0, l, a);
+        }
+        const float ad = (float)fabs(delta);
+        const float sf = sigma_estimate(m, ch, v3, v4);
+        if (ad > sf * 1.0001f + 1e-30f) continue;
+        if (ad < sf * 0.9999f) {
+          f(j, k, l, delta);
+        } else {
+          const double sigma = pair_sigma(m, ch, jj0, k, ss0, l);
+          if (fabs(delta) <= sigma) f(j, k, l, delta);
+        }
       }
     }
   }
@@ -119,8 +135,9 @@ __global__ void __launch_bounds__(256) k_prefilter(PrefilterArgs a) {
           for (int t = 0; t < 3; t++) qdp[t] = m.ijk[3 * jj0 + t];
           const int ss0 = partner(m, ch, q, qp, qdp);
           if (ch != 0 && ss0 < jj0) continue;
-          scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int, int, double delta, double sigma) {
+          scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int k, int l, double delta) {
             cnt += 1;
+            double sigma = pair_sigma(m, ch, jj0, k, ss0, l);
             if (sigma <= (double)kClamp) sigma = 1.0 / sqrt(kPi);
             const double num = exp(-(delta * delta) / (sigma * sigma));
             const double den = sigma * sqrt(kPi) * dn3 * dnb4;
@@ -151,12 +168,12 @@ __global__ void __launch_bounds__(256) k_prefilter(PrefilterArgs a) {
           ss0 = partner(m, ch, q, qp, qdp);
           if (ch != 0 && ss0 < jj0) ok = false;
         }
-        if (ok) scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int, int, double, double) { c++; });
+        if (ok) scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int, int, double) { c++; });
         const int off = wave_excl_scan(c, lane);
         const int tot = wave_sum_i(c);
         if (ok && c > 0) {
           long p = base + off;
-          scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int j, int k, int l, double, double) {
+          scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int j, int k, int l, double) {
             const long v2 = (long)ii0 * NB + j + 1, v3 = (long)jj0 * NB + k + 1, v4 = (long)ss0 * NB + l + 1;
             if (MODE == 1) {
               a.out32[0][p] = (int)v2;

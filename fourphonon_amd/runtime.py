@@ -34,7 +34,8 @@ EXPORTS = [
     "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
     "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_ind_only_4ph_modes", "fp4_ctx_kernel_stats",
-    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info",
+    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape",
+    "fp4_selftest_mfma",
 ]
 
 
@@ -72,6 +73,13 @@ def _p(a):
 def _check(rc):
     if rc != 0:
         raise FP4Error(rc, load().fp4_last_error().decode())
+
+
+def selftest_mfma(device: int = 0) -> float:
+    """Max abs error of a 16x16x4 FP32 MFMA against the exact product."""
+    err = C.c_double()
+    _check(load().fp4_selftest_mfma(C.c_int32(device), C.byref(err)))
+    return err.value
 
 
 class Context:
@@ -230,6 +238,12 @@ class Context:
         g, f = C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_info(self.h, C.byref(t), C.byref(it), C.byref(g), C.byref(f)))
         return {"triples": t.value, "items": it.value, "ngroups": g.value, "nfine": f.value}
+
+    def dense_shape(self):
+        """-> dict(mfma, tq, ib) of the last dense-form call."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load().fp4_ctx_dense_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"mfma": bool(a.value), "tq": b.value, "ib": c.value}
 
     def set_batch_capacity(self, processes):
         _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))

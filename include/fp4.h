@@ -156,6 +156,13 @@ int fp4_ctx_reset_stats(fp4_ctx *ctx);
    call: visited (q, q', q'') triples, work items, and the number of coarse
    (K, L, R_k-R_l) and fine (I, J, K, L, R_k-R_l) cluster groups. */
 int fp4_ctx_dense_info(fp4_ctx *ctx, int64_t *triples, int64_t *items, int32_t *ngroups, int32_t *nfine);
+/* Diagnostics; no counterpart in the reference.  Shape of the last dense
+   kernel: MFMA path (1) or VALU path (0), q'' per tile, modes per block. */
+int fp4_ctx_dense_shape(fp4_ctx *ctx, int32_t *mfma, int32_t *tq, int32_t *ib);
+/* Diagnostics; no counterpart in the reference.  Runs one 16x16x4 FP32 MFMA
+   with known operands and returns the largest deviation from the exact
+   product, checking the lane layout the dense kernel relies on. */
+int fp4_selftest_mfma(int32_t device, double *max_err);
 /* Upper bound on the processes materialised per batch (default 2^26); no
    counterpart in the reference, which materialises every list at once. */
 int fp4_ctx_set_batch_capacity(fp4_ctx *ctx, int64_t processes);

@@ -37,6 +37,11 @@ def kappa_rta(case, rate):
     return tconduct(case, F).sum(axis=0)
 
 
+def test_mfma_lane_layout():
+    """The dense kernel's MFMA operand/result layout, checked on the device."""
+    assert rt.selftest_mfma() == 0.0
+
+
 @pytest.fixture(scope="module", params=[1, 2], ids=["shell1", "shell2"])
 def pair(request):
     from oracle.oracle import Oracle
@@ -72,7 +77,8 @@ def test_dense_fp32_tolerances(pair):
     ref = c.RTA_driver_4ph(rt.PREC_FP64)
     got = c.RTA_driver_4ph(rt.PREC_FP32)
     err = rel_err(got["rate_scatt_4ph"], ref["rate_scatt_4ph"])
-    print(f"per-mode rate max rel err {err:.3e}")
+    print(f"per-mode rate max rel err {err:.3e}, shape {c.dense_shape()}")
+    assert c.dense_shape()["mfma"]
     assert err <= 1e-3
     k64 = kappa_rta(case, ref["rate_scatt_4ph"])
     k32 = kappa_rta(case, got["rate_scatt_4ph"])
