@@ -70,6 +70,24 @@ def dense_flops(case, ngroups, nfine, ntriples, nproc, nitems):
     return per_triple * ntriples + per_proc * nproc + per_item * nitems
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy device-to-device of nbytes, so that a counter run can
+    scale FETCH_SIZE / WRITE_SIZE by a kernel of known traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: {nbytes} bytes device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,10 +95,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_bytes > 0:
+        copy_calibration(args.copy_bytes)
 
     from fourphonon_amd import crystal
     from fourphonon_amd import runtime as rt
@@ -121,9 +143,11 @@ def main():
     flops = dense_flops(case, info["ngroups"], info["nfine"], ntriples, nproc_total, info["items"])
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
+    shape = ctx.dense_shape()
+    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{'mfma' if shape['mfma'] else 'valu'}>"
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_dense<float,6,6,4>", "kernel_ms": round(avg_ms, 3),
+                "kernel": kname, "kernel_ms": round(avg_ms, 3), "launches": k_n,
                 "flops_per_launch": flops}
 
     out = {

@@ -1,0 +1,171 @@
+! A small four-phonon RTA driver in the shape of ShengBTE.f90's 4ph steps
+! (Src/ShengBTE.f90:473-508, 960-964 / 988-1008, 1100-1125), calling the
+! drop-in procedures of module fp4_processes.
+!
+! Usage: fp4_driver <input dir> <output dir> [fp64|dense64|fp32] [list]
+!
+! The input directory holds meta.txt and raw little-endian arrays in the
+! reference's memory order (written by fourphonon_amd.fortran.io).  The
+! driver writes BTE.Numprocess_4ph and BTE.w_4ph* text files in the
+! reference's formats, plus raw copies of counts, rates and (with "list")
+! the process lists for machine comparison.
+program fp4_driver
+  use fp4_processes
+  implicit none
+  character(len=1024) :: indir, outdir, mode, listflag
+  integer(kind=4) :: ngrid(3), nbands, natoms, nelements, nlist, ntri, onlyh, nptk, nm, i, ll
+  real(kind=8) :: T, scalebroad, omega_max, rlattvec(3,3)
+  real(kind=8), allocatable :: masses(:), energy(:,:), velocity(:,:,:), Phi(:,:,:,:,:), R_j(:,:), R_k(:,:), R_l(:,:)
+  integer(kind=4), allocatable :: types(:), IJK(:,:), List(:), Index_i(:), Index_j(:), Index_k(:), Index_l(:)
+  complex(kind=8), allocatable :: eigenvect(:,:,:)
+  integer(kind=8), allocatable :: Npp(:), Npm(:), Nmm(:)
+  real(kind=8), allocatable :: Ppp(:,:), Ppm(:,:), Pmm(:,:)
+  real(kind=8), allocatable :: r4(:,:), rpp(:,:), rpm(:,:), rmm(:,:), wpp(:,:), wpm(:,:), wmm(:,:)
+  integer(kind=8), allocatable :: a2(:), a3(:), a4(:), b2(:), b3(:), b4(:), c2(:), c3(:), c4(:)
+  integer :: u
+
+  call get_command_argument(1, indir)
+  call get_command_argument(2, outdir)
+  mode = "dense64"
+  listflag = ""
+  if (command_argument_count() >= 3) call get_command_argument(3, mode)
+  if (command_argument_count() >= 4) call get_command_argument(4, listflag)
+
+  open(newunit=u, file=trim(indir)//"/meta.txt", status="old", action="read")
+  read(u, *) ngrid, nbands, natoms, nelements, nlist, ntri
+  read(u, *) T, scalebroad, omega_max, onlyh
+  read(u, *) rlattvec
+  allocate(masses(nelements), types(natoms))
+  read(u, *) masses
+  read(u, *) types
+  close(u)
+  nptk = ngrid(1) * ngrid(2) * ngrid(3)
+  nm = nbands * nlist
+  allocate(energy(nptk, nbands), velocity(nptk, nbands, 3), eigenvect(nptk, nbands, nbands))
+  allocate(IJK(3, nptk), List(nlist))
+  allocate(Phi(3, 3, 3, 3, ntri), R_j(3, ntri), R_k(3, ntri), R_l(3, ntri))
+  allocate(Index_i(ntri), Index_j(ntri), Index_k(ntri), Index_l(ntri))
+  call read_r8(trim(indir)//"/energy.bin", energy, size(energy))
+  call read_r8(trim(indir)//"/velocity.bin", velocity, size(velocity))
+  call read_c8(trim(indir)//"/eigenvect.bin", eigenvect, size(eigenvect))
+  call read_i4(trim(indir)//"/IJK.bin", IJK, size(IJK))
+  call read_i4(trim(indir)//"/List.bin", List, size(List))
+  call read_r8(trim(indir)//"/Phi.bin", Phi, size(Phi))
+  call read_r8(trim(indir)//"/R_j.bin", R_j, size(R_j))
+  call read_r8(trim(indir)//"/R_k.bin", R_k, size(R_k))
+  call read_r8(trim(indir)//"/R_l.bin", R_l, size(R_l))
+  call read_i4(trim(indir)//"/Index_i.bin", Index_i, size(Index_i))
+  call read_i4(trim(indir)//"/Index_j.bin", Index_j, size(Index_j))
+  call read_i4(trim(indir)//"/Index_k.bin", Index_k, size(Index_k))
+  call read_i4(trim(indir)//"/Index_l.bin", Index_l, size(Index_l))
+
+  call fp4_init(ngrid, nbands, natoms, rlattvec, masses, types, T, scalebroad, omega_max, onlyh /= 0)
+  select case (trim(mode))
+  case ("fp64")
+     call fp4_set_precision(FP4_PREC_FP64)
+  case ("fp32")
+     call fp4_set_precision(FP4_PREC_FP32)
+  case default
+     call fp4_set_precision(FP4_PREC_FP64_DENSE)
+  end select
+
+  ! counts and phase space (Src/ShengBTE.f90:473-508)
+  allocate(Npp(nm), Npm(nm), Nmm(nm), Ppp(nbands, nlist), Ppm(nbands, nlist), Pmm(nbands, nlist))
+  call NP_driver_4ph(energy, velocity, nlist, List, IJK, Npp, Ppp, Npm, Ppm, Nmm, Pmm)
+  open(newunit=u, file=trim(outdir)//"/BTE.Numprocess_4ph", status="replace")
+  do i = 1, nbands
+     do ll = 1, nlist
+        write(u, "(1X,I0,1X,I0,1X,I0)") Npp(ll + nlist * (i - 1)), Npm(ll + nlist * (i - 1)), Nmm(ll + nlist * (i - 1))
+     end do
+  end do
+  close(u)
+  call write_i8(trim(outdir)//"/N.bin", [Npp, Npm, Nmm])
+
+  if (trim(listflag) == "list") then
+     allocate(a2(max(1_8, sum(Npp))), a3(max(1_8, sum(Npp))), a4(max(1_8, sum(Npp))))
+     allocate(b2(max(1_8, sum(Npm))), b3(max(1_8, sum(Npm))), b4(max(1_8, sum(Npm))))
+     allocate(c2(max(1_8, sum(Nmm))), c3(max(1_8, sum(Nmm))), c4(max(1_8, sum(Nmm))))
+     call Ind_only_driver_4ph(energy, velocity, nlist, List, IJK, Npp, Npm, Nmm, a2, a3, a4, b2, b3, b4, c2, c3, c4)
+     call write_i8(trim(outdir)//"/lists_pp.bin", [a2(1:sum(Npp)), a3(1:sum(Npp)), a4(1:sum(Npp))])
+     call write_i8(trim(outdir)//"/lists_pm.bin", [b2(1:sum(Npm)), b3(1:sum(Npm)), b4(1:sum(Npm))])
+     call write_i8(trim(outdir)//"/lists_mm.bin", [c2(1:sum(Nmm)), c3(1:sum(Nmm)), c4(1:sum(Nmm))])
+  end if
+
+  ! RTA rates (Src/ShengBTE.f90:960-964) and BTE.w_4ph* (Src/ShengBTE.f90:1105-1125)
+  allocate(r4(nbands, nlist), rpp(nbands, nlist), rpm(nbands, nlist), rmm(nbands, nlist))
+  allocate(wpp(nbands, nlist), wpm(nbands, nlist), wmm(nbands, nlist))
+  call RTA_driver_4ph(energy, velocity, eigenvect, nlist, List, IJK, ntri, Phi, R_j, R_k, R_l, &
+       Index_i, Index_j, Index_k, Index_l, r4, rpp, rpm, rmm, wpp, wpm, wmm)
+  call write_w(trim(outdir)//"/BTE.w_4ph", r4)
+  call write_w(trim(outdir)//"/BTE.w_4ph_plusplus", rpp)
+  call write_w(trim(outdir)//"/BTE.w_4ph_plusminus", rpm)
+  call write_w(trim(outdir)//"/BTE.w_4ph_minusminus", rmm)
+  call write_r8(trim(outdir)//"/rates.bin", [reshape(r4, [nm]), reshape(rpp, [nm]), reshape(rpm, [nm]), &
+       reshape(rmm, [nm]), reshape(wpp, [nm]), reshape(wpm, [nm]), reshape(wmm, [nm])])
+  call fp4_finalize()
+  write(*, "(a,i0,a)") "fp4_driver: ", sum(Npp) + sum(Npm) + sum(Nmm), " processes"
+
+contains
+
+  subroutine write_w(path, rate)
+    character(len=*), intent(in) :: path
+    real(kind=8), intent(in) :: rate(:,:)
+    integer :: v, ii, l2
+    open(newunit=v, file=path, status="replace")
+    do ii = 1, nbands
+       do l2 = 1, nlist
+          write(v, "(2E20.10)") energy(List(l2), ii), rate(ii, l2)
+       end do
+    end do
+    close(v)
+  end subroutine write_w
+
+  subroutine read_r8(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    real(kind=8), intent(out) :: a(n)
+    integer :: v
+    open(newunit=v, file=path, access="stream", form="unformatted", status="old", action="read")
+    read(v) a
+    close(v)
+  end subroutine read_r8
+
+  subroutine read_c8(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    complex(kind=8), intent(out) :: a(n)
+    integer :: v
+    open(newunit=v, file=path, access="stream", form="unformatted", status="old", action="read")
+    read(v) a
+    close(v)
+  end subroutine read_c8
+
+  subroutine read_i4(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    integer(kind=4), intent(out) :: a(n)
+    integer :: v
+    open(newunit=v, file=path, access="stream", form="unformatted", status="old", action="read")
+    read(v) a
+    close(v)
+  end subroutine read_i4
+
+  subroutine write_i8(path, a)
+    character(len=*), intent(in) :: path
+    integer(kind=8), intent(in) :: a(:)
+    integer :: v
+    open(newunit=v, file=path, access="stream", form="unformatted", status="replace")
+    write(v) a
+    close(v)
+  end subroutine write_i8
+
+  subroutine write_r8(path, a)
+    character(len=*), intent(in) :: path
+    real(kind=8), intent(in) :: a(:)
+    integer :: v
+    open(newunit=v, file=path, access="stream", form="unformatted", status="replace")
+    write(v) a
+    close(v)
+  end subroutine write_r8
+
+end program fp4_driver
