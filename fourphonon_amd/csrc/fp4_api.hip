@@ -21,6 +21,7 @@
 
 #include "../../include/fp4.h"
 #include "fp4_dense.hpp"
+#include "fp4_iterate.hpp"
 #include "fp4_kernels.hpp"
 
 using namespace fp4;
@@ -71,7 +72,7 @@ struct DevBuf {
   }
 };
 
-enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_N = 4 };
+enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_ITER = 4, KS_N = 5 };
 
 }  // namespace
 
@@ -97,11 +98,12 @@ struct fp4_ctx {
   // scratch
   DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g;
   DenseState *dense = nullptr;
+  BteState *bte = nullptr;
   long long list_cap = 1LL << 26;   // processes per batch
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double ks_ms[KS_N] = {0, 0, 0, 0};
-  long long ks_n[KS_N] = {0, 0, 0, 0};
+  double ks_ms[KS_N] = {0, 0, 0, 0, 0};
+  long long ks_n[KS_N] = {0, 0, 0, 0, 0};
   DenseTuning dense_tuning;
 };
 
@@ -423,6 +425,8 @@ int fp4_ctx_destroy(fp4_ctx *c) {
   for (DevBuf *b : bufs) b->release();
   dense_destroy(c->dense);
   c->dense = nullptr;
+  bte_destroy(c->bte);
+  c->bte = nullptr;
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -855,6 +859,149 @@ int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
   }
+  return FP4_OK;
+}
+
+// ---- iterative solution ---------------------------------------------------
+static BteModes bte_modes(const fp4_ctx *c) {
+  BteModes m;
+  m.omega = c->energy.as<double>();
+  m.vel = c->vel.as<double>();
+  m.nptk = c->nptk;
+  m.nb = c->nb;
+  m.nlist = c->nlist;
+  m.stream = c->stream;
+  return m;
+}
+
+static int bte_ready(fp4_ctx *c) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!c->bte) {
+    c->bte = bte_create();
+    if (!c->bte) return fail(FP4_ERR_DEVICE, "event creation failed");
+  }
+  return 0;
+}
+
+#define BTE_CALL(x)                                      \
+  do {                                                   \
+    const int r_ = (x);                                  \
+    if (r_) return fail(r_, "%s", bte_error());          \
+  } while (0)
+
+int fp4_ctx_set_symmetry(fp4_ctx *c, int32_t nsymm_rot, const int32_t *ID_equi, const double *symmetrizers,
+                         const double *crotations) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  BTE_CALL(bte_set_symmetry(c->bte, bte_modes(c), nsymm_rot, ID_equi, symmetrizers, crotations));
+  return FP4_OK;
+}
+
+int fp4_iteration0(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                   const double *tau_zero, double *F_n) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (nlist != c->nlist) return fail(FP4_ERR_ARG, "Nlist=%d differs from the modes' %d", nlist, c->nlist);
+  const BteModes m = bte_modes(c);
+  BTE_CALL(bte_set_orbits(c->bte, m, Nequi, ALLEquiList, nullptr));
+  BTE_CALL(bte_iteration0(c->bte, m, tau_zero));
+  BTE_CALL(bte_get_F(c->bte, m, F_n));
+  return FP4_OK;
+}
+
+int fp4_bte_set_operator(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                         const int32_t *TypeofSymmetry, const int32_t *N_plus, const int32_t *N_minus,
+                         int32_t Ntotal_plus, int32_t Ntotal_minus, const int64_t *N_pp, const int64_t *N_pm,
+                         const int64_t *N_mm, int64_t Ntotal_pp, int64_t Ntotal_pm, int64_t Ntotal_mm,
+                         const int32_t *p2, const int32_t *p3, const int32_t *m2, const int32_t *m3,
+                         const int64_t *a2, const int64_t *a3, const int64_t *a4, const int64_t *b2,
+                         const int64_t *b3, const int64_t *b4, const int64_t *c2, const int64_t *c3,
+                         const int64_t *c4, const double *G_plus, const double *G_minus, const double *G_pp,
+                         const double *G_pm, const double *G_mm) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (nlist != c->nlist) return fail(FP4_ERR_ARG, "Nlist=%d differs from the modes' %d", nlist, c->nlist);
+  if (!TypeofSymmetry) return fail(FP4_ERR_ARG, "null TypeofSymmetry");
+  if (!N_pp || !N_pm || !N_mm) return fail(FP4_ERR_ARG, "null 4ph counts");
+  const int nm = c->nb * c->nlist;
+  // the stated list lengths must match the counts
+  const int32_t *N3[2] = {N_plus, N_minus};
+  const long long tot3[2] = {Ntotal_plus, Ntotal_minus};
+  for (int t = 0; t < 2; t++) {
+    long long s = 0;
+    if (N3[t])
+      for (int mm = 0; mm < nm; mm++) s += N3[t][mm];
+    if (s != tot3[t]) return fail(FP4_ERR_ARG, "3ph counts sum to %lld, Ntotal is %lld", s, tot3[t]);
+  }
+  const int64_t *N4[3] = {N_pp, N_pm, N_mm};
+  const long long tot4[3] = {Ntotal_pp, Ntotal_pm, Ntotal_mm};
+  for (int t = 0; t < 3; t++) {
+    long long s = 0;
+    for (int mm = 0; mm < nm; mm++) s += N4[t][mm];
+    if (s != tot4[t]) return fail(FP4_ERR_ARG, "4ph counts (channel %d) sum to %lld, Ntotal is %lld", t, s, tot4[t]);
+  }
+  const BteModes m = bte_modes(c);
+  const int32_t *L3[2][2] = {{p2, p3}, {m2, m3}};
+  const double *G3[2] = {G_plus, G_minus};
+  const int64_t *L4[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
+  const double *G4[3] = {G_pp, G_pm, G_mm};
+  BTE_CALL(bte_set_orbits(c->bte, m, Nequi, ALLEquiList, TypeofSymmetry));
+  BTE_CALL(bte_set_operator(c->bte, m, N3, L3, G3, N4, L4, G4));
+  return FP4_OK;
+}
+
+int fp4_bte_iterate(fp4_ctx *c, const double *tau_zero, double *F_n) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  const BteModes m = bte_modes(c);
+  BTE_CALL(bte_put_F(c->bte, m, F_n));
+  BTE_CALL(bte_sweep(c->bte, m, tau_zero));
+  c->ks_ms[KS_ITER] += bte_last_ms(c->bte);
+  c->ks_n[KS_ITER] += bte_last_launches(c->bte);
+  BTE_CALL(bte_get_F(c->bte, m, F_n));
+  return FP4_OK;
+}
+
+int fp4_iteration_4ph(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                      const int32_t *TypeofSymmetry, const int32_t *N_plus, const int32_t *N_minus,
+                      int32_t Ntotal_plus, int32_t Ntotal_minus, const int64_t *N_pp, const int64_t *N_pm,
+                      const int64_t *N_mm, int64_t Ntotal_pp, int64_t Ntotal_pm, int64_t Ntotal_mm,
+                      const int32_t *p2, const int32_t *p3, const int32_t *m2, const int32_t *m3, const int64_t *a2,
+                      const int64_t *a3, const int64_t *a4, const int64_t *b2, const int64_t *b3, const int64_t *b4,
+                      const int64_t *c2, const int64_t *c3, const int64_t *c4, const double *G_plus,
+                      const double *G_minus, const double *G_pp, const double *G_pm, const double *G_mm,
+                      const double *tau_zero, double *F_n) {
+  int rc;
+  if ((rc = fp4_bte_set_operator(c, nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus,
+                                 Ntotal_minus, N_pp, N_pm, N_mm, Ntotal_pp, Ntotal_pm, Ntotal_mm, p2, p3, m2, m3, a2,
+                                 a3, a4, b2, b3, b4, c2, c3, c4, G_plus, G_minus, G_pp, G_pm, G_mm)))
+    return rc;
+  return fp4_bte_iterate(c, tau_zero, F_n);
+}
+
+int fp4_tconduct(fp4_ctx *c, double V, const double *F_n, double *kappa, double *mode) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  const BteModes m = bte_modes(c);
+  BTE_CALL(bte_put_F(c->bte, m, F_n));
+  BTE_CALL(bte_tconduct(c->bte, m, c->T, V, kappa, mode));
+  return FP4_OK;
+}
+
+int fp4_bte_solve(fp4_ctx *c, double V, const double *tau_zero, int32_t maxiter, double eps, double *F_n,
+                  double *kappa, double *kappa_iter, double *relchange, int32_t *niter) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (!F_n || !niter) return fail(FP4_ERR_ARG, "null output");
+  if (c->nb > 64) return fail(FP4_ERR_UNSUPPORTED, "Nbands > 64");
+  const BteModes m = bte_modes(c);
+  int n = 0;
+  BTE_CALL(bte_solve(c->bte, m, c->T, V, tau_zero, maxiter, eps, kappa, kappa_iter, relchange, &n));
+  c->ks_ms[KS_ITER] += bte_last_ms(c->bte);
+  c->ks_n[KS_ITER] += bte_last_launches(c->bte);
+  *niter = n;
+  BTE_CALL(bte_get_F(c->bte, m, F_n));
   return FP4_OK;
 }
 

@@ -35,7 +35,8 @@ EXPORTS = [
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
     "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_ind_only_4ph_modes", "fp4_ctx_kernel_stats",
     "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape",
-    "fp4_selftest_mfma",
+    "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
+    "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve",
 ]
 
 
@@ -113,6 +114,7 @@ class Context:
                                   _p(np.asfortranarray(f.R_l, np.float64)),
                                   *[_p(np.ascontiguousarray(x, np.int32))
                                     for x in (f.Index_i, f.Index_j, f.Index_k, f.Index_l)]))
+        self.set_symmetry()
 
     def close(self):
         if getattr(self, "h", None):
@@ -220,8 +222,94 @@ class Context:
     def Vp_mm(self, rows):
         return self.Vp(2, rows)
 
+    # ------------------------------------------------------------ iterations
+    def set_symmetry(self):
+        """Uploads Nsymm_rot, ID_equi, symmetrizers and crotations of the case's modes."""
+        m = self.case.modes
+        _check(load().fp4_ctx_set_symmetry(self.h, C.c_int32(m.nsymm_rot),
+                                           _p(np.asfortranarray(m.ID_equi, np.int32)),
+                                           _p(np.asfortranarray(m.symmetrizers, np.float64)),
+                                           _p(np.asfortranarray(m.crotations, np.float64))))
+
+    def _orbits(self):
+        m = self.case.modes
+        return (np.ascontiguousarray(m.Nequi, np.int32), np.asfortranarray(m.ALLEquiList, np.int32),
+                np.asfortranarray(m.TypeofSymmetry, np.int32))
+
+    def iteration0(self, tau_zero):
+        """iteration0 -> F_n (Nb,nptk,3) F-order."""
+        Nequi, alleq, _ = self._orbits()
+        tau = np.asfortranarray(tau_zero, np.float64).reshape((self.nb, self.nlist), order="F")
+        F = np.zeros((self.nb, self.nptk, 3), order="F")
+        _check(load().fp4_iteration0(self.h, C.c_int32(self.nlist), _p(Nequi), _p(alleq), _p(tau), _p(F)))
+        return F
+
+    def _operator_args(self, N, lists, gammas, N3=None, lists3=None, gammas3=None):
+        nm = self.nmodes
+        Nequi, alleq, types = self._orbits()
+        N4 = [np.ascontiguousarray(n, np.int64) for n in N]
+        L4 = [[np.ascontiguousarray(x, np.int64) if len(x) else np.zeros(1, np.int64) for x in ch] for ch in lists]
+        G4 = [np.ascontiguousarray(g, np.float64) if len(g) else np.zeros(1) for g in gammas]
+        if N3 is None:
+            N3 = [np.zeros(nm, np.int32), np.zeros(nm, np.int32)]
+            lists3 = [[np.zeros(1, np.int32)] * 2] * 2
+            gammas3 = [np.zeros(1), np.zeros(1)]
+        N3 = [np.ascontiguousarray(n, np.int32) for n in N3]
+        L3 = [[np.ascontiguousarray(x, np.int32) if len(x) else np.zeros(1, np.int32) for x in ch] for ch in lists3]
+        G3 = [np.ascontiguousarray(g, np.float64) if len(g) else np.zeros(1) for g in gammas3]
+        keep = (Nequi, alleq, types, N4, L4, G4, N3, L3, G3)
+        args = [C.c_int32(self.nlist), _p(Nequi), _p(alleq), _p(types), _p(N3[0]), _p(N3[1]),
+                C.c_int32(int(N3[0].sum())), C.c_int32(int(N3[1].sum())), *[_p(n) for n in N4],
+                *[C.c_int64(int(n.sum())) for n in N4], *[_p(x) for ch in L3 for x in ch],
+                *[_p(x) for ch in L4 for x in ch], _p(G3[0]), _p(G3[1]), *[_p(g) for g in G4]]
+        return keep, args
+
+    def bte_set_operator(self, N, lists, gammas, N3=None, lists3=None, gammas3=None):
+        """Uploads the scattering operator (4ph counts/lists/Gamma, optional 3ph ones)."""
+        keep, args = self._operator_args(N, lists, gammas, N3, lists3, gammas3)
+        _check(load().fp4_bte_set_operator(self.h, *args))
+        del keep
+
+    def iteration_4ph(self, N, lists, gammas, tau_zero, F_n, N3=None, lists3=None, gammas3=None):
+        """iteration_4ph on a copy of F_n; returns the updated F_n."""
+        keep, args = self._operator_args(N, lists, gammas, N3, lists3, gammas3)
+        tau = np.asfortranarray(tau_zero, np.float64).reshape((self.nb, self.nlist), order="F")
+        F = np.array(F_n, dtype=np.float64, order="F", copy=True)
+        _check(load().fp4_iteration_4ph(self.h, *args, _p(tau), _p(F)))
+        del keep
+        return F
+
+    def bte_iterate(self, tau_zero, F_n):
+        tau = np.asfortranarray(tau_zero, np.float64).reshape((self.nb, self.nlist), order="F")
+        F = np.array(F_n, dtype=np.float64, order="F", copy=True)
+        _check(load().fp4_bte_iterate(self.h, _p(tau), _p(F)))
+        return F
+
+    def TConduct(self, F_n, V=None, mode=False):
+        """-> kappa (Nb,3,3) [, kappa_mode (nptk,Nb,3,3)] in W/(m K)."""
+        V = self.case.structure.volume if V is None else V
+        F = np.asfortranarray(F_n, np.float64)
+        kappa = np.zeros((self.nb, 3, 3), order="F")
+        km = np.zeros((self.nptk, self.nb, 3, 3), order="F") if mode else None
+        _check(load().fp4_tconduct(self.h, C.c_double(V), _p(F), _p(kappa), _p(km) if mode else None))
+        return (kappa, km) if mode else kappa
+
+    def bte_solve(self, tau_zero, maxiter=1000, eps=float(np.float32(1e-5)), V=None):
+        """Iterative solution with the current operator -> dict(F_n, kappa, kappa_iter, relchange, niter)."""
+        V = self.case.structure.volume if V is None else V
+        tau = np.asfortranarray(tau_zero, np.float64).reshape((self.nb, self.nlist), order="F")
+        F = np.zeros((self.nb, self.nptk, 3), order="F")
+        kappa = np.zeros((self.nb, 3, 3), order="F")
+        hist = np.zeros((3, 3, maxiter + 1), order="F")
+        rel = np.zeros(maxiter + 1)
+        n = C.c_int32()
+        _check(load().fp4_bte_solve(self.h, C.c_double(V), _p(tau), C.c_int32(maxiter), C.c_double(eps), _p(F),
+                                    _p(kappa), _p(hist), _p(rel), C.byref(n)))
+        k = n.value
+        return {"F_n": F, "kappa": kappa, "kappa_iter": hist[:, :, :k + 1], "relchange": rel[:k + 1], "niter": k}
+
     # ------------------------------------------------------------ diagnostics
-    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE = 0, 1, 2, 3
+    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE, KS_ITER = 0, 1, 2, 3, 4
 
     def kernel_stats(self, kind):
         ms = C.c_double()

@@ -146,10 +146,87 @@ int fp4_ind_only_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, i
                            int64_t *Indof4thPhonon_pm, int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm,
                            int64_t *Indof4thPhonon_mm);
 
+/* ---- iterative solution (Src/iterations.f90, Src/conductivity.f90) ------
+   F_n(Nb,nptk,3) and the scattering operator stay on the device between
+   calls; the functions below take and return the reference's host arrays. */
+
+/* The symmetry data the iterative solver reads as module globals:
+   Nsymm_rot, ID_equi(Nsymm_rot,nptk) as built by symmetry_map
+   (Src/config.f90:409-431, -1 where a rotation leaves the grid),
+   symmetrizers(3,3,nptk) (Src/ShengBTE.f90:1288-1290) and
+   crotations(3,3,Nsymm_rot) (symmetrize_tensor, Src/config.f90:503-518).
+   Needs fp4_ctx_set_modes first. */
+int fp4_ctx_set_symmetry(fp4_ctx *ctx, int32_t nsymm_rot, const int32_t *ID_equi, const double *symmetrizers,
+                         const double *crotations);
+
+/* iteration0, Src/iterations.f90:13-30: F_n = tau_zero*v*omega on every
+   star member.  Nequi(Nlist), ALLEquiList(Nsymm_rot,nptk),
+   tau_zero(Nb,Nlist); F_n(Nb,nptk,3) is overwritten. */
+int fp4_iteration0(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                   const double *tau_zero, double *F_n);
+
+/* The arguments of iteration_4ph (Src/iterations.f90:95-120) that define
+   the scattering operator, uploaded once and kept on the device: orbits,
+   3ph counts N_plus/N_minus(Nlist*Nb) int32 with their int32 lists, 4ph
+   counts (int64) with their int64 lists, and the per-process Gamma arrays.
+   Ntotal_* are the list lengths and must equal the sums of the counts.  3ph
+   pointers may be NULL when the 3ph totals are 0. */
+int fp4_bte_set_operator(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                         const int32_t *TypeofSymmetry, const int32_t *N_plus, const int32_t *N_minus,
+                         int32_t Ntotal_plus, int32_t Ntotal_minus, const int64_t *N_pp, const int64_t *N_pm,
+                         const int64_t *N_mm, int64_t Ntotal_pp, int64_t Ntotal_pm, int64_t Ntotal_mm,
+                         const int32_t *Indof2ndPhonon_plus, const int32_t *Indof3rdPhonon_plus,
+                         const int32_t *Indof2ndPhonon_minus, const int32_t *Indof3rdPhonon_minus,
+                         const int64_t *Indof2ndPhonon_pp, const int64_t *Indof3rdPhonon_pp,
+                         const int64_t *Indof4thPhonon_pp, const int64_t *Indof2ndPhonon_pm,
+                         const int64_t *Indof3rdPhonon_pm, const int64_t *Indof4thPhonon_pm,
+                         const int64_t *Indof2ndPhonon_mm, const int64_t *Indof3rdPhonon_mm,
+                         const int64_t *Indof4thPhonon_mm, const double *Gamma_plus, const double *Gamma_minus,
+                         const double *Gamma_pp, const double *Gamma_pm, const double *Gamma_mm);
+
+/* iteration_4ph, Src/iterations.f90:95-215, with the reference's argument
+   order (omega and velocity come from the context): fp4_bte_set_operator
+   followed by fp4_bte_iterate.  F_n(Nb,nptk,3) is updated in place, in the
+   reference's (ll, i, kk) order. */
+int fp4_iteration_4ph(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                      const int32_t *TypeofSymmetry, const int32_t *N_plus, const int32_t *N_minus,
+                      int32_t Ntotal_plus, int32_t Ntotal_minus, const int64_t *N_pp, const int64_t *N_pm,
+                      const int64_t *N_mm, int64_t Ntotal_pp, int64_t Ntotal_pm, int64_t Ntotal_mm,
+                      const int32_t *Indof2ndPhonon_plus, const int32_t *Indof3rdPhonon_plus,
+                      const int32_t *Indof2ndPhonon_minus, const int32_t *Indof3rdPhonon_minus,
+                      const int64_t *Indof2ndPhonon_pp, const int64_t *Indof3rdPhonon_pp,
+                      const int64_t *Indof4thPhonon_pp, const int64_t *Indof2ndPhonon_pm,
+                      const int64_t *Indof3rdPhonon_pm, const int64_t *Indof4thPhonon_pm,
+                      const int64_t *Indof2ndPhonon_mm, const int64_t *Indof3rdPhonon_mm,
+                      const int64_t *Indof4thPhonon_mm, const double *Gamma_plus, const double *Gamma_minus,
+                      const double *Gamma_pp, const double *Gamma_pm, const double *Gamma_mm,
+                      const double *tau_zero, double *F_n);
+
+/* One iteration_4ph sweep with the operator of the last
+   fp4_bte_set_operator / fp4_iteration_4ph call; F_n updated in place. */
+int fp4_bte_iterate(fp4_ctx *ctx, const double *tau_zero, double *F_n);
+
+/* TConduct, Src/conductivity.f90:11-37, at the context's T.  V is the cell
+   volume in nm^3 (the reference's module global V).  ThConductivity(Nb,3,3)
+   and, unless NULL, ThConductivityMode(nptk,Nb,3,3) are overwritten. */
+int fp4_tconduct(fp4_ctx *ctx, double V, const double *F_n, double *ThConductivity, double *ThConductivityMode);
+
+/* The iterative solution of Src/ShengBTE.f90:1250-1307 on the device with
+   the current operator: iteration0, TConduct and symmetrize_tensor, then up
+   to maxiter rounds of (iteration_4ph sweep, symmetrisers on every q,
+   TConduct, symmetrize_tensor) until the relative change of sum_b kappa in
+   the 2-norm is below eps.  Outputs: F_n(Nb,nptk,3), ThConductivity(Nb,3,3)
+   of the last round, and unless NULL kappa_iter(3,3,0:niter) (sum over
+   branches per round) and relchange(0:niter) (entry 0 is 0); *niter is the
+   number of sweeps made. */
+int fp4_bte_solve(fp4_ctx *ctx, double V, const double *tau_zero, int32_t maxiter, double eps, double *F_n,
+                  double *ThConductivity, double *kappa_iter, double *relchange, int32_t *niter);
+
 /* Diagnostics; no counterpart in the reference.  Accumulated device time of
    each kernel family, measured with HIP events on the context's stream:
    kind 0 = prefilter count pass, 1 = list emission, 2 = FP64 per-process
-   rate kernel, 3 = dense rate kernels. */
+   rate kernel, 3 = dense rate kernels, 4 = iteration_4ph sweeps (launches
+   counts (ll, i, kk) steps). */
 int fp4_ctx_kernel_stats(fp4_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 int fp4_ctx_reset_stats(fp4_ctx *ctx);
 /* Diagnostics; no counterpart in the reference.  Work of the last dense-form

@@ -543,9 +543,13 @@ void orc_iteration0(int nptk, int nbands, int nlist, int nsymm_rot, const int32_
       }
 }
 
-/* iterations.f90:95-215, four-phonon terms (the three-phonon lists are empty). */
+/* iterations.f90:95-215.  The 3ph arguments (N_plus, N_minus and their
+   int32 lists) may be NULL, which means empty 3ph lists. */
 void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot, const int32_t *Nequi,
                        const int32_t *ALLEquiList, const int32_t *TypeofSymmetry, const int32_t *ID_equi,
+                       const int32_t *N_plus, const int32_t *N_minus, const int32_t *i2_plus,
+                       const int32_t *i3_plus, const int32_t *i2_minus, const int32_t *i3_minus,
+                       const double *g_plus, const double *g_minus,
                        const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm, const int64_t *i2_pp,
                        const int64_t *i3_pp, const int64_t *i4_pp, const int64_t *i2_pm, const int64_t *i3_pm,
                        const int64_t *i4_pm, const int64_t *i2_mm, const int64_t *i3_mm, const int64_t *i4_mm,
@@ -554,6 +558,10 @@ void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot, const int
   const int nb = nbands;
   double *dF = (double *)calloc((size_t)nb * nptk * 3, sizeof(double));
   int64_t acc[3] = {0, 0, 0};
+  int64_t acc3[2] = {0, 0};
+  const int32_t *N3[2] = {N_plus, N_minus};
+  const int32_t *I2_3[2] = {i2_plus, i2_minus}, *I3_3[2] = {i3_plus, i3_minus};
+  const double *G3[2] = {g_plus, g_minus};
   const int64_t *Ns[3] = {N_pp, N_pm, N_mm};
   const int64_t *I2[3] = {i2_pp, i2_pm, i2_mm};
   const int64_t *I3[3] = {i3_pp, i3_pm, i3_mm};
@@ -563,11 +571,30 @@ void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot, const int
   for (int ll = 1; ll <= nlist; ll++) {
     for (int i = 1; i <= nb; i++) {
       const int mm = (ll - 1) * nb + i;
-      if (mm != 1)
+      if (mm != 1) {
         for (int ch = 0; ch < 3; ch++) acc[ch] = acc[ch] + Ns[ch][mm - 2];
+        for (int t = 0; t < 2; t++)
+          if (N3[t]) acc3[t] = acc3[t] + N3[t][mm - 2];
+      }
       for (int kk = 1; kk <= Nequi[ll - 1]; kk++) {
         const int qk = ALLEquiList[(kk - 1) + (size_t)nsymm_rot * (ll - 1)];
         const int sym = TypeofSymmetry[(kk - 1) + (size_t)nsymm_rot * (ll - 1)];
+        /* 3ph terms, iterations.f90:146-167 */
+        for (int t = 0; t < 2; t++) {
+          const int64_t n = N3[t] ? N3[t][mm - 1] : 0;
+          for (int64_t x = 0; x < n; x++) {
+            const int64_t p = acc3[t] + x;
+            const int j = (I2_3[t][p] - 1) % nb + 1, qm = (I2_3[t][p] - 1) / nb + 1;
+            const int k = (I3_3[t][p] - 1) % nb + 1, qn = (I3_3[t][p] - 1) / nb + 1;
+            const int m2 = IDEQ(sym, qm), n2 = IDEQ(sym, qn);
+            for (int a = 1; a <= 3; a++) {
+              const double Fj = FN(F_n, nb, nptk, j, m2, a);
+              const double Fk = FN(F_n, nb, nptk, k, n2, a);
+              const double term = (t == 0) ? G3[t][p] * (Fk - Fj) : G3[t][p] * (Fk + Fj) * 5.e-1;
+              FN(dF, nb, nptk, i, qk, a) = FN(dF, nb, nptk, i, qk, a) + term;
+            }
+          }
+        }
         for (int ch = 0; ch < 3; ch++) {
           const int64_t n = Ns[ch][mm - 1];
           for (int64_t t = 0; t < n; t++) {
@@ -638,4 +665,44 @@ void orc_tconduct(int nptk, int nbands, double T, double V, const double *omega,
   const double pre = (double)F_1E21 * (HBAR_D * HBAR_D);
   const double den = KB_D * T * T * V * nptk;
   for (int t = 0; t < nbands * 9; t++) kappa[t] = pre * kappa[t] / den;
+}
+
+/* F_n(:,q,:) = transpose(matmul(symmetrizers(:,:,q), transpose(F_n(:,q,:)))) for
+   every q, ShengBTE.f90:1288-1290 */
+void orc_symmetrize_F(int nptk, int nbands, const double *symmetrizers, double *F_n) {
+  for (int q = 1; q <= nptk; q++) {
+    const double *S = symmetrizers + 9 * (size_t)(q - 1);
+    for (int b = 1; b <= nbands; b++) {
+      double x[3], y[3];
+      for (int a = 0; a < 3; a++) x[a] = FN(F_n, nbands, nptk, b, q, a + 1);
+      for (int a = 0; a < 3; a++) {
+        double s = 0.0;
+        for (int c2 = 0; c2 < 3; c2++) s = s + S[a + 3 * c2] * x[c2];
+        y[a] = s;
+      }
+      for (int a = 0; a < 3; a++) FN(F_n, nbands, nptk, b, q, a + 1) = y[a];
+    }
+  }
+}
+
+/* symmetrize_tensor, config.f90:503-518: tensor(3,3) <- sum_isym C T C^T / Nsymm_rot */
+void orc_symmetrize_tensor(int nsymm_rot, const double *crotations, double *tensor) {
+  double tmp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int is = 0; is < nsymm_rot; is++) {
+    const double *Cr = crotations + 9 * (size_t)is;
+    double M1[9];
+    for (int a = 0; a < 3; a++)
+      for (int c2 = 0; c2 < 3; c2++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) s = s + tensor[a + 3 * k] * Cr[c2 + 3 * k];
+        M1[a + 3 * c2] = s;
+      }
+    for (int a = 0; a < 3; a++)
+      for (int c2 = 0; c2 < 3; c2++) {
+        double s = 0.0;
+        for (int k = 0; k < 3; k++) s = s + Cr[a + 3 * k] * M1[k + 3 * c2];
+        tmp[a + 3 * c2] = tmp[a + 3 * c2] + s;
+      }
+  }
+  for (int t = 0; t < 9; t++) tensor[t] = tmp[t] / nsymm_rot;
 }

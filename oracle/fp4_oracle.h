@@ -94,6 +94,9 @@ void orc_iteration0(int nptk, int nbands, int nlist, int nsymm_rot, const int32_
 void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot,
                        const int32_t *Nequi, const int32_t *ALLEquiList,
                        const int32_t *TypeofSymmetry, const int32_t *ID_equi,
+                       const int32_t *N_plus, const int32_t *N_minus, const int32_t *i2_plus,
+                       const int32_t *i3_plus, const int32_t *i2_minus, const int32_t *i3_minus,
+                       const double *g_plus, const double *g_minus,
                        const int64_t *N_pp, const int64_t *N_pm, const int64_t *N_mm,
                        const int64_t *i2_pp, const int64_t *i3_pp, const int64_t *i4_pp,
                        const int64_t *i2_pm, const int64_t *i3_pm, const int64_t *i4_pm,
@@ -103,6 +106,8 @@ void orc_iteration_4ph(int nptk, int nbands, int nlist, int nsymm_rot,
                        const double *tau_zero, const double *symmetrizers, double *F_n);
 void orc_tconduct(int nptk, int nbands, double T, double V, const double *omega,
                   const double *velocity, const double *F_n, double *kappa);
+void orc_symmetrize_F(int nptk, int nbands, const double *symmetrizers, double *F_n);
+void orc_symmetrize_tensor(int nsymm_rot, const double *crotations, double *tensor);
 
 #ifdef __cplusplus
 }

@@ -179,17 +179,79 @@ def iteration0(case, tau_zero):
     return F
 
 
-def iteration_4ph(case, N, lists, gammas, tau_zero, F_n):
+def iteration_4ph(case, N, lists, gammas, tau_zero, F_n, N3=None, lists3=None, gammas3=None, symmetrizers=None):
+    """iteration_4ph on a copy of F_n.  N3/lists3/gammas3: optional 3ph counts (int32), [[i2, i3] for +, -]
+    and Gamma; ``symmetrizers`` overrides the case's (3,3,nptk) array."""
     m = case.modes
     F = np.asfortranarray(F_n.copy())
     tz = np.asfortranarray(tau_zero, np.float64)
+    if N3 is None:
+        p3 = [None] * 8
+    else:
+        n3 = [np.ascontiguousarray(n, np.int32) for n in N3]
+        l3 = [np.ascontiguousarray(x, np.int32) if len(x) else np.zeros(1, np.int32) for ch in lists3 for x in ch]
+        g3 = [np.ascontiguousarray(g, np.float64) if len(g) else np.zeros(1) for g in gammas3]
+        keep3 = (n3, l3, g3)
+        p3 = [_ptr(x) for x in n3 + l3 + g3]
+    S = m.symmetrizers if symmetrizers is None else np.asfortranarray(symmetrizers, np.float64)
     args = [C.c_int(m.nptk), C.c_int(m.nbands), C.c_int(m.Nlist), C.c_int(m.nsymm_rot),
             _ptr(np.ascontiguousarray(m.Nequi, np.int32)), _ptr(m.ALLEquiList), _ptr(m.TypeofSymmetry),
-            _ptr(m.ID_equi)] + [_ptr(n) for n in N] + [_ptr(np.ascontiguousarray(o, np.int64)) for ch in lists for o in ch]
+            _ptr(m.ID_equi)] + p3 + [_ptr(n) for n in N]
+    args += [_ptr(np.ascontiguousarray(o, np.int64)) for ch in lists for o in ch]
     args += [_ptr(m.energy), _ptr(m.velocity)] + [_ptr(np.ascontiguousarray(g, np.float64)) for g in gammas]
-    args += [_ptr(tz), _ptr(m.symmetrizers), _ptr(F)]
+    args += [_ptr(tz), _ptr(S), _ptr(F)]
     lib().orc_iteration_4ph(*args)
     return F
+
+
+def symmetrize_F(case, F_n):
+    """Symmetrisers applied to F_n(:,q,:) for every q (ShengBTE.f90:1288-1290); returns a copy."""
+    m = case.modes
+    F = np.array(F_n, dtype=np.float64, order="F", copy=True)
+    lib().orc_symmetrize_F(C.c_int(m.nptk), C.c_int(m.nbands), _ptr(m.symmetrizers), _ptr(F))
+    return F
+
+
+def symmetrize_tensor(case, tensor):
+    """symmetrize_tensor (config.f90:503-518) of a (3,3) array; returns a copy."""
+    m = case.modes
+    t = np.array(tensor, dtype=np.float64, order="F", copy=True)
+    lib().orc_symmetrize_tensor(C.c_int(m.nsymm_rot), _ptr(np.asfortranarray(m.crotations, np.float64)), _ptr(t))
+    return t
+
+
+def twonorm3x3(a):
+    """2-norm of a 3x3 matrix: the largest singular value from LAPACK gesdd, as misc.f90:52-65."""
+    return float(np.linalg.svd(np.asarray(a, np.float64), compute_uv=False)[0])
+
+
+def bte_solve(case, N, lists, gammas, tau_zero, maxiter=1000, eps=float(np.float32(1e-5)), N3=None, lists3=None,
+              gammas3=None):
+    """The iterative loop of ShengBTE.f90:1250-1307 with the four-phonon iteration:
+    -> dict(F_n, kappa (Nb,3,3), kappa_iter (3,3,niter+1), relchange (niter+1), niter)."""
+    m = case.modes
+    F = iteration0(case, tau_zero)
+
+    def kappa_of(F):
+        k = tconduct(case, F)
+        for b in range(m.nbands):
+            k[b] = symmetrize_tensor(case, k[b])
+        return k
+
+    k = kappa_of(F)
+    hist, rel = [k.sum(axis=0)], [0.0]
+    it = 0
+    for it in range(1, maxiter + 1):
+        old = hist[-1]
+        F = iteration_4ph(case, N, lists, gammas, tau_zero, F, N3, lists3, gammas3)
+        F = symmetrize_F(case, F)
+        k = kappa_of(F)
+        ks = k.sum(axis=0)
+        hist.append(ks)
+        rel.append(twonorm3x3(ks - old) / twonorm3x3(old))
+        if rel[-1] < eps:
+            break
+    return {"F_n": F, "kappa": k, "kappa_iter": np.stack(hist, axis=-1), "relchange": np.array(rel), "niter": it}
 
 
 def tconduct(case, F_n):

@@ -1,0 +1,123 @@
+"""Device iterative solver (iteration0, iteration_4ph, TConduct, the
+convergence loop) against the oracle.
+
+Bars: iteration0 bit-exact (same products in the same order); one
+iteration_4ph sweep within 1e-12 relative (the per-step sums are
+parallel reductions, the oracle's are sequential); TConduct within 1e-12;
+the converged loop with the same iteration count and kappa within 1e-10.
+"""
+import numpy as np
+import pytest
+
+from fourphonon_amd import crystal
+from fourphonon_amd import runtime as rt
+from tests import iterops
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+
+
+def rel(got, want):
+    return float(np.abs(np.asarray(got) - np.asarray(want)).max() / np.abs(np.asarray(want)).max())
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    case = crystal.config_case(1, ngrid=(4, 4, 4))
+    banner("context diamond 4x4x4 (48 operations)")
+    c = rt.Context(case)
+    yield case, c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def physical(ctx):
+    """Lists and per-process Gamma from the device Ind_driver_4ph."""
+    case, c = ctx
+    N = c.NP_driver_4ph()
+    Ns = [N[0], N[2], N[4]]
+    lists, gammas, rates = c.Ind_driver_4ph(*Ns)
+    r = rates["rate_scatt_4ph"]
+    tau = np.asfortranarray(np.where(r > 0, 1.0 / np.where(r > 0, r, 1.0), 0.0))
+    return Ns, lists, gammas, tau
+
+
+def test_iteration0_bit_exact(ctx):
+    from oracle.oracle import iteration0
+    case, c = ctx
+    tau = np.asfortranarray(np.random.default_rng(1).uniform(0.5, 2.0, (case.modes.nbands, case.modes.Nlist)))
+    banner("iteration0")
+    assert np.array_equal(c.iteration0(tau), iteration0(case, tau))
+
+
+@pytest.mark.parametrize("with_3ph", [False, True], ids=["4ph", "3ph+4ph"])
+def test_sweep_synthetic_operator(ctx, with_3ph):
+    from oracle.oracle import iteration0, iteration_4ph
+    case, c = ctx
+    N4, L4, G4, N3, L3, G3 = iterops.synthetic_operator(case, seed=21, per_mode=40, with_3ph=with_3ph)
+    tau = iterops.contractive_tau(case, N4, G4, N3, G3, margin=0.9)
+    F = iteration0(case, tau)
+    banner(f"iteration_4ph synthetic operator, 3ph={with_3ph}")
+    got = c.iteration_4ph(N4, L4, G4, tau, F, N3, L3, G3)
+    want = iteration_4ph(case, N4, L4, G4, tau, F, N3, L3, G3)
+    assert rel(got, want) <= 1e-12
+    # a second sweep on the resident operator
+    got2 = c.bte_iterate(tau, got)
+    want2 = iteration_4ph(case, N4, L4, G4, tau, want, N3, L3, G3)
+    assert rel(got2, want2) <= 1e-12
+
+
+def test_sweep_physical_operator_and_reproducible(ctx, physical):
+    from oracle.oracle import iteration0, iteration_4ph
+    case, c = ctx
+    Ns, lists, gammas, tau = physical
+    F = iteration0(case, tau)
+    banner(f"iteration_4ph physical operator, {sum(int(n.sum()) for n in Ns)} processes")
+    got = c.iteration_4ph(Ns, lists, gammas, tau, F)
+    want = iteration_4ph(case, Ns, lists, gammas, tau, F)
+    assert rel(got, want) <= 1e-12
+    again = c.bte_iterate(tau, F)
+    assert np.array_equal(got, again)
+    ms, steps = c.kernel_stats(rt.Context.KS_ITER)
+    assert steps >= 2 * case.modes.nptk * case.modes.nbands and ms > 0
+
+
+def test_tconduct(ctx, physical):
+    from oracle.oracle import iteration0, tconduct
+    case, c = ctx
+    _, _, _, tau = physical
+    F = iteration0(case, tau)
+    banner("TConduct")
+    k, km = c.TConduct(F, mode=True)
+    want = tconduct(case, F)
+    assert rel(k, want) <= 1e-12
+    assert np.all(km[0] == 0.0)
+    assert rel(km.sum(axis=0), k) <= 1e-12
+
+
+def test_bte_solve_matches_oracle_loop(ctx, physical):
+    from oracle.oracle import bte_solve
+    case, c = ctx
+    Ns, lists, gammas, tau = physical
+    banner("bte_solve (physical operator)")
+    c.bte_set_operator(Ns, lists, gammas)
+    got = c.bte_solve(tau, maxiter=50)
+    want = bte_solve(case, Ns, lists, gammas, tau, maxiter=50)
+    print(f"niter {got['niter']} (oracle {want['niter']}), kappa trace "
+          f"{np.trace(got['kappa_iter'][:, :, -1]):.6e}, relchange {got['relchange'][-1]:.3e}")
+    assert got["niter"] == want["niter"]
+    assert rel(got["kappa_iter"], want["kappa_iter"]) <= 1e-10
+    assert rel(got["kappa"], want["kappa"]) <= 1e-10
+    assert rel(got["F_n"], want["F_n"]) <= 1e-10
+    assert np.allclose(got["relchange"], want["relchange"], rtol=1e-6, atol=1e-16)
+
+
+def test_bad_list_entry_is_rejected(ctx):
+    case, c = ctx
+    N4, L4, G4, _, _, _ = iterops.synthetic_operator(case, seed=3, with_3ph=False)
+    L4[1][2][0] = case.modes.nptk * case.modes.nbands + 1
+    with pytest.raises(rt.FP4Error):
+        c.bte_set_operator(N4, L4, G4)
