@@ -10,10 +10,15 @@
 !   RTA_driver_4ph_GPU_using_Ind  Src/processes.f90:5315-5359
 !   Ind_driver_4ph                Src/processes.f90:3551-3589
 !   Vp_pp / Vp_pm / Vp_mm         Src/processes.f90:142-330
+!   iteration0                    Src/iterations.f90:13-30
+!   iteration_4ph                 Src/iterations.f90:95-120
+!   TConduct                      Src/conductivity.f90:11-16
 ! The reference reads the crystal and run parameters from `module config`
 ! globals (Src/config.f90:9-51); here they are given once to fp4_init, which
 ! a caller invokes after read_config (and again after a change of T through
-! fp4_set_temperature).
+! fp4_set_temperature); the globals of the iterative solver (V, Nsymm_rot,
+! symmetrizers, crotations and the ID_equi table of symmetry_map) are given
+! to fp4_init_iterations.
 !
 ! Deviation from the reference: an error in the device path is reported on
 ! error_unit and stops the program (error stop), where the reference prints
@@ -122,17 +127,71 @@ module fp4_processes
        integer(c_int32_t), intent(in) :: idx(*)
        real(c_double), intent(out) :: out(*)
      end function c_vp
+     integer(c_int) function c_set_symmetry(ctx, nsymm, ideq, symm, crot) bind(C, name="fp4_ctx_set_symmetry")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsymm
+       integer(c_int32_t), intent(in) :: ideq(*)
+       real(c_double), intent(in) :: symm(*), crot(*)
+     end function c_set_symmetry
+     integer(c_int) function c_iteration0(ctx, nlist, Nequi, alleq, tau, F) bind(C, name="fp4_iteration0")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nlist
+       integer(c_int32_t), intent(in) :: Nequi(*), alleq(*)
+       real(c_double), intent(in) :: tau(*)
+       real(c_double), intent(inout) :: F(*)
+     end function c_iteration0
+     integer(c_int) function c_set_operator(ctx, nlist, Nequi, alleq, types, Np, Nm, Ntp, Ntm, Npp, Npm, Nmm, &
+          Ntpp, Ntpm, Ntmm, p2, p3, m2, m3, a2, a3, a4, b2, b3, b4, c2, c3, c4, gp, gm, gpp, gpm, gmm) &
+          bind(C, name="fp4_bte_set_operator")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nlist, Ntp, Ntm
+       integer(c_int64_t), value :: Ntpp, Ntpm, Ntmm
+       integer(c_int32_t), intent(in) :: Nequi(*), alleq(*), types(*), Np(*), Nm(*), p2(*), p3(*), m2(*), m3(*)
+       integer(c_int64_t), intent(in) :: Npp(*), Npm(*), Nmm(*), a2(*), a3(*), a4(*), b2(*), b3(*), b4(*), &
+            c2(*), c3(*), c4(*)
+       real(c_double), intent(in) :: gp(*), gm(*), gpp(*), gpm(*), gmm(*)
+     end function c_set_operator
+     integer(c_int) function c_bte_iterate(ctx, tau, F) bind(C, name="fp4_bte_iterate")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: tau(*)
+       real(c_double), intent(inout) :: F(*)
+     end function c_bte_iterate
+     integer(c_int) function c_tconduct(ctx, V, F, kappa, mode) bind(C, name="fp4_tconduct")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: V
+       real(c_double), intent(in) :: F(*)
+       real(c_double), intent(out) :: kappa(*), mode(*)
+     end function c_tconduct
+     integer(c_int) function c_bte_solve(ctx, V, tau, maxiter, eps, F, kappa, hist, rel, niter) &
+          bind(C, name="fp4_bte_solve")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: V, eps
+       integer(c_int32_t), value :: maxiter
+       real(c_double), intent(in) :: tau(*)
+       real(c_double), intent(out) :: F(*), kappa(*), hist(*), rel(*)
+       integer(c_int32_t), intent(out) :: niter
+     end function c_bte_solve
   end interface
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int32_t), save :: precision = FP4_PREC_FP64_DENSE
   integer(kind=4), save :: nptk_ = 0, nbands_ = 0
+  real(c_double), save :: volume_ = 0.d0
+  logical, save :: keep_operator_ = .false., operator_valid_ = .false.
   real(c_double), allocatable, target, save :: masses_(:)
   integer(c_int32_t), allocatable, target, save :: types_(:)
 
   public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision
   public :: NP_driver_4ph, Ind_only_driver_4ph, RTA_driver_4ph, RTA_driver_4ph_GPU_using_Ind, Ind_driver_4ph
   public :: Vp_pp, Vp_pm, Vp_mm
+  public :: fp4_init_iterations, fp4_keep_operator, fp4_set_operator_4ph, fp4_solve_4ph
+  public :: iteration0, iteration_4ph, TConduct
 
 contains
 
@@ -381,5 +440,125 @@ contains
     integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:), Index_l(:)
     Vp_mm = vp_one(2_c_int32_t, i, j, k, l, q, qprime, qdprime, qtprime)
   end function Vp_mm
+
+  ! ---------------------------------------------------------- iterations
+  ! Globals read by Src/iterations.f90 and Src/conductivity.f90: the cell
+  ! volume V (nm^3), Nsymm_rot, ID_equi from symmetry_map
+  ! (Src/config.f90:409-431), symmetrizers(3,3,nptk) and crotations.
+  subroutine fp4_init_iterations(V, Nsymm_rot, ID_equi, symmetrizers, crotations)
+    real(kind=8), intent(in) :: V, symmetrizers(:,:,:), crotations(:,:,:)
+    integer(kind=4), intent(in) :: Nsymm_rot, ID_equi(:,:)
+    volume_ = V
+    operator_valid_ = .false.
+    call check(c_set_symmetry(ctx, Nsymm_rot, ID_equi, symmetrizers, crotations), "fp4_ctx_set_symmetry")
+  end subroutine fp4_init_iterations
+
+  ! With flag .true., the scattering operator (lists and Gamma) uploaded by
+  ! the next iteration_4ph call stays on the device and later calls reuse it
+  ! without reading their list and Gamma arguments again; the caller
+  ! promises not to change them until fp4_keep_operator(.false.).  Default
+  ! .false.: every call uploads its arguments, as the reference reads them.
+  subroutine fp4_keep_operator(flag)
+    logical, intent(in) :: flag
+    keep_operator_ = flag
+    operator_valid_ = .false.
+  end subroutine fp4_keep_operator
+
+  ! iteration0 and iteration_4ph use the omega/velocity resident on the
+  ! device, i.e. those of the last driver call on this context; their omega
+  ! and velocity arguments are accepted for interface compatibility.
+  subroutine iteration0(Nlist, Nequi, ALLEquiList, omega, velocity, tau_zero, F_n)
+    integer(kind=4), intent(in) :: Nlist, Nequi(:), ALLEquiList(:,:)
+    real(kind=8), intent(in) :: omega(:,:), velocity(:,:,:), tau_zero(:,:)
+    real(kind=8), intent(out) :: F_n(:,:,:)
+    call check(c_iteration0(ctx, Nlist, Nequi, ALLEquiList, tau_zero, F_n), "iteration0")
+  end subroutine iteration0
+
+  subroutine fp4_set_operator_4ph(Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, &
+       Ntotal_minus, N_plusplus, N_plusminus, N_minusminus, Ntotal_plusplus, Ntotal_plusminus, Ntotal_minusminus, &
+       Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, Indof3rdPhonon_minus, &
+       Indof2ndPhonon_plusplus, Indof3rdPhonon_plusplus, Indof4thPhonon_plusplus, &
+       Indof2ndPhonon_plusminus, Indof3rdPhonon_plusminus, Indof4thPhonon_plusminus, &
+       Indof2ndPhonon_minusminus, Indof3rdPhonon_minusminus, Indof4thPhonon_minusminus, &
+       Gamma_plus, Gamma_minus, Gamma_plusplus, Gamma_plusminus, Gamma_minusminus)
+    integer(kind=4), intent(in) :: Nlist, Nequi(:), ALLEquiList(:,:), TypeofSymmetry(:,:)
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:), Ntotal_plus, Ntotal_minus
+    integer(kind=4), intent(in) :: Indof2ndPhonon_plus(:), Indof3rdPhonon_plus(:)
+    integer(kind=4), intent(in) :: Indof2ndPhonon_minus(:), Indof3rdPhonon_minus(:)
+    integer(kind=8), intent(in) :: N_plusplus(:), N_plusminus(:), N_minusminus(:)
+    integer(kind=8), intent(in) :: Ntotal_plusplus, Ntotal_plusminus, Ntotal_minusminus
+    integer(kind=8), intent(in) :: Indof2ndPhonon_plusplus(:), Indof3rdPhonon_plusplus(:), Indof4thPhonon_plusplus(:)
+    integer(kind=8), intent(in) :: Indof2ndPhonon_plusminus(:), Indof3rdPhonon_plusminus(:), Indof4thPhonon_plusminus(:)
+    integer(kind=8), intent(in) :: Indof2ndPhonon_minusminus(:), Indof3rdPhonon_minusminus(:), &
+         Indof4thPhonon_minusminus(:)
+    real(kind=8), intent(in) :: Gamma_plus(:), Gamma_minus(:), Gamma_plusplus(:), Gamma_plusminus(:), &
+         Gamma_minusminus(:)
+    call check(c_set_operator(ctx, Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, &
+         Ntotal_minus, N_plusplus, N_plusminus, N_minusminus, Ntotal_plusplus, Ntotal_plusminus, Ntotal_minusminus, &
+         Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, Indof3rdPhonon_minus, &
+         Indof2ndPhonon_plusplus, Indof3rdPhonon_plusplus, Indof4thPhonon_plusplus, &
+         Indof2ndPhonon_plusminus, Indof3rdPhonon_plusminus, Indof4thPhonon_plusminus, &
+         Indof2ndPhonon_minusminus, Indof3rdPhonon_minusminus, Indof4thPhonon_minusminus, &
+         Gamma_plus, Gamma_minus, Gamma_plusplus, Gamma_plusminus, Gamma_minusminus), "fp4_bte_set_operator")
+    operator_valid_ = .true.
+  end subroutine fp4_set_operator_4ph
+
+  subroutine iteration_4ph(Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, Ntotal_minus, &
+       N_plusplus, N_plusminus, N_minusminus, Ntotal_plusplus, Ntotal_plusminus, Ntotal_minusminus, &
+       Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, Indof3rdPhonon_minus, &
+       Indof2ndPhonon_plusplus, Indof3rdPhonon_plusplus, Indof4thPhonon_plusplus, &
+       Indof2ndPhonon_plusminus, Indof3rdPhonon_plusminus, Indof4thPhonon_plusminus, &
+       Indof2ndPhonon_minusminus, Indof3rdPhonon_minusminus, Indof4thPhonon_minusminus, &
+       omega, velocity, Gamma_plus, Gamma_minus, Gamma_plusplus, Gamma_plusminus, Gamma_minusminus, tau_zero, F_n)
+    integer(kind=4), intent(in) :: Nlist, Nequi(:), ALLEquiList(:,:), TypeofSymmetry(:,:)
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:), Ntotal_plus, Ntotal_minus
+    integer(kind=4), intent(in) :: Indof2ndPhonon_plus(:), Indof3rdPhonon_plus(:)
+    integer(kind=4), intent(in) :: Indof2ndPhonon_minus(:), Indof3rdPhonon_minus(:)
+    integer(kind=8), intent(in) :: N_plusplus(:), N_plusminus(:), N_minusminus(:)
+    integer(kind=8), intent(in) :: Ntotal_plusplus, Ntotal_plusminus, Ntotal_minusminus
+    integer(kind=8), intent(in) :: Indof2ndPhonon_plusplus(:), Indof3rdPhonon_plusplus(:), Indof4thPhonon_plusplus(:)
+    integer(kind=8), intent(in) :: Indof2ndPhonon_plusminus(:), Indof3rdPhonon_plusminus(:), Indof4thPhonon_plusminus(:)
+    integer(kind=8), intent(in) :: Indof2ndPhonon_minusminus(:), Indof3rdPhonon_minusminus(:), &
+         Indof4thPhonon_minusminus(:)
+    real(kind=8), intent(in) :: omega(:,:), velocity(:,:,:)
+    real(kind=8), intent(in) :: Gamma_plus(:), Gamma_minus(:), Gamma_plusplus(:), Gamma_plusminus(:), &
+         Gamma_minusminus(:)
+    real(kind=8), intent(in) :: tau_zero(:,:)
+    real(kind=8), intent(inout) :: F_n(:,:,:)
+    if (.not. (keep_operator_ .and. operator_valid_)) then
+       call fp4_set_operator_4ph(Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, &
+            Ntotal_minus, N_plusplus, N_plusminus, N_minusminus, Ntotal_plusplus, Ntotal_plusminus, &
+            Ntotal_minusminus, Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, &
+            Indof3rdPhonon_minus, Indof2ndPhonon_plusplus, Indof3rdPhonon_plusplus, Indof4thPhonon_plusplus, &
+            Indof2ndPhonon_plusminus, Indof3rdPhonon_plusminus, Indof4thPhonon_plusminus, &
+            Indof2ndPhonon_minusminus, Indof3rdPhonon_minusminus, Indof4thPhonon_minusminus, &
+            Gamma_plus, Gamma_minus, Gamma_plusplus, Gamma_plusminus, Gamma_minusminus)
+    end if
+    call check(c_bte_iterate(ctx, tau_zero, F_n), "iteration_4ph")
+  end subroutine iteration_4ph
+
+  ! TConduct at the context's T and the V of fp4_init_iterations.
+  subroutine TConduct(omega, velocity, F_n, ThConductivity, ThConductivityMode)
+    real(kind=8), intent(in) :: omega(:,:), velocity(:,:,:), F_n(:,:,:)
+    real(kind=8), intent(out) :: ThConductivity(:,:,:), ThConductivityMode(:,:,:,:)
+    call check(c_tconduct(ctx, volume_, F_n, ThConductivity, ThConductivityMode), "TConduct")
+  end subroutine TConduct
+
+  ! The convergence loop of Src/ShengBTE.f90:1250-1307 on the device, with
+  ! the operator of the last fp4_set_operator_4ph / iteration_4ph call:
+  ! kappa_iter(3,3,0:maxiter) and relchange(0:maxiter) receive entries
+  ! 0..niter.
+  subroutine fp4_solve_4ph(tau_zero, maxiter, eps, F_n, ThConductivity, kappa_iter, relchange, niter)
+    real(kind=8), intent(in) :: tau_zero(:,:), eps
+    integer(kind=4), intent(in) :: maxiter
+    real(kind=8), intent(out) :: F_n(:,:,:), ThConductivity(:,:,:), kappa_iter(:,:,0:), relchange(0:)
+    integer(kind=4), intent(out) :: niter
+    if (size(kappa_iter, 3) < maxiter + 1 .or. size(relchange) < maxiter + 1) then
+       write(error_unit, "(a)") "Error: fp4_solve_4ph needs kappa_iter(3,3,0:maxiter) and relchange(0:maxiter)"
+       error stop 1
+    end if
+    call check(c_bte_solve(ctx, volume_, tau_zero, maxiter, eps, F_n, ThConductivity, kappa_iter, relchange, &
+         niter), "fp4_bte_solve")
+  end subroutine fp4_solve_4ph
 
 end module fp4_processes

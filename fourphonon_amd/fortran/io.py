@@ -14,12 +14,16 @@ def write_inputs(case, path):
         fh.write(" ".join(repr(float(x)) for x in np.asarray(st.rlattvec).ravel(order="F")) + "\n")
         fh.write(" ".join(repr(float(x)) for x in st.masses) + "\n")
         fh.write(" ".join(str(int(x)) for x in st.types) + "\n")
+        fh.write(f"{m.nsymm_rot} {float(st.volume)!r}\n")
     arrays = {
         "energy": (m.energy, np.float64), "velocity": (m.velocity, np.float64),
         "eigenvect": (m.eigenvect, np.complex128), "IJK": (m.IJK, np.int32), "List": (m.List, np.int32),
         "Phi": (f.Phi, np.float64), "R_j": (f.R_j, np.float64), "R_k": (f.R_k, np.float64),
         "R_l": (f.R_l, np.float64), "Index_i": (f.Index_i, np.int32), "Index_j": (f.Index_j, np.int32),
         "Index_k": (f.Index_k, np.int32), "Index_l": (f.Index_l, np.int32),
+        "Nequi": (m.Nequi, np.int32), "ALLEquiList": (m.ALLEquiList, np.int32),
+        "TypeofSymmetry": (m.TypeofSymmetry, np.int32), "ID_equi": (m.ID_equi, np.int32),
+        "symmetrizers": (m.symmetrizers, np.float64), "crotations": (m.crotations, np.float64),
     }
     for name, (a, dt) in arrays.items():
         np.asarray(a, dtype=dt).ravel(order="F").tofile(os.path.join(path, name + ".bin"))
@@ -34,3 +38,18 @@ def read_outputs(path, nmodes, lists=False):
             x = np.fromfile(os.path.join(path, f"lists_{ch}.bin"), dtype=np.int64)
             out["lists_" + ch] = x.reshape(3, -1)
     return out
+
+
+def read_iteration_outputs(path, nbands, nptk, sweeps=3):
+    """Outputs of the driver's "iterate" mode."""
+    F = np.fromfile(os.path.join(path, "iter_F.bin"), dtype=np.float64).reshape((nbands, nptk, 3), order="F")
+    kap = np.fromfile(os.path.join(path, "iter_kappa.bin"), dtype=np.float64)
+    kap = kap.reshape((nbands, 3, 3, sweeps + 1), order="F")
+    raw = np.fromfile(os.path.join(path, "solve.bin"), dtype=np.float64)
+    niter = int(raw[0])
+    hist = raw[1:1 + 9 * (niter + 1)].reshape((3, 3, niter + 1), order="F")
+    rel = raw[1 + 9 * (niter + 1):1 + 10 * (niter + 1)]
+    kappa = raw[1 + 10 * (niter + 1):].reshape((nbands, 3, 3), order="F")
+    tau = np.fromfile(os.path.join(path, "tau.bin"), dtype=np.float64).reshape((nbands, -1), order="F")
+    return {"F_n": F, "kappa_sweeps": kap, "niter": niter, "kappa_iter": hist, "relchange": rel, "kappa": kappa,
+            "tau": tau}
