@@ -57,17 +57,43 @@ def kappa_rta(case, rate):
     return k * float(np.float32(1e21)) * hbar ** 2 / (kb * T * T * V * m.nptk)
 
 
+def dense_algorithmic_flops(case, ntriples):
+    """Algorithmic work of the MFMA form: per visited momentum triple
+    (channel, q, q', q'') with at least one surviving process, covering all
+    Nb^4 branch combinations of all Nb modes at q, the fold of every IFC
+    cluster with its phase (324 * Ntri4: one real x complex MAC per IFC
+    element) plus the four-stage complex contraction (4 * Nb^5 complex MACs,
+    8 flops each)."""
+    nb = case.nbands
+    return (324.0 * case.ifc4.Ntri + 32.0 * nb ** 5) * ntriples
+
+
 def dense_flops(case, ngroups, nfine, ntriples, nproc, nitems):
-    """Algorithmic flops of one dense-form pass (Nb = number of branches):
-    per visited (q, q', q'') triple the skinny GEMM, 8 flops per complex MAC
-    over ngroups * Nb^2 * 9 entries; per process the contraction,
-    8 * (Nb^2 + Nb); per work item (channel, q, q') the A' build,
+    """Flops the regrouped dense kernel actually executes (Nb = number of
+    branches): per triple the skinny GEMM, 8 flops per complex MAC over
+    ngroups * Nb^2 * 9 entries; per process the contraction, 8 * (Nb^2 + Nb);
+    per work item (channel, q, q') the A' build,
     5 * 81 * Ntri + nfine * 8 * 27 * Nb * (3 + Nb)."""
     nb = case.nbands
     per_triple = 8.0 * ngroups * nb * nb * 9
     per_proc = 8.0 * (nb * nb + nb)
     per_item = 5.0 * 81 * case.ifc4.Ntri + nfine * 8.0 * 27 * nb * (3 + nb)
     return per_triple * ntriples + per_proc * nproc + per_item * nitems
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
+    None when no summary covers this kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", "dense_fp32_diamond16_shell2_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        mt = re.match(r"traffic_bytes_per_launch:\s*(\d+)\s+kernel\s+(\S+)", line.strip())
+        if mt and mt.group(2) == kernel:
+            return int(mt.group(1))
+    return None
 
 
 def copy_calibration(nbytes):
@@ -140,15 +166,18 @@ def main():
     # ---- roofline of the dense kernel
     info = ctx.dense_info()
     ntriples = info["triples"]
-    flops = dense_flops(case, info["ngroups"], info["nfine"], ntriples, nproc_total, info["items"])
+    flops = dense_algorithmic_flops(case, ntriples)
+    executed = dense_flops(case, info["ngroups"], info["nfine"], ntriples, nproc_total, info["items"])
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
     shape = ctx.dense_shape()
     kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{'mfma' if shape['mfma'] else 'valu'}>"
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(kname),
                 "kernel": kname, "kernel_ms": round(avg_ms, 3), "launches": k_n,
-                "flops_per_launch": flops}
+                "flops_per_launch": flops, "flops_per_triple": flops / max(ntriples, 1),
+                "executed_flops_per_launch": executed,
+                "executed_tflops": round(executed / (avg_ms * 1e-3) / 1e12, 3)}
 
     out = {
         "metric": "4ph processes/s on 1 MI355X, MFMA form vs FP64 form, kappa within 1e-4 of FP64",

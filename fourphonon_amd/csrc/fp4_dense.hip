@@ -99,7 +99,7 @@ struct DenseArgs {
   double *out_g;
   double *out_w;
   int *out_n;
-  int *out_t;            // [nitems] visited (q, q', q'') triples
+  int *out_t;            // [nitems] (q, q', q'') triples with >= 1 surviving process
   int ntri;
   int smem_static;       // bytes of DenseSmem at the start of dynamic LDS
   float bf[9];           // rlattvec(:,nu)/ngrid(nu) as floats
@@ -200,6 +200,7 @@ struct DenseSmem {
   double vjj[TQ][NB][3], vss[TQ][NB][3];
   double sig[TQ][NB][NB];  // exact sigma of (q'',k) / (q''',l), -1 when excluded
   int jj[TQ], ss[TQ];
+  int hit[TQ];             // tile triple t has at least one surviving process
   double2 e1[NB][IB];
   double2 e2[NB][NB];
   int vj[NT_DENSE], vs[NT_DENSE];
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
 
     double acc_g[IB], acc_w[IB];
     int acc_n[IB];
-    int ntrip = 0;
+    int nhit = 0;   // (q, q', q'') triples of this item with >= 1 surviving process
 #pragma unroll
     for (int il = 0; il < IB; il++) {
       acc_g[il] = 0.0;
@@ -378,7 +379,6 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
         }
         int tot;
         const int pos = block_excl_scan<NW>(ok, s.wsum, tot);
-        ntrip += tot;
         if (ok) {
           s.vj[pos] = jj;
           s.vs[pos] = ssv;
@@ -389,6 +389,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           if (tid < TQ) {
             s.jj[tid] = (tid < tq) ? s.vj[t0 + tid] : 0;
             s.ss[tid] = (tid < tq) ? s.vs[t0 + tid] : 0;
+            s.hit[tid] = 0;
           }
           __syncthreads();
           // tile data: eigenvectors of phonons 3 and 4, frequencies, velocities, q'' phases
@@ -585,6 +586,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                     if (fabs(delta) <= sg) mask |= 1 << j;
                   }
                 }
+                if (mask) s.hit[t] = 1;
               }
             }
             int tot2;
@@ -604,6 +606,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           }
           if (qlen > 0) contract(qlen);
           __syncthreads();
+          if (tid < tq && s.hit[tid]) nhit++;
         }
       }
     }
@@ -632,7 +635,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
       a.out_w[item * IB + tid] = w;
       a.out_n[item * IB + tid] = (int)n;
     }
-    if (tid == 0) a.out_t[item] = ntrip;
+    const int nh = wave_sum_i(nhit);
+    if (tid == 0) a.out_t[item] = nh;
     __syncthreads();
   }
 }

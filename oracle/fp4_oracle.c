@@ -17,12 +17,23 @@
  *   Ind_* (iterative)    processes.f90:3254-3545, Ind_driver_4ph 3551-3822
  *   RTA_*                processes.f90:3828-4129, RTA_driver_4ph 5174-5311
  *   iteration0           iterations.f90:13-30
- *   iteration_4ph        iterations.f90:95-215 (4ph terms; 3ph lists empty)
+ *   iteration_4ph        iterations.f90:95-215 (3ph and 4ph terms)
  *   TConduct             conductivity.f90:11-37
+ *   symmetrisers on F_n  ShengBTE.f90:1288-1290
+ *   symmetrize_tensor    config.f90:503-518
  *
  * Float32-promoted literals of the reference (unsuffixed reals are default
  * REAL): 0.001 (sigma clamp), 3.37617087 (4ph rate units), 1e22 (hbarp),
  * 1e21 (kappa prefactor).  1.25, 0.5, 2. and 8.d0 are exact.
+ *
+ * Pinning.  The reference cannot be compiled in this image, so every
+ * routine here is checked by an independent route (brute-force enumeration,
+ * closed forms, invariants, dense linear algebra; tests/test_oracle*.py).
+ * Parity unpinned: which candidates exactly at the window edge the
+ * reference's compiled code keeps (this file follows its expression order and
+ * rounding, but only a run of the reference could confirm ties), and the
+ * reference's example count files, which need the harmonic front end to
+ * reproduce.
  */
 #include "fp4_oracle.h"
 
