@@ -132,6 +132,13 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
   return base + incl_w - v;
 }
 
+// Bit mask with bit i*nb set for every i < ib.
+constexpr unsigned long long rep_mask(int ib, int nb) {
+  unsigned long long r = 0ull;
+  for (int i = 0; i < ib; i++) r |= 1ull << (i * nb);
+  return r;
+}
+
 // Velocities of the tile's third and fourth phonons live in LDS, so sigma is
 // recomputed here with the same operations as pair_sigma().
 __device__ __forceinline__ double tile_sigma(const ModeView &m, int ch, const double *v3, const double *v4) {
@@ -225,6 +232,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   constexpr int QPT = QMAX / NT_DENSE;        // survivors per thread in a contraction
   static_assert(!MF || (sizeof(T) == 4 && 2 * TQ == 16), "MFMA path: float, 8 q'' per tile");
   static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
+  constexpr unsigned long long kJall = (1ull << NB) - 1ull;   // bits of all j of one i
+  constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
   extern __shared__ __align__(16) unsigned char dyn[];
   S &s = *reinterpret_cast<S *>(dyn);
   C2<T> *s_A = reinterpret_cast<C2<T> *>(dyn + a.smem_static);   // [ng][NOUT2]
@@ -274,6 +283,13 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
     int any = 0;
 #pragma unroll
     for (int il = 0; il < IB; il++) any |= s.act[il];
+    // (i, j) pairs allowed for the whole item: active mode i, nonzero w'_j
+    unsigned long long base_ok = 0ull;
+#pragma unroll
+    for (int il = 0; il < IB; il++)
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+        if (s.act[il] && s.Eii[j] != 0.0) base_ok |= 1ull << (il * NB + j);
 
     double acc_g[IB], acc_w[IB];
     int acc_n[IB];
@@ -431,19 +447,6 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
             s_ph[t * a.ng + gi] = p;
           }
           __syncthreads();
-          // exact sigma of every (t, k, l) of the tile
-          for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
-            const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
-            double sg = -1.0;
-            if (t < tq) {
-              const bool okkl = (s.Ejj[t][k] != 0.0) && (s.Ess[t][l] != 0.0) &&
-                                !(ch != 0 && s.ss[t] == s.jj[t] && l < k);
-              if (okkl) sg = tile_sigma(m, ch, s.vjj[t][k], s.vss[t][l]);
-            }
-            s.sig[t][k][l] = sg;
-          }
-          __syncthreads();
-
           // contraction of the queued survivors: for each (K,L) pair, Psi of
           // the tile (skinny GEMM over that pair's phase groups), then the
           // survivors' partial sums; finally weights and rates in FP64.
@@ -559,53 +562,91 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
             }
           };
 
-          // exact filter.  Lane task = (t, k, l, i), looping over j; each
-          // candidate is two FP64 additions in the reference's order,
-          // ((w +- w') +- w'') - w''', and one comparison with the exact sigma.
-          constexpr int NTASK = TQ * NB * NB * IB;
-          int qlen = 0;
-          for (int tb = 0; tb < NTASK; tb += NT_DENSE) {
-            const int task = tb + tid;
-            int mask = 0;
-            if (task < NTASK) {
-              const int il = task % IB, l = (task / IB) % NB, k = (task / (IB * NB)) % NB, t = task / (IB * NB * NB);
-              const double sg = s.sig[t][k][l];
-              if (sg >= 0.0 && s.act[il]) {
-                const int jj0 = s.jj[t], ss0 = s.ss[t];
-                const int i0 = ib * IB + il;
-                const double odp = s.Ejj[t][k], otp = s.Ess[t][l];
-                const bool exk = (ch == 1) && ((qi0 == jj0 && i0 == k) || (qi0 == ss0 && i0 == l));
-                if (!exk) {
-#pragma unroll
-                  for (int j = 0; j < NB; j++) {
-                    if (!(s.Eii[j] != 0.0)) continue;
-                    if (ch != 1 && jj0 == ii0 && k < j) continue;
-                    if (ch == 1 && ((ii0 == jj0 && j == k) || (ii0 == ss0 && j == l))) continue;
-                    const double x = s.x1[il][j];
-                    const double delta = ((ch == 0) ? (x + odp) : (x - odp)) - otp;
-                    if (fabs(delta) <= sg) mask |= 1 << j;
-                  }
+          // exact filter, fused with sigma: one thread per (t, k, l) computes
+          // the exact sigma of phonons 3 and 4 and tests every (i, j) of the
+          // tile, each candidate being two FP64 additions in the reference's
+          // order, ((w +- w') +- w'') - w''', and one comparison.  Survivors
+          // are queued in ascending code order (t, k, l, i, j); the queue is
+          // contracted whenever it holds QMAX codes and at the end.
+          constexpr int NKL = TQ * NB * NB;
+          static_assert(IB * NB <= 64, "candidate mask is 64 bits");
+          int qlen = 0, x0 = 0, done = 0, tot2 = 0, pos2 = 0;
+          unsigned long long mask = 0ull;
+          for (;;) {
+            if (done == tot2 && x0 < NKL) {
+              // next round of (t, k, l) tasks
+              const int x = x0 + tid;
+              mask = 0ull;
+              if (x < NKL) {
+                const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
+                double sg = -1.0;
+                if (t < tq) {
+                  const bool okkl = (s.Ejj[t][k] != 0.0) && (s.Ess[t][l] != 0.0) &&
+                                    !(ch != 0 && s.ss[t] == s.jj[t] && l < k);
+                  if (okkl) sg = tile_sigma(m, ch, s.vjj[t][k], s.vss[t][l]);
                 }
-                if (mask) s.hit[t] = 1;
+                s.sig[t][k][l] = sg;
+                if (sg >= 0.0) {
+                  const int jj0 = s.jj[t], ss0 = s.ss[t];
+                  // exclusions of the reference's loops as bit masks over (i, j)
+                  unsigned long long excl = 0ull;
+                  if (ch != 1 && jj0 == ii0) excl |= kRep * (kJall & ~((2ull << k) - 1ull));   // j > k
+                  if (ch == 1) {
+                    if (ii0 == jj0) excl |= kRep << k;                                         // j == k
+                    if (ii0 == ss0) excl |= kRep << l;                                         // j == l
+                    const int ilk = k - ib * IB, ill = l - ib * IB;
+                    if (qi0 == jj0 && ilk >= 0 && ilk < IB) excl |= kJall << (ilk * NB);      // i == k
+                    if (qi0 == ss0 && ill >= 0 && ill < IB) excl |= kJall << (ill * NB);      // i == l
+                  }
+                  const unsigned long long allowed = base_ok & ~excl;
+                  // delta = ((w +- w') +- w'') - w''' in the reference's order; x - w''
+                  // is evaluated as x + (-w''), which IEEE defines identically.
+                  const double odp = (ch == 0) ? s.Ejj[t][k] : -s.Ejj[t][k];
+                  const double otp = s.Ess[t][l];
+                  unsigned long long hitm = 0ull;
+#pragma unroll 1
+                  for (int il = 0; il < IB; il++) {
+                    double xv[NB];
+#pragma unroll
+                    for (int j = 0; j < NB; j++) xv[j] = s.x1[il][j];
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                      const double delta = (xv[j] + odp) - otp;
+                      hitm |= (unsigned long long)(fabs(delta) <= sg) << (il * NB + j);
+                    }
+                  }
+                  mask = hitm & allowed;
+                  if (mask) s.hit[t] = 1;
+                }
+              }
+              pos2 = block_excl_scan<NW>(__popcll(mask), s.wsum, tot2);
+              done = 0;
+              x0 += NT_DENSE;
+            }
+            // queue this round's survivors with positions done .. done+take-1
+            const int take = min(QMAX - qlen, tot2 - done);
+            {
+              const int code0 = (x0 - NT_DENSE + tid) * (IB * NB);   // ((t*NB + k)*NB + l) * IB * NB
+              unsigned long long mb = mask;
+              int p = pos2;
+              while (mb) {
+                const int b = __builtin_ctzll(mb);
+                mb &= mb - 1;
+                if (p >= done && p < done + take) s.q[qlen + p - done] = (unsigned short)(code0 + b);
+                p++;
               }
             }
-            int tot2;
-            const int pos2 = block_excl_scan<NW>(__popc(mask), s.wsum, tot2);
-            if (qlen + tot2 > QMAX) {
-              // queue full: contract what is queued, then continue
-              contract(qlen);
-              qlen = 0;
+            qlen += take;
+            done += take;
+            const bool last = (done == tot2) && (x0 >= NKL);
+            if (qlen == QMAX || (last && qlen > 0)) {
               __syncthreads();
+              contract(qlen);
+              __syncthreads();
+              qlen = 0;
             }
-            int o = qlen + pos2;
-#pragma unroll
-            for (int j = 0; j < NB; j++)
-              if ((mask >> j) & 1) s.q[o++] = (unsigned short)(task * NB + j);
-            qlen += tot2;
-            __syncthreads();
+            if (last) break;
           }
-          if (qlen > 0) contract(qlen);
-          __syncthreads();
           if (tid < tq && s.hit[tid]) nhit++;
         }
       }
