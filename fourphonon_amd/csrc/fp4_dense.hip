@@ -61,7 +61,7 @@ struct DenseState {
   FGroup *f = nullptr;
   DCluster *c = nullptr;
   double *phi = nullptr;
-  int *kl_first = nullptr;
+  int klf[17] = {};       // first coarse group of each (K, L) pair, natoms <= 4
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   void *outb = nullptr;
@@ -85,7 +85,8 @@ struct DenseArgs {
   const FGroup *f;
   const DCluster *c;
   const double *phi;
-  const int *kl_first;
+  int klf[17];           // first coarse group of each (K, L) pair, then ng
+  int roots_lds;         // the root table is copied to LDS (L <= kRootsLds)
   int ng;
   const int *lls;        // selected irreducible points (0-based ll), ascending
   int nls;
@@ -130,6 +131,27 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
   }
   __syncthreads();
   return base + incl_w - v;
+}
+
+// Dynamic LDS behind DenseSmem: A' [ng][NOUT2], the cluster coefficients
+// [ntri], the tile's phase operands, the root table (when L <= kRootsLds) and
+// the coarse groups' d vectors; each region starts on a 16-byte boundary.
+constexpr int kRootsLds = 256;
+struct DynLayout {
+  size_t A, c, ph, rt, gd, end;
+};
+__host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+template <class T, int NB, int IB, int TQ, bool MF>
+__host__ __device__ DynLayout dyn_layout(size_t stat, int ng, int ntri, int L_lds) {
+  DynLayout d{};
+  d.A = align16(stat);
+  d.c = align16(d.A + sizeof(C2<T>) * (size_t)ng * IB * NB * 9);
+  d.ph = align16(d.c + sizeof(C2<T>) * (size_t)ntri);
+  // MFMA path: the A operand of the Psi GEMM, [t][g][qq][part] floats
+  d.rt = align16(d.ph + (MF ? sizeof(float) * 4 : sizeof(C2<T>)) * (size_t)TQ * ng);
+  d.gd = align16(d.rt + sizeof(double2) * (size_t)L_lds);
+  d.end = align16(d.gd + sizeof(int) * 3 * (size_t)ng);
+  return d;
 }
 
 // Bit mask with bit i*nb set for every i < ib.
@@ -208,6 +230,10 @@ struct DenseSmem {
   double sig[TQ][NB][NB];  // exact sigma of (q'',k) / (q''',l), -1 when excluded
   int jj[TQ], ss[TQ];
   int hit[TQ];             // tile triple t has at least one surviving process
+  int qjj[TQ][3];          // grid coordinates of the tile's q''
+  double fb1[NB];          // Bose-Einstein factors of phonon 2 (the item's q')
+  double fb3[TQ][NB];      // ... of phonon 3 (the tile's q'')
+  double fb4[TQ][NB];      // ... of phonon 4 (the tile's q''')
   double2 e1[NB][IB];
   double2 e2[NB][NB];
   int vj[NT_DENSE], vs[NT_DENSE];
@@ -236,15 +262,23 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
   extern __shared__ __align__(16) unsigned char dyn[];
   S &s = *reinterpret_cast<S *>(dyn);
-  C2<T> *s_A = reinterpret_cast<C2<T> *>(dyn + a.smem_static);   // [ng][NOUT2]
-  C2<T> *s_c = s_A + (size_t)a.ng * NOUT2;                        // [ntri]
-  C2<T> *s_ph = s_c + a.ntri;                                     // [TQ][ng]
+  const DynLayout dl = dyn_layout<T, NB, IB, TQ, MF>(a.smem_static, a.ng, a.ntri, a.roots_lds ? a.L : 0);
+  C2<T> *s_A = reinterpret_cast<C2<T> *>(dyn + dl.A);     // [ng][NOUT2]
+  C2<T> *s_c = reinterpret_cast<C2<T> *>(dyn + dl.c);     // [ntri]
+  C2<T> *s_ph = reinterpret_cast<C2<T> *>(dyn + dl.ph);   // [TQ][ng] (VALU path)
+  float *s_phA = reinterpret_cast<float *>(dyn + dl.ph);  // [TQ][ng][2][2] (MFMA path)
+  double2 *s_rt = reinterpret_cast<double2 *>(dyn + dl.rt);
+  int *s_gd = reinterpret_cast<int *>(dyn + dl.gd);       // [ng][3]
 
   const ModeView &m = a.m;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double hbarp = kHbar * (double)k1e22;
   const double gpre = hbarp * hbarp * kPi / 8.0;
+  for (int x = tid; x < 3 * a.ng; x += NT_DENSE) s_gd[x] = a.g[x / 3].d[x % 3];
+  if (a.roots_lds)
+    for (int x = tid; x < a.L; x += NT_DENSE) s_rt[x] = a.roots[x];
+  __syncthreads();
 
   for (;;) {
     if (tid == 0) s.item = atomicAdd(a.counter, 1);
@@ -274,7 +308,10 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
       s.om[tid] = w;
       s.act[tid] = (a.sel[ll0 * NB + i0] != 0) && (w != 0.0) && (w <= m.omega_max);
     }
-    if (tid < NB) s.Eii[tid] = E(m, ii0, tid);
+    if (tid < NB) {
+      s.Eii[tid] = E(m, ii0, tid);
+      s.fb1[tid] = bose(s.Eii[tid], a.T);
+    }
     __syncthreads();
     if (tid < IB * NB) {
       const int il = tid / NB, j = tid % NB;
@@ -402,19 +439,23 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
         __syncthreads();
         for (int t0 = 0; t0 < tot; t0 += TQ) {
           const int tq = min(TQ, tot - t0);
+          // tile data straight from the scan buffer (one round of global loads):
+          // q'' and q''' of the tile, their grid coordinates, eigenvectors of
+          // phonons 3 and 4, frequencies, Bose-Einstein factors and velocities
           if (tid < TQ) {
-            s.jj[tid] = (tid < tq) ? s.vj[t0 + tid] : 0;
+            const int jv = (tid < tq) ? s.vj[t0 + tid] : 0;
+            s.jj[tid] = jv;
             s.ss[tid] = (tid < tq) ? s.vs[t0 + tid] : 0;
             s.hit[tid] = 0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) s.qjj[tid][c] = m.ijk[3 * jv + c];
           }
-          __syncthreads();
-          // tile data: eigenvectors of phonons 3 and 4, frequencies, velocities, q'' phases
           for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
             const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
             C2<T> v3 = {(T)0, (T)0}, v4 = {(T)0, (T)0};
             if (t < tq) {
-              const double2 e3 = m.eig[((size_t)s.jj[t] * NB + b) * NB + comp];
-              const double2 e4 = m.eig[((size_t)s.ss[t] * NB + b) * NB + comp];
+              const double2 e3 = m.eig[((size_t)s.vj[t0 + t] * NB + b) * NB + comp];
+              const double2 e4 = m.eig[((size_t)s.vs[t0 + t] * NB + b) * NB + comp];
               v3 = C2<T>{(T)e3.x, (T)((ch != 0) ? -e3.y : e3.y)};
               v4 = C2<T>{(T)e4.x, (T)(-e4.y)};
             }
@@ -424,9 +465,11 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           for (int x = tid; x < TQ * NB; x += NT_DENSE) {
             const int t = x / NB, b = x % NB;
             if (t < tq) {
-              const int jv = s.jj[t], sv = s.ss[t];
+              const int jv = s.vj[t0 + t], sv = s.vs[t0 + t];
               s.Ejj[t][b] = E(m, jv, b);
               s.Ess[t][b] = E(m, sv, b);
+              s.fb3[t][b] = bose(s.Ejj[t][b], a.T);
+              s.fb4[t][b] = bose(s.Ess[t][b], a.T);
 #pragma unroll
               for (int d = 0; d < 3; d++) {
                 s.vjj[t][b][d] = Vel(m, jv, b, d);
@@ -434,17 +477,26 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               }
             }
           }
+          __syncthreads();
           for (int x = tid; x < TQ * a.ng; x += NT_DENSE) {
             const int t = x / a.ng, gi = x % a.ng;
-            C2<T> p = {(T)0, (T)0};
+            double2 rt = make_double2(0.0, 0.0);
             if (t < tq) {
-              const int vjt = s.jj[t];
-              const DGroup G = a.g[gi];
-              const int k = s3 * (m.ijk[3 * vjt] * G.d[0] + m.ijk[3 * vjt + 1] * G.d[1] + m.ijk[3 * vjt + 2] * G.d[2]);
-              const double2 rt = a.roots[fmod_pos(k, a.L)];
-              p = C2<T>{(T)rt.x, (T)rt.y};
+              const int k = s3 * (s.qjj[t][0] * s_gd[3 * gi] + s.qjj[t][1] * s_gd[3 * gi + 1] +
+                                  s.qjj[t][2] * s_gd[3 * gi + 2]);
+              const int kk = fmod_pos(k, a.L);
+              rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
             }
-            s_ph[t * a.ng + gi] = p;
+            if constexpr (MF) {
+              // A operand rows (t, part), k = (g, qq): (re, im | -im, re)
+              float *o = s_phA + (size_t)x * 4;
+              o[0] = (float)rt.x;
+              o[1] = (float)rt.y;
+              o[2] = -(float)rt.y;
+              o[3] = (float)rt.x;
+            } else {
+              s_ph[x] = C2<T>{(T)rt.x, (T)rt.y};
+            }
           }
           __syncthreads();
           // contraction of the queued survivors: for each (K,L) pair, Psi of
@@ -452,31 +504,50 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           // survivors' partial sums; finally weights and rates in FP64.
           auto contract = [&](int qlen) {
             C2<T> V[QPT];
+            // survivor u of this thread: packed (psi row offset, t, k, l), -1 when empty
+            int info[QPT];
 #pragma unroll
-            for (int u = 0; u < QPT; u++) V[u] = C2<T>{(T)0, (T)0};
+            for (int u = 0; u < QPT; u++) {
+              V[u] = C2<T>{(T)0, (T)0};
+              const int qi = tid + u * NT_DENSE;
+              info[u] = -1;
+              if (qi < qlen) {
+                const int cc = s.q[qi];
+                const int j = cc % NB, il = (cc / NB) % IB, l = (cc / (NB * IB)) % NB;
+                const int k = (cc / (NB * IB * NB)) % NB, t = cc / (NB * IB * NB * NB);
+                info[u] = ((t * NOUT2 + (il * NB + j) * 9) << 12) | (t << 8) | (k << 4) | l;
+              }
+            }
             for (int kl = 0; kl < NA * NA; kl++) {
-              const int g0 = a.kl_first[kl], g1 = a.kl_first[kl + 1];
+              const int g0 = a.klf[kl], g1 = a.klf[kl + 1];
               if (g1 == g0) continue;
               if constexpr (MF) {
                 constexpr int NTILE = (NOUT2 + 15) / 16;
                 const int ksteps = (2 * (g1 - g0) + 3) / 4;
                 const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
+                const float *Af = reinterpret_cast<const float *>(s_A);   // [g][n][re, im]
                 for (int nt = wave; nt < NTILE; nt += NW) {
                   const int n = nt * 16 + (lane & 15);
+                  const bool nin = n < NOUT2;
+                  const int nc = nin ? n : NOUT2 - 1;
                   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-                  for (int ks = 0; ks < ksteps; ks++) {
-                    const int kp = 4 * ks + kg;
-                    const int g = g0 + (kp >> 1), qq = kp & 1;
-                    float av = 0.f, bv = 0.f;
-                    if (g < g1) {
-                      const C2<T> p = s_ph[t * a.ng + g];
-                      av = part == 0 ? (qq == 0 ? (float)p.re : -(float)p.im) : (qq == 0 ? (float)p.im : (float)p.re);
-                      if (n < NOUT2) {
-                        const C2<T> A = s_A[(size_t)g * NOUT2 + n];
-                        bv = qq == 0 ? (float)A.re : (float)A.im;
-                      }
+                  // operands of four k-steps are loaded before their MFMAs
+                  for (int ks0 = 0; ks0 < ksteps; ks0 += 4) {
+                    float av[4], bv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                      const int kp = 4 * (ks0 + u) + kg;
+                      const int g = g0 + (kp >> 1), qq = kp & 1;
+                      const bool ok = g < g1;
+                      const int gc = ok ? g : g0;
+                      const float x = s_phA[((t * a.ng + gc) * 2 + qq) * 2 + part];
+                      const float y = Af[((size_t)gc * NOUT2 + nc) * 2 + qq];
+                      av[u] = ok ? x : 0.f;
+                      bv[u] = (ok && nin) ? y : 0.f;
                     }
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                      if (ks0 + u < ksteps) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
                   }
                   if (n < NOUT2) {
 #pragma unroll
@@ -511,12 +582,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               const int K = kl / NA, L = kl % NA;
 #pragma unroll
               for (int u = 0; u < QPT; u++) {
-                const int qi = tid + u * NT_DENSE;
-                if (qi < qlen) {
-                  const int cc = s.q[qi];
-                  const int j = cc % NB, il = (cc / NB) % IB, l = (cc / (NB * IB)) % NB;
-                  const int k = (cc / (NB * IB * NB)) % NB, t = cc / (NB * IB * NB * NB);
-                  const C2<T> *psi = &s.u.psi[t * NOUT2 + (il * NB + j) * 9];
+                if (info[u] >= 0) {
+                  const int t = (info[u] >> 8) & 15, k = (info[u] >> 4) & 15, l = info[u] & 15;
+                  const C2<T> *psi = &s.u.psi[info[u] >> 12];
 #pragma unroll
                   for (int sx = 0; sx < 3; sx++) {
                     C2<T> inner = {(T)0, (T)0};
@@ -545,7 +613,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                 const double delta = mismatch(ch, w, op, odp, otp);
                 double sigma = s.sig[t][k][l];
                 if (sigma <= (double)kClamp) sigma = 1.0 / sqrt(kPi);
-                const double wpv = wp4_of(ch, w, op, odp, otp, delta, sigma, a.T);
+                const double wpv = wp4_given(ch, w, op, odp, otp, s.fb1[j], s.fb3[t][k], s.fb4[t][l], delta, sigma);
                 double g = 0.0;
                 if (!a.onlyharmonic) {
                   const double vr = (double)V[u].re, vi = (double)V[u].im;
@@ -771,14 +839,14 @@ DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<do
             hipMalloc(&s->f, sizeof(FGroup) * F.size()) == hipSuccess &&
             hipMalloc(&s->c, sizeof(DCluster) * n) == hipSuccess &&
             hipMalloc(&s->phi, sizeof(double) * 81 * n) == hipSuccess &&
-            hipMalloc(&s->kl_first, sizeof(int) * klf.size()) == hipSuccess &&
             hipMalloc(&s->counter, sizeof(int)) == hipSuccess && hipEventCreate(&s->e0) == hipSuccess &&
             hipEventCreate(&s->e1) == hipSuccess;
   ok = ok && hipMemcpy(s->g, G.data(), sizeof(DGroup) * G.size(), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(s->f, F.data(), sizeof(FGroup) * F.size(), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(s->c, Cs.data(), sizeof(DCluster) * n, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(s->phi, P.data(), sizeof(double) * 81 * n, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(s->kl_first, klf.data(), sizeof(int) * klf.size(), hipMemcpyHostToDevice) == hipSuccess;
+       hipMemcpy(s->phi, P.data(), sizeof(double) * 81 * n, hipMemcpyHostToDevice) == hipSuccess;
+  if (natoms <= 4)
+    for (size_t q = 0; q < klf.size(); q++) s->klf[q] = klf[q];
   if (!ok) {
     dense_destroy(s);
     *rc = derr(FP4_ERR_NOMEM, "dense_create: device allocation or copy failed");
@@ -789,7 +857,7 @@ DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<do
 
 void dense_destroy(DenseState *s) {
   if (!s) return;
-  void *ps[] = {s->g, s->f, s->c, s->phi, s->kl_first, s->scratch, s->outb, s->counter};
+  void *ps[] = {s->g, s->f, s->c, s->phi, s->scratch, s->outb, s->counter};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (s->e0) (void)hipEventDestroy(s->e0);
@@ -829,16 +897,16 @@ size_t static_bytes() {
   return (sizeof(DenseSmem<T, NB, IB, TQ>) + 15) & ~size_t(15);
 }
 
-template <class T, int NB, int IB, int TQ>
-size_t lds_bytes(int ng, int ntri) {
-  return static_bytes<T, NB, IB, TQ>() + sizeof(C2<T>) * ((size_t)ng * IB * NB * 9 + ntri + (size_t)TQ * ng);
+template <class T, int NB, int IB, int TQ, bool MF>
+size_t lds_bytes(int ng, int ntri, int L) {
+  return dyn_layout<T, NB, IB, TQ, MF>(static_bytes<T, NB, IB, TQ>(), ng, ntri, L <= kRootsLds ? L : 0).end;
 }
 
 // One kernel shape: modes per block (IB) and q'' per tile (TQ).
 struct Shape {
   int prec, nb, IB, TQ;
   bool mfma;
-  size_t (*bytes)(int, int);
+  size_t (*bytes)(int, int, int);
   size_t (*stat)();
   void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
 };
@@ -851,7 +919,7 @@ void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
 }
 
 #define FP4_SHAPE(T, P, NB, IB, TQ, MF) \
-  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>}
+  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>}
 
 // In order of preference; the first shape whose LDS fits is used.  The FP32
 // shapes with 8 q'' per tile run the Psi GEMM on MFMA.
@@ -864,9 +932,9 @@ const Shape kShapes[] = {
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, false),
 };
 
-const Shape *pick_shape(int prec, int nb, int ng, int ntri, size_t lds_max) {
+const Shape *pick_shape(int prec, int nb, int ng, int ntri, int L, size_t lds_max) {
   for (const Shape &s : kShapes)
-    if (s.prec == prec && s.nb == nb && s.bytes(ng, ntri) <= lds_max) return &s;
+    if (s.prec == prec && s.nb == nb && s.bytes(ng, ntri, L) <= lds_max) return &s;
   return nullptr;
 }
 
@@ -891,7 +959,8 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   int dev = 0, ldsmax = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  const Shape *shp = pick_shape(r.precision, nb, s->ng, s->nc, (size_t)ldsmax);
+  if (s->natoms > 4) return derr(FP4_ERR_UNSUPPORTED, "dense form: more than 4 atoms per cell");
+  const Shape *shp = pick_shape(r.precision, nb, s->ng, s->nc, s->L, (size_t)ldsmax);
   if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
   const Shape &sh = *shp;
   s->last_mfma = sh.mfma;
@@ -949,7 +1018,8 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.f = s->f;
   a.c = s->c;
   a.phi = s->phi;
-  a.kl_first = s->kl_first;
+  for (int q = 0; q < 17; q++) a.klf[q] = s->klf[q];
+  a.roots_lds = s->L <= kRootsLds;
   a.ng = s->ng;
   a.lls = dlls;
   a.nls = nls;
@@ -974,7 +1044,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.out_n = on;
   a.out_t = ot;
   if (hipEventRecord(s->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  sh.launch(a, slots, sh.bytes(s->ng, s->nc), st);
+  sh.launch(a, slots, sh.bytes(s->ng, s->nc, s->L), st);
   if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense kernel launch failed");
   if (hipEventRecord(s->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
   hipLaunchKernelGGL(k_dense_reduce, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, st, og, ow, on, nseg, sh.IB,

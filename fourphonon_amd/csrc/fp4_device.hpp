@@ -157,10 +157,11 @@ __device__ __forceinline__ int partner(const ModeView &m, int ch, const int *q, 
 
 __device__ __forceinline__ double bose(double w, double T) { return 1.0 / (exp(kHbar * w / kKb / T) - 1.0); }
 
-// WP4 of one process (processes.f90:3895-3901, 3988-3994, 4095-4101)
-__device__ __forceinline__ double wp4_of(int ch, double w, double wp, double wdp, double wtp, double delta,
-                                         double sigma, double T) {
-  const double fp = bose(wp, T), fdp = bose(wdp, T), ftp = bose(wtp, T);
+// WP4 of one process (processes.f90:3895-3901, 3988-3994, 4095-4101), with
+// the Bose-Einstein factors fp, fdp, ftp of phonons 2..4 given (bose() of
+// their frequencies).
+__device__ __forceinline__ double wp4_given(int ch, double w, double wp, double wdp, double wtp, double fp,
+                                            double fdp, double ftp, double delta, double sigma) {
   double occ;
   if (ch == 0)
     occ = fp * fdp * (1 + ftp) - (1 + fp) * (1 + fdp) * ftp;
@@ -171,6 +172,11 @@ __device__ __forceinline__ double wp4_of(int ch, double w, double wp, double wdp
   double r = occ * exp(-(delta * delta) / (sigma * sigma)) / sigma / sqrt(kPi) / (w * wp * wdp * wtp);
   if (wp <= 1.25 || wdp <= 1.25 || wtp <= 1.25) r = 0.0;
   return r;
+}
+
+__device__ __forceinline__ double wp4_of(int ch, double w, double wp, double wdp, double wtp, double delta,
+                                         double sigma, double T) {
+  return wp4_given(ch, w, wp, wdp, wtp, bose(wp, T), bose(wdp, T), bose(wtp, T), delta, sigma);
 }
 
 // Deterministic wave-wide sums (fixed butterfly order).
