@@ -129,6 +129,10 @@ ModeView mode_view(const fp4_ctx *c) {
   for (int nu = 0; nu < 3; nu++)
     for (int k = 0; k < 3; k++) m.bf[3 * nu + k] = (float)(c->rlattvec[3 * nu + k] / c->ngrid[nu]);
   m.sbf = (float)std::fabs(c->scalebroad);
+  for (int nu = 0; nu < 3; nu++) {
+    const int n = c->ngrid[nu];
+    m.inv_n[nu] = (n > 0 && (n & (n - 1)) == 0) ? 1.0 / (double)n : 0.0;
+  }
   return m;
 }
 

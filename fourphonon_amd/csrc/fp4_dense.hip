@@ -64,6 +64,8 @@ struct DenseState {
   int klf[17] = {};       // first coarse group of each (K, L) pair, natoms <= 4
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  void *fbose = nullptr;   // [Nb][nptk] Bose-Einstein factors at the last T
+  size_t fbose_bytes = 0;
   void *outb = nullptr;
   size_t out_bytes = 0;
   int *counter = nullptr;
@@ -86,6 +88,7 @@ struct DenseArgs {
   const DCluster *c;
   const double *phi;
   int klf[17];           // first coarse group of each (K, L) pair, then ng
+  const double *fbose;   // [Nb][nptk] Bose-Einstein factor of every mode at T
   int roots_lds;         // the root table is copied to LDS (L <= kRootsLds)
   int ng;
   const int *lls;        // selected irreducible points (0-based ll), ascending
@@ -310,7 +313,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
     }
     if (tid < NB) {
       s.Eii[tid] = E(m, ii0, tid);
-      s.fb1[tid] = bose(s.Eii[tid], a.T);
+      s.fb1[tid] = a.fbose[(size_t)tid * m.nptk + ii0];
     }
     __syncthreads();
     if (tid < IB * NB) {
@@ -468,8 +471,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               const int jv = s.vj[t0 + t], sv = s.vs[t0 + t];
               s.Ejj[t][b] = E(m, jv, b);
               s.Ess[t][b] = E(m, sv, b);
-              s.fb3[t][b] = bose(s.Ejj[t][b], a.T);
-              s.fb4[t][b] = bose(s.Ess[t][b], a.T);
+              s.fb3[t][b] = a.fbose[(size_t)b * m.nptk + jv];
+              s.fb4[t][b] = a.fbose[(size_t)b * m.nptk + sv];
 #pragma unroll
               for (int d = 0; d < 3; d++) {
                 s.vjj[t][b][d] = Vel(m, jv, b, d);
@@ -526,37 +529,55 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                 const int ksteps = (2 * (g1 - g0) + 3) / 4;
                 const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
                 const float *Af = reinterpret_cast<const float *>(s_A);   // [g][n][re, im]
-                for (int nt = wave; nt < NTILE; nt += NW) {
-                  const int n = nt * 16 + (lane & 15);
-                  const bool nin = n < NOUT2;
-                  const int nc = nin ? n : NOUT2 - 1;
-                  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-                  // operands of four k-steps are loaded before their MFMAs
-                  for (int ks0 = 0; ks0 < ksteps; ks0 += 4) {
-                    float av[4], bv[4];
+                // each wave owns column tiles wave, wave + NW, ...; their MFMA
+                // chains are independent and share the A operand, so they are
+                // interleaved, with the operands of four k-steps loaded first
+                constexpr int TPW = (NTILE + NW - 1) / NW;
+                int ncol[TPW];
+                bool nin[TPW];
+                floatx4 acc[TPW];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                      const int kp = 4 * (ks0 + u) + kg;
-                      const int g = g0 + (kp >> 1), qq = kp & 1;
-                      const bool ok = g < g1;
-                      const int gc = ok ? g : g0;
-                      const float x = s_phA[((t * a.ng + gc) * 2 + qq) * 2 + part];
-                      const float y = Af[((size_t)gc * NOUT2 + nc) * 2 + qq];
-                      av[u] = ok ? x : 0.f;
-                      bv[u] = (ok && nin) ? y : 0.f;
+                for (int c = 0; c < TPW; c++) {
+                  const int n = (wave + NW * c) * 16 + (lane & 15);
+                  nin[c] = n < NOUT2;
+                  ncol[c] = nin[c] ? n : NOUT2 - 1;
+                  acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+                }
+                for (int ks0 = 0; ks0 < ksteps; ks0 += 4) {
+                  float av[4], bv[TPW][4];
+#pragma unroll
+                  for (int u = 0; u < 4; u++) {
+                    const int kp = 4 * (ks0 + u) + kg;
+                    const int g = g0 + (kp >> 1), qq = kp & 1;
+                    const bool ok = g < g1;
+                    const int gc = ok ? g : g0;
+                    const float x = s_phA[((t * a.ng + gc) * 2 + qq) * 2 + part];
+                    av[u] = ok ? x : 0.f;
+#pragma unroll
+                    for (int c = 0; c < TPW; c++) {
+                      const float y = Af[((size_t)gc * NOUT2 + ncol[c]) * 2 + qq];
+                      bv[c][u] = (ok && nin[c]) ? y : 0.f;
                     }
-#pragma unroll
-                    for (int u = 0; u < 4; u++)
-                      if (ks0 + u < ksteps) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
                   }
+#pragma unroll
+                  for (int u = 0; u < 4; u++)
+                    if (ks0 + u < ksteps)
+#pragma unroll
+                      for (int c = 0; c < TPW; c++)
+                        if (wave + NW * c < NTILE)
+                          acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[c][u], acc[c], 0, 0, 0);
+                }
+#pragma unroll
+                for (int c = 0; c < TPW; c++) {
+                  const int n = (wave + NW * c) * 16 + (lane & 15);
                   if (n < NOUT2) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                       const int row = 4 * (lane >> 4) + r;
                       if ((row & 1) == 0)
-                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)acc[r];
+                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)acc[c][r];
                       else
-                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)acc[r];
+                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)acc[c][r];
                     }
                   }
                 }
@@ -638,12 +659,14 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           // contracted whenever it holds QMAX codes and at the end.
           constexpr int NKL = TQ * NB * NB;
           static_assert(IB * NB <= 64, "candidate mask is 64 bits");
-          int qlen = 0, x0 = 0, done = 0, tot2 = 0, pos2 = 0;
+          int qlen = 0, x0 = 0, done = 0, tot2 = 0, pos2 = 0, x = 0;
           unsigned long long mask = 0ull;
           for (;;) {
             if (done == tot2 && x0 < NKL) {
-              // next round of (t, k, l) tasks
-              const int x = x0 + tid;
+              // next round of (t, k, l) tasks, spread evenly over the waves
+              // (task order still follows thread order)
+              const int per = min(64, (NKL - x0 + NW - 1) / NW);
+              x = (lane < per) ? x0 + wave * per + lane : NKL;
               mask = 0ull;
               if (x < NKL) {
                 const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
@@ -689,12 +712,12 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               }
               pos2 = block_excl_scan<NW>(__popcll(mask), s.wsum, tot2);
               done = 0;
-              x0 += NT_DENSE;
+              x0 += NW * min(64, (NKL - x0 + NW - 1) / NW);
             }
             // queue this round's survivors with positions done .. done+take-1
             const int take = min(QMAX - qlen, tot2 - done);
             {
-              const int code0 = (x0 - NT_DENSE + tid) * (IB * NB);   // ((t*NB + k)*NB + l) * IB * NB
+              const int code0 = x * (IB * NB);   // ((t*NB + k)*NB + l) * IB * NB
               unsigned long long mb = mask;
               int p = pos2;
               while (mb) {
@@ -751,6 +774,12 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
 }
 
 // Sum over the q' window of each (channel, point, mode-block, mode), in order.
+// Bose-Einstein factor of every mode (the bose() the per-process form uses).
+__global__ void k_bose(const double *energy, long n, double T, double *out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = bose(energy[t], T);
+}
+
 __global__ void k_dense_reduce(const double *og, const double *ow, const int *on, long nseg, int ib_sz, int nii,
                                double *sg, double *sw, long long *sn) {
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -857,7 +886,7 @@ DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<do
 
 void dense_destroy(DenseState *s) {
   if (!s) return;
-  void *ps[] = {s->g, s->f, s->c, s->phi, s->scratch, s->outb, s->counter};
+  void *ps[] = {s->g, s->f, s->c, s->phi, s->scratch, s->outb, s->fbose, s->counter};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (s->e0) (void)hipEventDestroy(s->e0);
@@ -1010,6 +1039,11 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
       hipMemcpyAsync(dsel, sel.data(), nmodes_all, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemsetAsync(s->counter, 0, sizeof(int), st) != hipSuccess)
     return derr(FP4_ERR_DEVICE, "dense: upload failed");
+  const long nmt = (long)nb * r.m.nptk;
+  if ((rc = ensure(s->fbose, s->fbose_bytes, sizeof(double) * nmt))) return rc;
+  hipLaunchKernelGGL(k_bose, dim3((unsigned)((nmt + 255) / 256)), dim3(256), 0, st, r.m.energy, nmt, r.T,
+                     reinterpret_cast<double *>(s->fbose));
+  if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense: Bose-Einstein table launch failed");
   DenseArgs a{};
   a.m = r.m;
   a.roots = r.roots;
@@ -1020,6 +1054,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.phi = s->phi;
   for (int q = 0; q < 17; q++) a.klf[q] = s->klf[q];
   a.roots_lds = s->L <= kRootsLds;
+  a.fbose = reinterpret_cast<const double *>(s->fbose);
   a.ng = s->ng;
   a.lls = dlls;
   a.nls = nls;

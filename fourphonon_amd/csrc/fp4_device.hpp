@@ -55,6 +55,7 @@ struct ModeView {
   double omega_max;
   float bf[9];     // rlattvec(:,nu)/ngrid(nu) in float, for the sigma estimate
   float sbf;       // |scalebroad| in float
+  double inv_n[3]; // 1/ngrid(nu) when ngrid(nu) is a power of two, else 0
 };
 
 struct IfcView {
@@ -92,7 +93,8 @@ __device__ __forceinline__ double base_sigma(const ModeView &m, double v0, doubl
     d = d + m.b[3 * nu + 0] * v0;
     d = d + m.b[3 * nu + 1] * v1;
     d = d + m.b[3 * nu + 2] * v2;
-    const double t = d / (double)ng[nu];
+    // d / N; for N = 2^k the product with 2^-k rounds the same exact value
+    const double t = (m.inv_n[nu] != 0.0) ? d * m.inv_n[nu] : d / (double)ng[nu];
     s[nu] = t * t;
   }
   double mx = s[0];
