@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix peak
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0    # dense BF16 matrix peak
 
 
 def log(*a):
@@ -121,6 +122,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
+    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32",
+                    help="MFMA form of the timed steps: FP32 MFMA or BF16 hi/lo MFMA (FP32 accumulate)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -148,14 +151,15 @@ def main():
     nproc_total = int(Npp.sum() + Npm.sum() + Nmm.sum())
     log(f"prefilter: {nproc_total} processes in {t_pref:.2f}s")
 
-    # ---- timed dense FP32 steps over the whole workload
+    # ---- timed dense (MFMA) steps over the whole workload
+    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
     for w in range(args.warmup):
-        ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP32)
+        ctx.RTA_driver_4ph_modes(all_modes, prec)
         log(f"warmup {w + 1}/{args.warmup} done")
     ctx.reset_stats()
     t1 = time.perf_counter()
     for s in range(args.steps):
-        r32, nproc = ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP32)
+        r32, nproc = ctx.RTA_driver_4ph_modes(all_modes, prec)
         log(f"step {s + 1}/{args.steps} done")
     wall = time.perf_counter() - t1
     assert nproc == nproc_total, (nproc, nproc_total)
@@ -171,9 +175,10 @@ def main():
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
     shape = ctx.dense_shape()
-    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{'mfma' if shape['mfma'] else 'valu'}>"
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(kname),
+    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{['valu', 'mfma', 'mfma-bf16x3'][shape['mfma']]}>"
+    peak = FP32_PEAK_TFLOPS if args.precision == "fp32" else BF16_PEAK_TFLOPS
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname),
                 "kernel": kname, "kernel_ms": round(avg_ms, 3), "launches": k_n,
                 "flops_per_launch": flops, "flops_per_triple": flops / max(ntriples, 1),
                 "executed_flops_per_launch": executed,
@@ -183,7 +188,7 @@ def main():
         "metric": "4ph processes/s on 1 MI355X, MFMA form vs FP64 form, kappa within 1e-4 of FP64",
         "value": value, "unit": "processes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
+        "dtype": "fp32" if args.precision == "fp32" else "bf16x3 (bf16 hi/lo MFMA, fp32 accumulate)", "data": "synthetic",
         "config": {"workload": "diamond 16x16x16, 2nd-shell 4th-order IFCs (Ntri=1522), 4ph RTA at 300 K, "
                                "all 870 modes", "ngrid": [16, 16, 16], "nbands": 6, "nlist": int(m.Nlist),
                    "ntri4": int(case.ifc4.Ntri), "processes": nproc_total, "triples": ntriples},
@@ -205,7 +210,7 @@ def main():
         ms_emit, _ = ctx.kernel_stats(rt.Context.KS_EMIT)
         ms_cnt, _ = ctx.kernel_stats(rt.Context.KS_PREFILTER)
         t1 = time.perf_counter()
-        r32s, n32s = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP32, ii_lo=lo, ii_hi=hi)
+        r32s, n32s = ctx.RTA_driver_4ph_modes(sample, prec, ii_lo=lo, ii_hi=hi)
         t32 = time.perf_counter() - t1
         assert n32s == n64s
         fp64_pp = n64s / t64

@@ -144,14 +144,14 @@ struct DynLayout {
   size_t A, c, ph, rt, gd, end;
 };
 __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
-template <class T, int NB, int IB, int TQ, bool MF>
+template <class T, int NB, int IB, int TQ, int MF>
 __host__ __device__ DynLayout dyn_layout(size_t stat, int ng, int ntri, int L_lds) {
   DynLayout d{};
   d.A = align16(stat);
   d.c = align16(d.A + sizeof(C2<T>) * (size_t)ng * IB * NB * 9);
   d.ph = align16(d.c + sizeof(C2<T>) * (size_t)ntri);
   // MFMA path: the A operand of the Psi GEMM, [t][g][qq][part] floats
-  d.rt = align16(d.ph + (MF ? sizeof(float) * 4 : sizeof(C2<T>)) * (size_t)TQ * ng);
+  d.rt = align16(d.ph + (MF != 0 ? sizeof(float) * 4 : sizeof(C2<T>)) * (size_t)TQ * ng);
   d.gd = align16(d.rt + sizeof(double2) * (size_t)L_lds);
   d.end = align16(d.gd + sizeof(int) * 3 * (size_t)ng);
   return d;
@@ -214,6 +214,53 @@ __global__ void k_mfma_selftest(double *err) {
   if (l == 0) *err = e;
 }
 
+// BF16 operands: round-to-nearest-even to the upper 16 bits of a float.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned bf16_rn(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_val(unsigned h) { return __uint_as_float(h << 16); }
+// f = hi + lo + O(2^-16 |f|): the hi/lo split of one float
+__device__ __forceinline__ void bf16_split(float f, unsigned &hi, unsigned &lo) {
+  hi = bf16_rn(f);
+  lo = bf16_rn(f - bf16_val(hi));
+}
+// two (lo16 = k, hi16 = k+1) words -> the 4 consecutive-k operand of one lane
+__device__ __forceinline__ bf16x4 bf16_pack(unsigned w0, unsigned w1) {
+  bf16x4 v;
+  v[0] = (short)(w0 & 0xFFFFu);
+  v[1] = (short)(w0 >> 16);
+  v[2] = (short)(w1 & 0xFFFFu);
+  v[3] = (short)(w1 >> 16);
+  return v;
+}
+
+// v_mfma_f32_16x16x16bf16_1k lane layout used by the BF16 form: A[i][k] in
+// lane i + 16*(k/4), element k%4; B[k][j] in lane j + 16*(k/4), element k%4;
+// D as for the FP32 instruction.  Checked exactly with small integers.
+__global__ void k_mfma_bf16_selftest(double *err) {
+  const int l = threadIdx.x;
+  const int i = l & 15, kg = l >> 4;
+  bf16x4 av, bv;
+  for (int q = 0; q < 4; q++) {
+    const int k = 4 * kg + q;
+    av[q] = (short)bf16_rn((float)(1 + i + 3 * k));
+    bv[q] = (short)bf16_rn((float)(2 + 5 * (l & 15) - k));
+  }
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+  double e = 0.0;
+  for (int r = 0; r < 4; r++) {
+    const int ii = 4 * (l >> 4) + r, jj = l & 15;
+    double want = 0.0;
+    for (int k = 0; k < 16; k++) want += (double)(1 + ii + 3 * k) * (double)(2 + 5 * jj - k);
+    e = fmax(e, fabs(want - (double)acc[r]));
+  }
+  for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+  if (l == 0) *err = e;
+}
+
 template <class T, int NB, int IB, int TQ>
 struct DenseSmem {
   static constexpr int NOUT2 = IB * NB * 9;   // A' entries per group = Psi entries per (K,L) and triple
@@ -250,7 +297,8 @@ struct DenseSmem {
   double red[NT_DENSE / 64][3 * IB];
 };
 
-template <class T, int NB, int IB, int TQ, bool MF>
+// MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi.
+template <class T, int NB, int IB, int TQ, int MF>
 __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   using S = DenseSmem<T, NB, IB, TQ>;
   constexpr int NA = NB / 3;
@@ -259,7 +307,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   constexpr int NOUT2 = S::NOUT2;
   constexpr int NW = NT_DENSE / 64;
   constexpr int QPT = QMAX / NT_DENSE;        // survivors per thread in a contraction
-  static_assert(!MF || (sizeof(T) == 4 && 2 * TQ == 16), "MFMA path: float, 8 q'' per tile");
+  static_assert(MF == 0 || (sizeof(T) == 4 && 2 * TQ == 16), "MFMA path: float, 8 q'' per tile");
   static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
   constexpr unsigned long long kJall = (1ull << NB) - 1ull;   // bits of all j of one i
   constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
@@ -417,7 +465,18 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           }
           __syncthreads();
         }
-        if (tid < NOUT2) s_A[(size_t)gi * NOUT2 + tid] = C2<T>{(T)acc.x, (T)acc.y};
+        if (tid < NOUT2) {
+          if constexpr (MF == 2) {
+            unsigned rh, rl, ih, il;
+            bf16_split((float)acc.x, rh, rl);
+            bf16_split((float)acc.y, ih, il);
+            unsigned *bh = reinterpret_cast<unsigned *>(s_A);
+            bh[(size_t)gi * NOUT2 + tid] = rh | (ih << 16);
+            bh[(size_t)a.ng * NOUT2 + (size_t)gi * NOUT2 + tid] = rl | (il << 16);
+          } else {
+            s_A[(size_t)gi * NOUT2 + tid] = C2<T>{(T)acc.x, (T)acc.y};
+          }
+        }
       }
       __syncthreads();
 
@@ -490,7 +549,18 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               const int kk = fmod_pos(k, a.L);
               rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
             }
-            if constexpr (MF) {
+            if constexpr (MF == 2) {
+              // A operand rows (t, part), k = (g, qq): part 0 (re, -im), part 1 (im, re), hi and lo planes
+              unsigned rh, rl, ih, il;
+              bf16_split((float)rt.x, rh, rl);
+              bf16_split((float)rt.y, ih, il);
+              unsigned *ph = reinterpret_cast<unsigned *>(s_phA);
+              unsigned *pl = ph + (size_t)TQ * a.ng * 2;
+              ph[2 * x] = rh | ((ih ^ 0x8000u) << 16);
+              ph[2 * x + 1] = ih | (rh << 16);
+              pl[2 * x] = rl | ((il ^ 0x8000u) << 16);
+              pl[2 * x + 1] = il | (rl << 16);
+            } else if constexpr (MF == 1) {
               // A operand rows (t, part), k = (g, qq): (re, im | -im, re)
               float *o = s_phA + (size_t)x * 4;
               o[0] = (float)rt.x;
@@ -524,7 +594,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
             for (int kl = 0; kl < NA * NA; kl++) {
               const int g0 = a.klf[kl], g1 = a.klf[kl + 1];
               if (g1 == g0) continue;
-              if constexpr (MF) {
+              if constexpr (MF == 1) {
                 constexpr int NTILE = (NOUT2 + 15) / 16;
                 const int ksteps = (2 * (g1 - g0) + 3) / 4;
                 const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
@@ -566,6 +636,61 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                       for (int c = 0; c < TPW; c++)
                         if (wave + NW * c < NTILE)
                           acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[c][u], acc[c], 0, 0, 0);
+                }
+#pragma unroll
+                for (int c = 0; c < TPW; c++) {
+                  const int n = (wave + NW * c) * 16 + (lane & 15);
+                  if (n < NOUT2) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                      const int row = 4 * (lane >> 4) + r;
+                      if ((row & 1) == 0)
+                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)acc[c][r];
+                      else
+                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)acc[c][r];
+                    }
+                  }
+                }
+              } else if constexpr (MF == 2) {
+                // BF16 hi/lo form: Psi = A_hi B_hi + A_hi B_lo + A_lo B_hi on
+                // v_mfma_f32_16x16x16bf16_1k, FP32 accumulation; a k-step is
+                // 8 groups, lane group kg holding groups 2kg and 2kg+1
+                constexpr int NTILE = (NOUT2 + 15) / 16;
+                constexpr int TPW = (NTILE + NW - 1) / NW;
+                const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
+                const unsigned *Bh = reinterpret_cast<const unsigned *>(s_A);       // [g][n] (re | im << 16)
+                const unsigned *Bl = Bh + (size_t)a.ng * NOUT2;
+                const unsigned *Ph = reinterpret_cast<const unsigned *>(s_phA);     // [t][g][part] (qq0 | qq1 << 16)
+                const unsigned *Pl = Ph + (size_t)TQ * a.ng * 2;
+                int ncol[TPW];
+                bool nin[TPW];
+                floatx4 acc[TPW];
+#pragma unroll
+                for (int c = 0; c < TPW; c++) {
+                  const int n = (wave + NW * c) * 16 + (lane & 15);
+                  nin[c] = n < NOUT2;
+                  ncol[c] = nin[c] ? n : NOUT2 - 1;
+                  acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+                }
+                const int nks = (g1 - g0 + 7) / 8;
+                for (int ks = 0; ks < nks; ks++) {
+                  const int g = g0 + 8 * ks + 2 * kg;
+                  const bool v0 = g < g1, v1 = g + 1 < g1;
+                  const int gc0 = v0 ? g : g0, gc1 = v1 ? g + 1 : g0;
+                  const unsigned pa = (t * a.ng + gc0) * 2 + part, pb = (t * a.ng + gc1) * 2 + part;
+                  const bf16x4 ah = bf16_pack(v0 ? Ph[pa] : 0u, v1 ? Ph[pb] : 0u);
+                  const bf16x4 al = bf16_pack(v0 ? Pl[pa] : 0u, v1 ? Pl[pb] : 0u);
+#pragma unroll
+                  for (int c = 0; c < TPW; c++) {
+                    if (wave + NW * c >= NTILE) continue;
+                    const size_t b0 = (size_t)gc0 * NOUT2 + ncol[c], b1 = (size_t)gc1 * NOUT2 + ncol[c];
+                    const bool w0 = v0 && nin[c], w1 = v1 && nin[c];
+                    const bf16x4 bh = bf16_pack(w0 ? Bh[b0] : 0u, w1 ? Bh[b1] : 0u);
+                    const bf16x4 bl = bf16_pack(w0 ? Bl[b0] : 0u, w1 ? Bl[b1] : 0u);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[c], 0, 0, 0);
+                  }
                 }
 #pragma unroll
                 for (int c = 0; c < TPW; c++) {
@@ -899,9 +1024,16 @@ int dense_ngroups(const DenseState *s) { return s ? s->ng : 0; }
 int dense_selftest_mfma(double *err) {
   double *d = nullptr;
   if (hipMalloc(&d, sizeof(double)) != hipSuccess) return derr(FP4_ERR_NOMEM, "selftest: hipMalloc failed");
+  double e32 = 0.0, e16 = 0.0;
   hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, 0, d);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpy(err, d, sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&e32, d, sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_bf16_selftest, dim3(1), dim3(64), 0, 0, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&e16, d, sizeof(double), hipMemcpyDeviceToHost);
+  *err = e32 > e16 ? e32 : e16;
   (void)hipFree(d);
   return e == hipSuccess ? 0 : derr(FP4_ERR_DEVICE, "selftest: kernel failed");
 }
@@ -926,7 +1058,7 @@ size_t static_bytes() {
   return (sizeof(DenseSmem<T, NB, IB, TQ>) + 15) & ~size_t(15);
 }
 
-template <class T, int NB, int IB, int TQ, bool MF>
+template <class T, int NB, int IB, int TQ, int MF>
 size_t lds_bytes(int ng, int ntri, int L) {
   return dyn_layout<T, NB, IB, TQ, MF>(static_bytes<T, NB, IB, TQ>(), ng, ntri, L <= kRootsLds ? L : 0).end;
 }
@@ -934,13 +1066,13 @@ size_t lds_bytes(int ng, int ntri, int L) {
 // One kernel shape: modes per block (IB) and q'' per tile (TQ).
 struct Shape {
   int prec, nb, IB, TQ;
-  bool mfma;
+  int mfma;   // 0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA
   size_t (*bytes)(int, int, int);
   size_t (*stat)();
   void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
 };
 
-template <class T, int NB, int IB, int TQ, bool MF>
+template <class T, int NB, int IB, int TQ, int MF>
 void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
   (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
@@ -953,12 +1085,15 @@ void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
 // In order of preference; the first shape whose LDS fits is used.  The FP32
 // shapes with 8 q'' per tile run the Psi GEMM on MFMA.
 const Shape kShapes[] = {
-    FP4_SHAPE(float, FP4_PREC_FP32, 3, 3, 8, true),          FP4_SHAPE(float, FP4_PREC_FP32, 6, 6, 8, true),
-    FP4_SHAPE(float, FP4_PREC_FP32, 6, 2, 8, true),          FP4_SHAPE(float, FP4_PREC_FP32, 9, 3, 8, true),
-    FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 8, true),         FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 4, false),
-    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 3, 3, 4, false),  FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 3, 4, false),
-    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 1, 4, false),  FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 9, 1, 2, false),
-    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, false),
+    FP4_SHAPE(float, FP4_PREC_FP32, 3, 3, 8, 1),             FP4_SHAPE(float, FP4_PREC_FP32, 6, 6, 8, 1),
+    FP4_SHAPE(float, FP4_PREC_FP32, 6, 2, 8, 1),             FP4_SHAPE(float, FP4_PREC_FP32, 9, 3, 8, 1),
+    FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 8, 1),            FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 4, 0),
+    FP4_SHAPE(float, FP4_PREC_BF16X3, 3, 3, 8, 2),           FP4_SHAPE(float, FP4_PREC_BF16X3, 6, 6, 8, 2),
+    FP4_SHAPE(float, FP4_PREC_BF16X3, 6, 2, 8, 2),           FP4_SHAPE(float, FP4_PREC_BF16X3, 9, 3, 8, 2),
+    FP4_SHAPE(float, FP4_PREC_BF16X3, 12, 1, 8, 2),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 3, 3, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 3, 4, 0),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 1, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 9, 1, 2, 0),
+    FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, 0),
 };
 
 const Shape *pick_shape(int prec, int nb, int ng, int ntri, int L, size_t lds_max) {
@@ -982,7 +1117,7 @@ int ensure(void *&p, size_t &bytes, size_t need) {
 int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
                 std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms) {
   if (!s) return derr(FP4_ERR_STATE, "dense state missing");
-  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE)
+  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3)
     return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
   const int nb = r.m.nb;
   int dev = 0, ldsmax = 0;

@@ -331,7 +331,7 @@ class Context:
         """-> dict(mfma, tq, ib) of the last dense-form call."""
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"mfma": bool(a.value), "tq": b.value, "ib": c.value}
+        return {"mfma": a.value, "tq": b.value, "ib": c.value}   # mfma: 0 VALU, 1 FP32, 2 BF16 hi/lo
 
     def set_batch_capacity(self, processes):
         _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))

@@ -87,6 +87,23 @@ def test_dense_fp32_tolerances(pair):
     assert kerr <= 1e-4
 
 
+def test_dense_bf16x3_tolerances(pair):
+    """BF16 hi/lo MFMA form: per-mode rates within 1e-3 and kappa within 1e-4 of FP64."""
+    case, c, o = pair
+    banner("dense BF16x3 vs FP64")
+    ref = c.RTA_driver_4ph(rt.PREC_FP64)
+    got = c.RTA_driver_4ph(rt.PREC_BF16X3)
+    assert c.dense_shape()["mfma"] == 2
+    err = rel_err(got["rate_scatt_4ph"], ref["rate_scatt_4ph"])
+    print(f"per-mode rate max rel err {err:.3e}")
+    assert err <= 1e-3
+    k64 = kappa_rta(case, ref["rate_scatt_4ph"])
+    k16 = kappa_rta(case, got["rate_scatt_4ph"])
+    kerr = abs(np.trace(k16) - np.trace(k64)) / abs(np.trace(k64))
+    print(f"kappa rel err {kerr:.3e}")
+    assert kerr <= 1e-4
+
+
 def test_dense_golden_diamond_8cube():
     path = os.path.join(GOLDEN, "cfg1.npz")
     if not os.path.exists(path):
@@ -95,7 +112,7 @@ def test_dense_golden_diamond_8cube():
     banner("dense vs golden diamond 8x8x8")
     c = rt.Context(crystal.config_case(1))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
         r, n = c.RTA_driver_4ph_modes(rm, prec)
         assert n == int(g["N"][:, rm - 1].sum())
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
@@ -112,7 +129,7 @@ def test_dense_golden_diamond_16cube_window():
     banner("dense vs golden diamond 16x16x16 shell 2 (q' window)")
     c = rt.Context(crystal.config_case(2))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
         r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
                         ("rate_scatt_minusminus", "rate_mm")):
