@@ -480,6 +480,76 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
       }
       __syncthreads();
 
+      // Registers carrying the next tile's global data: loaded while the
+      // current tile is filtered and contracted (barriers wait for LDS only),
+      // written to LDS when that tile starts.
+      constexpr bool PF = (MF != 0) && (TQ * NB * NB <= NT_DENSE) && (10 * TQ * NB <= NT_DENSE);
+      double2 pf_e3 = make_double2(0.0, 0.0), pf_e4 = make_double2(0.0, 0.0);
+      double pf_s = 0.0;
+      int pf_q = 0;
+      bool have_pf = false;
+      auto fetch = [&](int tb, int tqb) {
+        if (tid < TQ * NB * NB) {   // eigenvector components of phonons 3 and 4
+          const int t = tid / (NB * NB), comp = (tid / NB) % NB, b = tid % NB;
+          if (t < tqb) {
+            pf_e3 = m.eig[((size_t)s.vj[tb + t] * NB + b) * NB + comp];
+            pf_e4 = m.eig[((size_t)s.vs[tb + t] * NB + b) * NB + comp];
+          }
+        }
+        if (tid < 10 * TQ * NB) {   // one of w'', w''', f'', f''', v''(3), v'''(3)
+          const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
+          if (t < tqb) {
+            const int jv = s.vj[tb + t], sv = s.vs[tb + t];
+            if (f == 0)
+              pf_s = E(m, jv, b);
+            else if (f == 1)
+              pf_s = E(m, sv, b);
+            else if (f == 2)
+              pf_s = a.fbose[(size_t)b * m.nptk + jv];
+            else if (f == 3)
+              pf_s = a.fbose[(size_t)b * m.nptk + sv];
+            else if (f < 7)
+              pf_s = Vel(m, jv, b, f - 4);
+            else
+              pf_s = Vel(m, sv, b, f - 7);
+          }
+        }
+        if (tid < 3 * TQ) {         // grid coordinates of q''
+          const int t = tid / 3, c = tid % 3;
+          pf_q = m.ijk[3 * ((t < tqb) ? s.vj[tb + t] : 0) + c];
+        }
+      };
+      auto stash = [&](int tqb) {
+        if (tid < TQ * NB * NB) {
+          const int t = tid / (NB * NB), comp = (tid / NB) % NB, b = tid % NB;
+          C2<T> v3 = {(T)0, (T)0}, v4 = {(T)0, (T)0};
+          if (t < tqb) {
+            v3 = C2<T>{(T)pf_e3.x, (T)((ch != 0) ? -pf_e3.y : pf_e3.y)};
+            v4 = C2<T>{(T)pf_e4.x, (T)(-pf_e4.y)};
+          }
+          s.e3[t][comp][b] = v3;
+          s.e4[t][comp][b] = v4;
+        }
+        if (tid < 10 * TQ * NB) {
+          const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
+          if (t < tqb) {
+            if (f == 0)
+              s.Ejj[t][b] = pf_s;
+            else if (f == 1)
+              s.Ess[t][b] = pf_s;
+            else if (f == 2)
+              s.fb3[t][b] = pf_s;
+            else if (f == 3)
+              s.fb4[t][b] = pf_s;
+            else if (f < 7)
+              s.vjj[t][b][f - 4] = pf_s;
+            else
+              s.vss[t][b][f - 7] = pf_s;
+          }
+        }
+        if (tid < 3 * TQ) s.qjj[tid / 3][tid % 3] = pf_q;
+      };
+
       // ------------- q'' loop
       const int jj_first = (ch == 1) ? 0 : ii0;
       for (int base = jj_first; base < m.nptk; base += NT_DENSE) {
@@ -505,37 +575,47 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           // q'' and q''' of the tile, their grid coordinates, eigenvectors of
           // phonons 3 and 4, frequencies, Bose-Einstein factors and velocities
           if (tid < TQ) {
-            const int jv = (tid < tq) ? s.vj[t0 + tid] : 0;
-            s.jj[tid] = jv;
+            s.jj[tid] = (tid < tq) ? s.vj[t0 + tid] : 0;
             s.ss[tid] = (tid < tq) ? s.vs[t0 + tid] : 0;
             s.hit[tid] = 0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) s.qjj[tid][c] = m.ijk[3 * jv + c];
           }
-          for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
-            const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
-            C2<T> v3 = {(T)0, (T)0}, v4 = {(T)0, (T)0};
-            if (t < tq) {
-              const double2 e3 = m.eig[((size_t)s.vj[t0 + t] * NB + b) * NB + comp];
-              const double2 e4 = m.eig[((size_t)s.vs[t0 + t] * NB + b) * NB + comp];
-              v3 = C2<T>{(T)e3.x, (T)((ch != 0) ? -e3.y : e3.y)};
-              v4 = C2<T>{(T)e4.x, (T)(-e4.y)};
+          if constexpr (PF) {
+            if (!have_pf) fetch(t0, tq);
+            stash(tq);
+            const int tn = t0 + TQ;
+            have_pf = tn < tot;
+            if (have_pf) fetch(tn, min(TQ, tot - tn));
+          } else {
+            if (tid < TQ) {
+              const int jv = (tid < tq) ? s.vj[t0 + tid] : 0;
+#pragma unroll
+              for (int c = 0; c < 3; c++) s.qjj[tid][c] = m.ijk[3 * jv + c];
             }
-            s.e3[t][comp][b] = v3;
-            s.e4[t][comp][b] = v4;
-          }
-          for (int x = tid; x < TQ * NB; x += NT_DENSE) {
-            const int t = x / NB, b = x % NB;
-            if (t < tq) {
-              const int jv = s.vj[t0 + t], sv = s.vs[t0 + t];
-              s.Ejj[t][b] = E(m, jv, b);
-              s.Ess[t][b] = E(m, sv, b);
-              s.fb3[t][b] = a.fbose[(size_t)b * m.nptk + jv];
-              s.fb4[t][b] = a.fbose[(size_t)b * m.nptk + sv];
+            for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
+              const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
+              C2<T> v3 = {(T)0, (T)0}, v4 = {(T)0, (T)0};
+              if (t < tq) {
+                const double2 e3 = m.eig[((size_t)s.vj[t0 + t] * NB + b) * NB + comp];
+                const double2 e4 = m.eig[((size_t)s.vs[t0 + t] * NB + b) * NB + comp];
+                v3 = C2<T>{(T)e3.x, (T)((ch != 0) ? -e3.y : e3.y)};
+                v4 = C2<T>{(T)e4.x, (T)(-e4.y)};
+              }
+              s.e3[t][comp][b] = v3;
+              s.e4[t][comp][b] = v4;
+            }
+            for (int x = tid; x < TQ * NB; x += NT_DENSE) {
+              const int t = x / NB, b = x % NB;
+              if (t < tq) {
+                const int jv = s.vj[t0 + t], sv = s.vs[t0 + t];
+                s.Ejj[t][b] = E(m, jv, b);
+                s.Ess[t][b] = E(m, sv, b);
+                s.fb3[t][b] = a.fbose[(size_t)b * m.nptk + jv];
+                s.fb4[t][b] = a.fbose[(size_t)b * m.nptk + sv];
 #pragma unroll
-              for (int d = 0; d < 3; d++) {
-                s.vjj[t][b][d] = Vel(m, jv, b, d);
-                s.vss[t][b][d] = Vel(m, sv, b, d);
+                for (int d = 0; d < 3; d++) {
+                  s.vjj[t][b][d] = Vel(m, jv, b, d);
+                  s.vss[t][b][d] = Vel(m, sv, b, d);
+                }
               }
             }
           }
