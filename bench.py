@@ -87,7 +87,7 @@ def measured_traffic(kernel):
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
     None when no summary covers this kernel."""
     import re
-    path = os.path.join(ROOT, "profiles", "dense_fp32_diamond16_shell2_traffic.md")
+    path = os.path.join(ROOT, "profiles", "dense_bf16x3_diamond16_shell2_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):

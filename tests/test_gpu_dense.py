@@ -135,3 +135,23 @@ def test_dense_golden_diamond_16cube_window():
                         ("rate_scatt_minusminus", "rate_mm")):
             assert rel_err(r[key].ravel(order="F")[rm - 1], g[wk]) <= tol, (prec, key)
     c.close()
+
+
+def test_dense_golden_wurtzite_12x12x8_window():
+    """Nb = 12 dense shapes (FP64 VALU, FP32 MFMA, BF16 hi/lo MFMA) against the
+    oracle's rates of the 12x12x8 wurtzite, 2nd shell, on a q' window."""
+    path = os.path.join(GOLDEN, "cfg3.npz")
+    if not os.path.exists(path):
+        pytest.skip("cfg3.npz not generated")
+    g = np.load(path)
+    banner("dense vs golden wurtzite 12x12x8 shell 2 (q' window)")
+    c = rt.Context(crystal.config_case(3))
+    rm = g["rate_modes"]
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
+        r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
+        for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+                        ("rate_scatt_minusminus", "rate_mm")):
+            assert rel_err(r[key].ravel(order="F")[rm - 1], g[wk]) <= tol, (prec, key)
+        if prec != rt.PREC_FP64_DENSE:
+            assert c.dense_shape()["mfma"] == (1 if prec == rt.PREC_FP32 else 2)
+    c.close()

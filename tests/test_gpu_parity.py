@@ -249,3 +249,22 @@ def test_golden_diamond_16cube_shell2():
                     ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusplus", "wp4_pp")):
         assert_close(r[key].ravel(order="F")[rm - 1], g[wk], RTOL, key)
     c.close()
+
+
+def test_golden_wurtzite_12x12x8_shell2():
+    g = _golden("cfg3")
+    banner("golden wurtzite 12x12x8 shell 2 (workload 3, Nb = 12)")
+    case = crystal.config_case(3)
+    c = rt.Context(case)
+    N = c.NP_driver_4ph()
+    modes = g["modes"]
+    for ch in range(3):
+        assert np.array_equal(N[2 * ch][modes - 1], g["N"][ch]), ch
+        assert_close(N[2 * ch + 1].ravel(order="F")[modes - 1], g["P"][ch], RTOL, f"P {ch}")
+    _check_golden_lists(c, g, modes, 1)
+    rm = g["rate_modes"]
+    r, _ = c.RTA_driver_4ph_modes(rm, rt.PREC_FP64, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
+    for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+                    ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusminus", "wp4_pm")):
+        assert_close(r[key].ravel(order="F")[rm - 1], g[wk], RTOL, key)
+    c.close()
