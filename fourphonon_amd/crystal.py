@@ -16,7 +16,10 @@ indices stored in the data):
   duplicates dropped (``Src/config.f90:272-376``);
 * fourth-order clusters ``Phi(3,3,3,3,Ntri)`` in eV/A^4, ``R_j/R_k/R_l(3,Ntri)``
   in nm and exactly on lattice vectors, ``Index_i..l(Ntri)`` 1-based atoms
-  (``Src/input.f90:141-182``).
+  (``Src/input.f90:141-182``);
+* third-order clusters ``Phi(3,3,3,Ntri)`` in eV/A^3, ``R_j/R_k(3,Ntri)`` in
+  nm, ``Index_i..k(Ntri)`` (``Src/input.f90:102-140``), generated the same
+  way as the fourth-order ones.
 
 The harmonic model is a nearest-neighbour central + transverse spring model
 (every bond contributes ``-(k_c n n^T + k_t (1 - n n^T))``; the on-site block
@@ -87,6 +90,18 @@ class IFC4:
 
 
 @dataclass
+class IFC3:
+    Ntri: int
+    Phi: np.ndarray              # (3,3,3,Ntri) Fortran order, eV/A^3
+    R_j: np.ndarray              # (3,Ntri) Fortran order, nm
+    R_k: np.ndarray
+    Index_i: np.ndarray          # (Ntri,) int32, 1-based
+    Index_j: np.ndarray
+    Index_k: np.ndarray
+    cells: np.ndarray            # (Ntri,2,3) int lattice coordinates of j,k
+
+
+@dataclass
 class Modes:
     ngrid: np.ndarray            # (3,) int32
     energy: np.ndarray           # (nptk,Nb) F-order
@@ -124,6 +139,7 @@ class Case:
     omega_max: float = 1.0e100
     onlyharmonic: bool = False
     meta: dict = field(default_factory=dict)
+    ifc3: IFC3 | None = None
 
     @property
     def nptk(self) -> int:
@@ -253,6 +269,25 @@ def enumerate_quartets(st: Structure, rc: float):
     return np.array(out, dtype=np.int64)
 
 
+def enumerate_triplets(st: Structure, rc: float):
+    """Ordered triplets (I@0, j, k), atoms pairwise within rc, repeats allowed.
+
+    Returns sites (n,3,4) int: [atom, c1, c2, c3] per position, first at cell 0.
+    """
+    out = []
+    for I, (nsites, nxyz) in enumerate(_neighbours(st, rc)):
+        n = len(nsites)
+        dmat = np.linalg.norm(nxyz[:, None, :] - nxyz[None, :, :], axis=2)
+        ok = dmat <= rc + 1e-9
+        home = np.nonzero((nsites[:, 0] == I) & np.all(nsites[:, 1:] == 0, axis=1))[0][0]
+        for a in range(n):
+            if not ok[home, a]:
+                continue
+            for b in np.nonzero(ok[home] & ok[a])[0]:
+                out.append((nsites[home], nsites[a], nsites[b]))
+    return np.array(out, dtype=np.int64)
+
+
 def _canon(sites):
     """Translate each quartet so that its first site sits in cell 0."""
     s = sites.copy()
@@ -312,6 +347,60 @@ def make_ifc4(st: Structure, shell: int, seed: int, sym_iters: int = 400) -> IFC
     Phi = np.asfortranarray(np.transpose(phi, (1, 2, 3, 4, 0)))   # (3,3,3,3,Ntri)
     idx = [np.ascontiguousarray(sites[:, m, 0] + 1, dtype=np.int32) for m in range(4)]
     return IFC4(n, Phi, R[0], R[1], R[2], idx[0], idx[1], idx[2], idx[3], cells)
+
+
+def make_ifc3(st: Structure, shell: int, seed: int, sym_iters: int = 400) -> IFC3:
+    """Third-order constants in the layout of FORCE_CONSTANTS_3RD as the
+    reference holds it (Src/input.f90:102-140): Phi(3,3,3,Ntri) in eV/A^3,
+    R_j, R_k in nm on lattice vectors, 1-based atoms.  Seeded N(0,1) values on
+    every ordered triplet whose atoms are pairwise within the cutoff, projected
+    alternately onto the index-permutation symmetric subspace and the
+    acoustic-sum-rule subspace (sum over the third site's atom and cell)."""
+    rc = cluster_cutoff(st, shell)
+    sites = _canon(enumerate_triplets(st, rc))
+    order = np.lexsort(sites.reshape(len(sites), -1).T[::-1])
+    sites = sites[order]
+    n = len(sites)
+    index = {k: i for i, k in enumerate(_keys(sites))}
+    rng = np.random.Generator(np.random.PCG64(seed))
+    phi = rng.standard_normal((n, 3, 3, 3))
+
+    perms = list(itertools.permutations(range(3)))
+    pmaps = []
+    for p in perms:
+        ps = _canon(sites[:, list(p), :])
+        pmaps.append(np.array([index[k] for k in _keys(ps)], dtype=np.int64))
+    inv_axes = [tuple(int(np.argsort(p)[a]) + 1 for a in range(3)) for p in perms]
+
+    g_keys = _keys(sites[:, :2, :])
+    gid = {}
+    groups = np.array([gid.setdefault(k, len(gid)) for k in g_keys], dtype=np.int64)
+    counts = np.bincount(groups)
+
+    def symmetrise(x):
+        acc = np.zeros_like(x)
+        for pm, ax in zip(pmaps, inv_axes):
+            acc += np.transpose(x[pm], (0,) + ax)
+        return acc / len(perms)
+
+    def asr(x):
+        s = np.zeros((len(counts), 3, 3, 3))
+        np.add.at(s, groups, x)
+        return x - (s / counts[:, None, None, None])[groups]
+
+    for _ in range(sym_iters):
+        phi = symmetrise(asr(phi))
+        s = np.zeros((len(counts), 3, 3, 3))
+        np.add.at(s, groups, phi)
+        if np.abs(s).max() < 1e-13 * max(1.0, np.abs(phi).max()):
+            break
+
+    lat = st.lattvec
+    cells = sites[:, 1:, 1:]                                       # (n,2,3) cells of j,k
+    R = [np.asfortranarray((cells[:, m, :] @ lat.T).T) for m in range(2)]
+    Phi = np.asfortranarray(np.transpose(phi, (1, 2, 3, 0)))      # (3,3,3,Ntri)
+    idx = [np.ascontiguousarray(sites[:, m, 0] + 1, dtype=np.int32) for m in range(3)]
+    return IFC3(n, Phi, R[0], R[1], idx[0], idx[1], idx[2], cells)
 
 
 # ---------------------------------------------------------------------------
@@ -546,5 +635,55 @@ def config_case(k: int, ngrid=None, shell=None, T: float = 300.0, scalebroad: fl
     fc2 = make_fc2(st, seed=s + 2)
     modes = make_modes(st, fc2, ngrid, symmetric=symmetric)
     ifc4 = make_ifc4(st, shell, seed=s + 4)
-    return Case(st, modes, ifc4, T=T, scalebroad=scalebroad,
+    ifc3 = make_ifc3(st, shell, seed=s + 3)
+    return Case(st, modes, ifc4, T=T, scalebroad=scalebroad, ifc3=ifc3,
                 meta={"workload": k, "ngrid": list(ngrid), "shell": shell, "structure": st.name})
+
+
+# the batch of 24 small crystals: kinds and cell sizes cycle so that every
+# (kind, natoms) pair appears twice; the mesh range shrinks with the cell so
+# that no crystal dominates the batch (candidates grow as nptk^2 Nb^4)
+BATCH_SIZE = 24
+BATCH_KINDS = ("fcc", "zincblende", "rocksalt")
+BATCH_NATOMS = (2, 4, 6, 8)
+BATCH_MESH = {2: (10, 14), 4: (8, 12), 6: (6, 10), 8: (6, 8)}
+
+
+def batch_crystal(c: int) -> dict:
+    """Draw crystal ``c`` (0..23) of the batch: kind, natoms, masses, a and mesh.
+
+    Structure draw from PCG64(SEED_BASE + 1000 + c): masses U(10, 120) amu for
+    the two sublattices, a U(3.5, 6.0) A, mesh edge uniform in the natoms range
+    of ``BATCH_MESH`` (6 to 14 overall).
+    """
+    if not 0 <= c < BATCH_SIZE:
+        raise ValueError(f"batch crystal index {c} outside 0..{BATCH_SIZE - 1}")
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 1000 + c))
+    natoms = BATCH_NATOMS[c % 4]
+    kind = BATCH_KINDS[(c // 4) % 3]
+    masses = rng.uniform(10.0, 120.0, size=2)
+    a_ang = float(rng.uniform(3.5, 6.0))
+    lo, hi = BATCH_MESH[natoms]
+    n = int(rng.integers(lo, hi + 1))
+    return {"kind": kind, "natoms": natoms, "masses": masses, "a_ang": a_ang, "mesh": n}
+
+
+def batch_case(c: int, ngrid=None, shell: int = 1, T: float = 300.0, scalebroad: float = 1.0,
+               symmetric: bool = True) -> Case:
+    """Crystal ``c`` of the 24-crystal batch as a hot-path case.
+
+    FC2 jitter from seed SEED_BASE + 1000 + c + 100 and FC4 from + 300 (+ 200
+    is the third-order seed); clusters to the 1st shell unless ``shell`` says
+    otherwise; ``ngrid`` overrides the drawn mesh (tests run smaller meshes).
+    """
+    d = batch_crystal(c)
+    st = cubic_family(d["kind"], d["natoms"], d["a_ang"], d["masses"])
+    ngrid = tuple((d["mesh"],) * 3 if ngrid is None else ngrid)
+    s = SEED_BASE + 1000 + c
+    fc2 = make_fc2(st, seed=s + 100)
+    modes = make_modes(st, fc2, ngrid, symmetric=symmetric)
+    ifc4 = make_ifc4(st, shell, seed=s + 300)
+    ifc3 = make_ifc3(st, shell, seed=s + 200)
+    return Case(st, modes, ifc4, T=T, scalebroad=scalebroad, ifc3=ifc3,
+                meta={"workload": 4, "crystal": c, "ngrid": list(ngrid), "shell": shell,
+                      "structure": st.name, "a_ang": d["a_ang"], "masses": list(map(float, d["masses"]))})

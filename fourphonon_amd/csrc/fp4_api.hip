@@ -72,7 +72,7 @@ struct DevBuf {
   }
 };
 
-enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_ITER = 4, KS_N = 5 };
+enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_ITER = 4, KS_SCAN3 = 5, KS_RATE3 = 6, KS_N = 7 };
 
 }  // namespace
 
@@ -87,7 +87,7 @@ struct fp4_ctx {
   int onlyharmonic = 0;
   std::vector<double> masses;
   std::vector<int> types;
-  bool have_modes = false, have_ifc = false;
+  bool have_modes = false, have_ifc = false, have_ifc3 = false;
   // mode data
   DevBuf energy, vel, vbound, eig, ijk, list;
   // clusters
@@ -95,6 +95,9 @@ struct fp4_ctx {
   DevBuf clusters, phi, roots;
   std::vector<ClusterRec> h_clusters;
   std::vector<double> h_phi;
+  // third-order clusters
+  int ntri3 = 0;
+  DevBuf clusters3, phi3;
   // scratch
   DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g;
   DenseState *dense = nullptr;
@@ -102,8 +105,8 @@ struct fp4_ctx {
   long long list_cap = 1LL << 26;   // processes per batch
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double ks_ms[KS_N] = {0, 0, 0, 0, 0};
-  long long ks_n[KS_N] = {0, 0, 0, 0, 0};
+  double ks_ms[KS_N] = {};
+  long long ks_n[KS_N] = {};
   DenseTuning dense_tuning;
 };
 
@@ -134,6 +137,16 @@ ModeView mode_view(const fp4_ctx *c) {
     m.inv_n[nu] = (n > 0 && (n & (n - 1)) == 0) ? 1.0 / (double)n : 0.0;
   }
   return m;
+}
+
+Ifc3View ifc3_view(const fp4_ctx *c) {
+  Ifc3View f;
+  f.cl = c->clusters3.as<Cluster3Rec>();
+  f.phi = c->phi3.as<double>();
+  f.roots = c->roots.as<double2>();
+  f.ntri = c->ntri3;
+  f.L = c->L;
+  return f;
 }
 
 IfcView ifc_view(const fp4_ctx *c) {
@@ -167,12 +180,18 @@ int timed(fp4_ctx *c, int kind, F &&launch) {
 // ---------------------------------------------------------------- pipeline
 struct Plan {
   std::vector<int> modes;          // mm (1-based) of the subset
+  // items per (channel, mode) segment: the q' window (four phonons: ii_lo0,
+  // nii) or the Nb second-phonon branches (three phonons: nii = Nb, window
+  // win_lo0, win_n)
   int ii_lo0 = 0, nii = 0;
+  int win_lo0 = 0, win_n = 0;
   long items = 0;
   std::vector<long long> off;      // exclusive offsets, size items+1
   std::vector<long long> segN;     // [3*nsub] processes per (ch, mode)
   std::vector<double> segP;        // [3*nsub] phase space partial sums
 };
+
+int finish_counts(fp4_ctx *c, Plan &pl, int nseg, bool want_offsets);
 
 int count_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
   const int nsub = (int)pl.modes.size();
@@ -199,6 +218,13 @@ int count_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
          return launch_prefilter(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
        })))
     return rc;
+  return finish_counts(c, pl, nseg, want_offsets);
+}
+
+// Per-segment totals of the count pass and, when asked, the exclusive item
+// offsets (host and device copies).
+int finish_counts(fp4_ctx *c, Plan &pl, int nseg, bool want_offsets) {
+  int rc;
   launch_reduce_counts(c->cnt.as<int>(), c->psum.as<double>(), nseg, pl.nii, c->segN.as<long long>(),
                        c->segP.as<double>(), c->stream);
   HIPCHECK(hipGetLastError());
@@ -295,9 +321,11 @@ int emit_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide) {
   return timed(c, KS_EMIT, [&] { return launch_prefilter(a, wide ? 2 : 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
 }
 
-// Rate kernel over a batch whose lists are in c->lists (int32, or int64 when wide).
-int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store_gamma, std::vector<double> &acc_g,
-               std::vector<double> &acc_w) {
+// Block plan upload, one rate launch, per-segment reduction and accumulation
+// into acc_g / acc_w (in batch order) for the lists of one batch.
+template <typename F>
+int rate_common(fp4_ctx *c, const Plan &pl, const Batch &b, bool store_gamma, std::vector<double> &acc_g,
+                std::vector<double> &acc_w, F &&launch) {
   std::vector<BlockDesc> blocks;
   std::vector<int> seg_first, piece_seg;
   plan_blocks(pl, b, blocks, seg_first, piece_seg);
@@ -313,23 +341,7 @@ int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store
   if (store_gamma && (rc = c->proc_g.ensure(sizeof(double) * (size_t)std::max(b.p1 - b.p0, 1LL)))) return rc;
   HIPCHECK(hipMemcpyAsync(c->blk.p, blocks.data(), sizeof(BlockDesc) * nblk, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(c->seg_first.p, seg_first.data(), sizeof(int) * (npc + 1), hipMemcpyHostToDevice, c->stream));
-  RateArgs a{};
-  a.m = mode_view(c);
-  a.f = ifc_view(c);
-  for (int t = 0; t < 3; t++) {
-    a.lst[t] = wide ? nullptr : c->lists[t].as<int>();
-    a.lst64[t] = wide ? c->lists[t].as<long long>() : nullptr;
-  }
-  a.blocks = c->blk.as<BlockDesc>();
-  a.nblocks = nblk;
-  a.T = c->T;
-  a.onlyharmonic = c->onlyharmonic;
-  a.store_gamma = store_gamma;
-  a.proc_gamma = c->proc_g.as<double>();
-  a.blk_gamma = c->blk_g.as<double>();
-  a.blk_wp4 = c->blk_w.as<double>();
-  if ((rc = timed(c, KS_RATE64, [&] { return launch_rate64(a, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; })))
-    return rc;
+  if ((rc = launch(nblk))) return rc;
   launch_seg_reduce(c->blk_g.as<double>(), c->blk_w.as<double>(), c->seg_first.as<int>(), npc,
                     c->seg_g.as<double>(), c->seg_w.as<double>(), c->stream);
   HIPCHECK(hipGetLastError());
@@ -340,6 +352,157 @@ int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store
   for (int t = 0; t < npc; t++) {
     acc_g[piece_seg[t]] = acc_g[piece_seg[t]] + g[t];
     acc_w[piece_seg[t]] = acc_w[piece_seg[t]] + w[t];
+  }
+  return 0;
+}
+
+// Four-phonon rate kernel over a batch whose lists are in c->lists (int32,
+// or int64 when wide).
+int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store_gamma, std::vector<double> &acc_g,
+               std::vector<double> &acc_w) {
+  return rate_common(c, pl, b, store_gamma, acc_g, acc_w, [&](int nblk) {
+    RateArgs a{};
+    a.m = mode_view(c);
+    a.f = ifc_view(c);
+    for (int t = 0; t < 3; t++) {
+      a.lst[t] = wide ? nullptr : c->lists[t].as<int>();
+      a.lst64[t] = wide ? c->lists[t].as<long long>() : nullptr;
+    }
+    a.blocks = c->blk.as<BlockDesc>();
+    a.nblocks = nblk;
+    a.T = c->T;
+    a.onlyharmonic = c->onlyharmonic;
+    a.store_gamma = store_gamma;
+    a.proc_gamma = c->proc_g.as<double>();
+    a.blk_gamma = c->blk_g.as<double>();
+    a.blk_wp4 = c->blk_w.as<double>();
+    return timed(c, KS_RATE64, [&] { return launch_rate64(a, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+  });
+}
+
+// ------------------------------------------------------- three phonons
+// Items are (channel, mode-in-subset, j); the q' window is win_lo0/win_n.
+int count3_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
+  const int nsub = (int)pl.modes.size();
+  pl.nii = c->nb;
+  pl.items = 2L * nsub * c->nb;
+  const int nseg = 2 * nsub;
+  int rc;
+  if ((rc = c->modes_sub.ensure(sizeof(int) * std::max(nsub, 1)))) return rc;
+  if ((rc = c->cnt.ensure(sizeof(int) * std::max(pl.items, 1L)))) return rc;
+  if ((rc = c->psum.ensure(sizeof(double) * std::max(pl.items, 1L)))) return rc;
+  if ((rc = c->segN.ensure(sizeof(long long) * std::max(nseg, 1)))) return rc;
+  if ((rc = c->segP.ensure(sizeof(double) * std::max(nseg, 1)))) return rc;
+  HIPCHECK(hipMemcpyAsync(c->modes_sub.p, pl.modes.data(), sizeof(int) * nsub, hipMemcpyHostToDevice, c->stream));
+  Scan3Args a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = nsub;
+  a.nii = pl.win_n;
+  a.ii_lo0 = pl.win_lo0;
+  a.item_lo = 0;
+  a.item_hi = pl.items;
+  a.cnt = c->cnt.as<int>();
+  a.psum = c->psum.as<double>();
+  if ((rc = timed(c, KS_SCAN3, [&] {
+         return launch_scan3(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+       })))
+    return rc;
+  return finish_counts(c, pl, nseg, want_offsets);
+}
+
+int emit3_batch(fp4_ctx *c, const Plan &pl, const Batch &b) {
+  int rc;
+  const long long n = b.p1 - b.p0;
+  for (int t = 0; t < 2; t++)
+    if ((rc = c->lists[t].ensure(4 * (size_t)std::max(n, 1LL)))) return rc;
+  Scan3Args a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = (int)pl.modes.size();
+  a.nii = pl.win_n;
+  a.ii_lo0 = pl.win_lo0;
+  a.item_lo = b.item_lo;
+  a.item_hi = b.item_hi;
+  a.offsets = c->offsets.as<long long>();
+  a.out_base = b.p0;
+  for (int t = 0; t < 2; t++) a.out[t] = c->lists[t].as<int>();
+  return timed(c, KS_SCAN3, [&] { return launch_scan3(a, 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+}
+
+// Three-phonon rate kernel over a batch whose int32 lists are in c->lists[0..1].
+int rate3_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool store_gamma, std::vector<double> &acc_g,
+                std::vector<double> &acc_w) {
+  return rate_common(c, pl, b, store_gamma, acc_g, acc_w, [&](int nblk) {
+    Rate3Args a{};
+    a.m = mode_view(c);
+    a.f = ifc3_view(c);
+    for (int t = 0; t < 2; t++) a.lst[t] = c->lists[t].as<int>();
+    a.blocks = c->blk.as<BlockDesc>();
+    a.nblocks = nblk;
+    a.T = c->T;
+    a.onlyharmonic = c->onlyharmonic;
+    a.store_gamma = store_gamma;
+    a.proc_gamma = c->proc_g.as<double>();
+    a.blk_gamma = c->blk_g.as<double>();
+    a.blk_wp = c->blk_w.as<double>();
+    return timed(c, KS_RATE3, [&] { return launch_rate3(a, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+  });
+}
+
+// Per-mode outputs of the three-phonon drivers: "+" Gamma * 5.60626442 * 1e8
+// / nptk (processes.f90:1389), "-" the same times 1/2 (:1742); WP3 / nptk and
+// WP3 * 0.5 / nptk (:1387, :1467).  scale_gamma = false when the segment sums
+// are already sums of per-process Gamma in THz (Ind_driver, :623, :637).
+void write3_rates(const fp4_ctx *c, const Plan &pl, const std::vector<double> &g, const std::vector<double> &w,
+                  double *rate, double *r_plus, double *r_minus, double *w_plus, double *w_minus, bool scale_gamma) {
+  const int nsub = (int)pl.modes.size();
+  for (int mi = 0; mi < nsub; mi++) {
+    const int m = pl.modes[mi] - 1;
+    const double gp = g[mi], gm = g[nsub + mi];
+    r_plus[m] = scale_gamma ? gp * (double)kGamma3Units * 1.e8 / c->nptk : gp;
+    r_minus[m] = (scale_gamma ? gm * (double)kGamma3Units * 1.e8 / c->nptk : gm) * 5.e-1;
+    w_plus[m] = w[mi] / c->nptk;
+    w_minus[m] = w[nsub + mi] * 5.e-1 / c->nptk;
+    rate[m] = r_plus[m] + r_minus[m];
+  }
+}
+
+int check_ready3(const fp4_ctx *c, bool need_ifc3) {
+  if (!c) return fail(FP4_ERR_ARG, "null context");
+  if (!c->have_modes) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes has not been called");
+  if (need_ifc3 && !c->have_ifc3) return fail(FP4_ERR_STATE, "fp4_ctx_set_ifc3 has not been called");
+  HIPCHECK(hipSetDevice(c->device));
+  return 0;
+}
+
+// e^{2 pi i k / L} for k < L = lcm(N1, N2, N3), shared by both cluster sets.
+int ensure_roots(fp4_ctx *c) {
+  const int L = std::lcm(std::lcm(c->ngrid[0], c->ngrid[1]), c->ngrid[2]);
+  if (c->L == L && c->roots.p) return 0;
+  std::vector<double> roots(2 * (size_t)L);
+  for (int k = 0; k < L; k++) {
+    const double x = 2.0 * M_PI * (double)k / (double)L;
+    roots[2 * k] = std::cos(x);
+    roots[2 * k + 1] = std::sin(x);
+  }
+  int rc;
+  if ((rc = c->roots.ensure(sizeof(double) * 2 * L))) return rc;
+  HIPCHECK(hipMemcpy(c->roots.p, roots.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
+  c->L = L;
+  return 0;
+}
+
+// Lattice coordinates of R (nm) times L / N_nu: the phase multipliers of a cluster site.
+int phase_multipliers(const fp4_ctx *c, const double *R, int L, int *out, int cluster) {
+  const int ng[3] = {c->ngrid[0], c->ngrid[1], c->ngrid[2]};
+  for (int nu = 0; nu < 3; nu++) {
+    // b_nu . R = 2 pi m_nu for a lattice vector R
+    const double *bv = c->rlattvec + 3 * nu;
+    const double x = (bv[0] * R[0] + bv[1] * R[1] + bv[2] * R[2]) / (2.0 * M_PI);
+    const double xr = std::nearbyint(x);
+    if (std::fabs(x - xr) > 1e-6) return fail(FP4_ERR_ARG, "R of cluster %d is not a lattice vector", cluster);
+    out[nu] = (int)xr * (L / ng[nu]);
   }
   return 0;
 }
@@ -424,6 +587,7 @@ int fp4_ctx_destroy(fp4_ctx *c) {
   if (!c) return FP4_OK;
   (void)hipSetDevice(c->device);
   DevBuf *bufs[] = {&c->energy, &c->vel, &c->vbound, &c->eig, &c->ijk, &c->list, &c->clusters, &c->phi, &c->roots,
+                    &c->clusters3, &c->phi3,
                     &c->modes_sub, &c->cnt, &c->psum, &c->offsets, &c->segN, &c->segP, &c->lists[0], &c->lists[1],
                     &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g};
   for (DevBuf *b : bufs) b->release();
@@ -501,9 +665,9 @@ int fp4_ctx_set_ifc4(fp4_ctx *c, int32_t ntri, const double *Phi, const double *
   if (!c || !Phi || !R_j || !R_k || !R_l || !Ii || !Ij || !Ik || !Il) return fail(FP4_ERR_ARG, "null argument");
   if (ntri < 1) return fail(FP4_ERR_ARG, "Ntri must be positive");
   HIPCHECK(hipSetDevice(c->device));
-  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
-  const int L = std::lcm(std::lcm(n1, n2), n3);
-  const int ng[3] = {n1, n2, n3};
+  int rc;
+  if ((rc = ensure_roots(c))) return rc;
+  const int L = c->L;
   c->h_clusters.assign(ntri, ClusterRec{});
   for (int t = 0; t < ntri; t++) {
     ClusterRec &r = c->h_clusters[t];
@@ -514,37 +678,19 @@ int fp4_ctx_set_ifc4(fp4_ctx *c, int32_t ntri, const double *Phi, const double *
     }
     const double *Rs[3] = {R_j + 3 * (size_t)t, R_k + 3 * (size_t)t, R_l + 3 * (size_t)t};
     int *outs[3] = {r.pj, r.pk, r.pl};
-    for (int w = 0; w < 3; w++) {
-      for (int nu = 0; nu < 3; nu++) {
-        // b_nu . R = 2 pi m_nu for a lattice vector R
-        const double *bv = c->rlattvec + 3 * nu;
-        const double x = (bv[0] * Rs[w][0] + bv[1] * Rs[w][1] + bv[2] * Rs[w][2]) / (2.0 * M_PI);
-        const double xr = std::nearbyint(x);
-        if (std::fabs(x - xr) > 1e-6) return fail(FP4_ERR_ARG, "R of cluster %d is not a lattice vector", t + 1);
-        outs[w][nu] = (int)xr * (L / ng[nu]);
-      }
-    }
+    for (int w = 0; w < 3; w++)
+      if ((rc = phase_multipliers(c, Rs[w], L, outs[w], t + 1))) return rc;
     double mp = c->masses[c->types[r.atom[0]] - 1] * c->masses[c->types[r.atom[1]] - 1];
     mp = mp * c->masses[c->types[r.atom[2]] - 1];
     mp = mp * c->masses[c->types[r.atom[3]] - 1];
     r.minv = 1.0 / std::sqrt(mp);
   }
-  std::vector<double> roots(2 * L);
-  for (int k = 0; k < L; k++) {
-    const double x = 2.0 * M_PI * (double)k / (double)L;
-    roots[2 * k] = std::cos(x);
-    roots[2 * k + 1] = std::sin(x);
-  }
-  int rc;
-  if ((rc = c->clusters.ensure(sizeof(ClusterRec) * ntri)) || (rc = c->phi.ensure(sizeof(double) * 81 * ntri)) ||
-      (rc = c->roots.ensure(sizeof(double) * 2 * L)))
+  if ((rc = c->clusters.ensure(sizeof(ClusterRec) * ntri)) || (rc = c->phi.ensure(sizeof(double) * 81 * ntri)))
     return rc;
   c->h_phi.assign(Phi, Phi + 81 * (size_t)ntri);
   HIPCHECK(hipMemcpy(c->clusters.p, c->h_clusters.data(), sizeof(ClusterRec) * ntri, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->phi.p, Phi, sizeof(double) * 81 * ntri, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->roots.p, roots.data(), sizeof(double) * 2 * L, hipMemcpyHostToDevice));
   c->ntri = ntri;
-  c->L = L;
   dense_destroy(c->dense);
   int drc = 0;
   c->dense = dense_create(c->h_clusters, c->h_phi, c->natoms, c->nb, L, &drc);
@@ -824,13 +970,9 @@ int fp4_vp_4ph(fp4_ctx *c, int32_t channel, int64_t n, const int32_t *idx, doubl
   return FP4_OK;
 }
 
-int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t cap, int64_t *N_pp,
-                           int64_t *N_pm, int64_t *N_mm, int64_t *a2, int64_t *a3, int64_t *a4, int64_t *b2,
-                           int64_t *b3, int64_t *b4, int64_t *c2, int64_t *c3, int64_t *c4) {
-  int rc;
-  if ((rc = check_ready(c, false))) return rc;
-  if (nmodes < 1 || !modes || !N_pp || !N_pm || !N_mm) return fail(FP4_ERR_ARG, "bad arguments");
-  Plan pl;
+// A strictly increasing subset of modes over the whole q' range.
+static int subset_plan(const fp4_ctx *c, int32_t nmodes, const int32_t *modes, Plan &pl) {
+  if (nmodes < 1 || !modes) return fail(FP4_ERR_ARG, "bad mode subset");
   pl.modes.assign(modes, modes + nmodes);
   for (size_t t = 0; t < pl.modes.size(); t++) {
     if (pl.modes[t] < 1 || pl.modes[t] > c->nb * c->nlist) return fail(FP4_ERR_ARG, "mode %d out of range", pl.modes[t]);
@@ -838,6 +980,35 @@ int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int
   }
   pl.ii_lo0 = 0;
   pl.nii = c->nptk;
+  return 0;
+}
+
+int fp4_np_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp, int64_t *N_pm,
+                     double *P_pm, int64_t *N_mm, double *P_mm) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!N_pp || !P_pp || !N_pm || !P_pm || !N_mm || !P_mm) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  if ((rc = count_pass(c, pl, false))) return rc;
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  double *P[3] = {P_pp, P_pm, P_mm};
+  for (int ch = 0; ch < 3; ch++)
+    for (int m = 0; m < nmodes; m++) {
+      N[ch][m] = pl.segN[ch * nmodes + m];
+      P[ch][m] = pl.segP[ch * nmodes + m];
+    }
+  return FP4_OK;
+}
+
+int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t cap, int64_t *N_pp,
+                           int64_t *N_pm, int64_t *N_mm, int64_t *a2, int64_t *a3, int64_t *a4, int64_t *b2,
+                           int64_t *b3, int64_t *b4, int64_t *c2, int64_t *c3, int64_t *c4) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (nmodes < 1 || !modes || !N_pp || !N_pm || !N_mm) return fail(FP4_ERR_ARG, "bad arguments");
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
   if ((rc = count_pass(c, pl, true))) return rc;
   int64_t *N[3] = {N_pp, N_pm, N_mm};
   int64_t *outs[3][3] = {{a2, a3, a4}, {b2, b3, b4}, {c2, c3, c4}};
@@ -867,6 +1038,280 @@ int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int
 }
 
 // ---- iterative solution ---------------------------------------------------
+// ============================================================ three phonons
+int fp4_ctx_set_ifc3(fp4_ctx *c, int32_t ntri, const double *Phi, const double *R_j, const double *R_k,
+                     const int32_t *Ii, const int32_t *Ij, const int32_t *Ik) {
+  if (!c || !Phi || !R_j || !R_k || !Ii || !Ij || !Ik) return fail(FP4_ERR_ARG, "null argument");
+  if (ntri < 1) return fail(FP4_ERR_ARG, "Ntri must be positive");
+  HIPCHECK(hipSetDevice(c->device));
+  int rc;
+  if ((rc = ensure_roots(c))) return rc;
+  std::vector<Cluster3Rec> cl(ntri, Cluster3Rec{});
+  for (int t = 0; t < ntri; t++) {
+    Cluster3Rec &r = cl[t];
+    const int32_t *ix[3] = {Ii, Ij, Ik};
+    for (int a = 0; a < 3; a++) {
+      if (ix[a][t] < 1 || ix[a][t] > c->natoms) return fail(FP4_ERR_ARG, "Index out of range at cluster %d", t + 1);
+      r.atom[a] = ix[a][t] - 1;
+    }
+    if ((rc = phase_multipliers(c, R_j + 3 * (size_t)t, c->L, r.pj, t + 1))) return rc;
+    if ((rc = phase_multipliers(c, R_k + 3 * (size_t)t, c->L, r.pk, t + 1))) return rc;
+    // 1/sqrt(m_i*m_j*m_k), processes.f90:59-60
+    double mp = c->masses[c->types[r.atom[0]] - 1] * c->masses[c->types[r.atom[1]] - 1];
+    mp = mp * c->masses[c->types[r.atom[2]] - 1];
+    r.minv = 1.0 / std::sqrt(mp);
+  }
+  if ((rc = c->clusters3.ensure(sizeof(Cluster3Rec) * ntri)) || (rc = c->phi3.ensure(sizeof(double) * 27 * ntri)))
+    return rc;
+  HIPCHECK(hipMemcpy(c->clusters3.p, cl.data(), sizeof(Cluster3Rec) * ntri, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->phi3.p, Phi, sizeof(double) * 27 * ntri, hipMemcpyHostToDevice));
+  c->ntri3 = ntri;
+  c->have_ifc3 = true;
+  return FP4_OK;
+}
+
+static void all_modes3(const fp4_ctx *c, Plan &pl) {
+  pl.modes.resize((size_t)c->nb * c->nlist);
+  std::iota(pl.modes.begin(), pl.modes.end(), 1);
+  pl.win_lo0 = 0;
+  pl.win_n = c->nptk;
+}
+
+// counts of the reference's int32 kind; totals must fit the int32 list sizes
+static int counts3_fit(const Plan &pl) {
+  const int nsub = (int)pl.modes.size();
+  for (int ch = 0; ch < 2; ch++) {
+    long long tot = 0;
+    for (int m = 0; m < nsub; m++) tot += pl.segN[ch * nsub + m];
+    if (tot > INT32_MAX)
+      return fail(FP4_ERR_OVERFLOW, "channel %d has %lld three-phonon processes, beyond the int32 lists", ch, tot);
+  }
+  return 0;
+}
+
+static int verify_counts3(const Plan &pl, const int32_t *N_plus, const int32_t *N_minus) {
+  const int nm = (int)pl.modes.size();
+  const int32_t *N[2] = {N_plus, N_minus};
+  for (int ch = 0; ch < 2; ch++)
+    for (int m = 0; m < nm; m++)
+      if ((long long)N[ch][m] != pl.segN[ch * nm + m])
+        return fail(FP4_ERR_OVERFLOW, "count mismatch: channel %d mode %d has %lld processes, caller passed %d", ch,
+                    m + 1, (long long)pl.segN[ch * nm + m], (int)N[ch][m]);
+  return 0;
+}
+
+int fp4_np_3ph(fp4_ctx *c, int32_t *N_plus, double *P_plus, int32_t *N_minus, double *P_minus) {
+  int rc;
+  if ((rc = check_ready3(c, false))) return rc;
+  if (!N_plus || !P_plus || !N_minus || !P_minus) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  all_modes3(c, pl);
+  if ((rc = count3_pass(c, pl, false))) return rc;
+  if ((rc = counts3_fit(pl))) return rc;
+  const int nm = (int)pl.modes.size();
+  int32_t *N[2] = {N_plus, N_minus};
+  double *P[2] = {P_plus, P_minus};
+  for (int ch = 0; ch < 2; ch++)
+    for (int m = 0; m < nm; m++) {
+      N[ch][m] = (int32_t)pl.segN[ch * nm + m];
+      P[ch][m] = pl.segP[ch * nm + m];
+    }
+  return FP4_OK;
+}
+
+// Copies each channel's part of a batch from c->lists (and c->proc_g) to the host.
+static int copy3_batch(fp4_ctx *c, const Plan &pl, const Batch &b, int32_t *const outl[2][2], double *const outg[2]) {
+  const long per_ch = pl.items / 2;
+  for (int ch = 0; ch < 2; ch++) {
+    const long lo = std::max(b.item_lo, ch * per_ch), hi = std::min(b.item_hi, (ch + 1) * per_ch);
+    if (hi <= lo) continue;
+    const long long p0 = pl.off[lo], p1 = pl.off[hi], base = pl.off[ch * per_ch];
+    if (p1 <= p0) continue;
+    for (int t = 0; t < 2; t++) {
+      if (!outl[ch][t]) return fail(FP4_ERR_ARG, "null list output");
+      HIPCHECK(hipMemcpyAsync(outl[ch][t] + (p0 - base), c->lists[t].as<int>() + (p0 - b.p0),
+                              sizeof(int) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (outg) {
+      if (!outg[ch]) return fail(FP4_ERR_ARG, "null Gamma output");
+      HIPCHECK(hipMemcpyAsync(outg[ch] + (p0 - base), c->proc_g.as<double>() + (p0 - b.p0),
+                              sizeof(double) * (p1 - p0), hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fp4_ind_only_3ph(fp4_ctx *c, const int32_t *N_plus, const int32_t *N_minus, int32_t *a2, int32_t *a3,
+                     int32_t *b2, int32_t *b3) {
+  int rc;
+  if ((rc = check_ready3(c, false))) return rc;
+  if (!N_plus || !N_minus) return fail(FP4_ERR_ARG, "null counts");
+  Plan pl;
+  all_modes3(c, pl);
+  if ((rc = count3_pass(c, pl, true))) return rc;
+  if ((rc = counts3_fit(pl)) || (rc = verify_counts3(pl, N_plus, N_minus))) return rc;
+  int32_t *const outl[2][2] = {{a2, a3}, {b2, b3}};
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit3_batch(c, pl, b))) return rc;
+    if ((rc = copy3_batch(c, pl, b, outl, nullptr))) return rc;
+  }
+  return FP4_OK;
+}
+
+int fp4_rta_3ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi, double *rate,
+                      double *r_plus, double *r_minus, double *w_plus, double *w_minus, int64_t *nproc) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  if (!rate || !r_plus || !r_minus || !w_plus || !w_minus) return fail(FP4_ERR_ARG, "null output");
+  if (ii_lo < 1 || ii_hi > c->nptk || ii_hi < ii_lo) return fail(FP4_ERR_ARG, "bad ii window %d..%d", ii_lo, ii_hi);
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  pl.win_lo0 = ii_lo - 1;
+  pl.win_n = ii_hi - ii_lo + 1;
+  if ((rc = count3_pass(c, pl, true))) return rc;
+  std::vector<double> g(2 * (size_t)nmodes, 0.0), w(2 * (size_t)nmodes, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit3_batch(c, pl, b))) return rc;
+    if ((rc = rate3_batch(c, pl, b, false, g, w))) return rc;
+  }
+  write3_rates(c, pl, g, w, rate, r_plus, r_minus, w_plus, w_minus, true);
+  if (nproc) *nproc = std::accumulate(pl.segN.begin(), pl.segN.end(), 0LL);
+  return FP4_OK;
+}
+
+int fp4_rta_3ph(fp4_ctx *c, double *rate, double *r_plus, double *r_minus, double *w_plus, double *w_minus) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  const size_t nm = (size_t)c->nb * c->nlist;
+  double *outs[5] = {rate, r_plus, r_minus, w_plus, w_minus};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  std::vector<int> modes(nm);
+  std::iota(modes.begin(), modes.end(), 1);
+  return fp4_rta_3ph_modes(c, (int)nm, modes.data(), 1, c->nptk, rate, r_plus, r_minus, w_plus, w_minus, nullptr);
+}
+
+int fp4_rta_3ph_using_ind(fp4_ctx *c, const int32_t *N_plus, const int32_t *N_minus, const int32_t *a2,
+                          const int32_t *b2, const int32_t *a3, const int32_t *b3, double *rate, double *r_plus,
+                          double *r_minus, double *w_plus, double *w_minus) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  const int nm = c->nb * c->nlist;
+  const int32_t *N[2] = {N_plus, N_minus};
+  const int32_t *L2[2][2] = {{a2, a3}, {b2, b3}};
+  double *outs[5] = {rate, r_plus, r_minus, w_plus, w_minus};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  // one item per (channel, mode); offsets from the given counts
+  Plan pl;
+  pl.modes.resize(nm);
+  std::iota(pl.modes.begin(), pl.modes.end(), 1);
+  pl.nii = 1;
+  pl.items = 2L * nm;
+  pl.off.assign(pl.items + 1, 0);
+  for (int ch = 0; ch < 2; ch++)
+    for (int m = 0; m < nm; m++) {
+      if (!N[ch] || N[ch][m] < 0) return fail(FP4_ERR_ARG, "bad counts");
+      pl.off[(long)ch * nm + m + 1] = pl.off[(long)ch * nm + m] + N[ch][m];
+    }
+  const long long maxv = (long long)c->nptk * c->nb;
+  std::vector<double> g(2 * (size_t)nm, 0.0), w(2 * (size_t)nm, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    const long long n = b.p1 - b.p0;
+    for (int t = 0; t < 2; t++)
+      if ((rc = c->lists[t].ensure(4 * (size_t)n))) return rc;
+    for (long it = b.item_lo; it < b.item_hi; it++) {
+      const int ch = (int)(it / nm);
+      const long long p0 = pl.off[it], p1 = pl.off[it + 1];
+      if (p1 <= p0) continue;
+      const long long base = pl.off[(long)ch * nm];
+      for (int t = 0; t < 2; t++) {
+        if (!L2[ch][t]) return fail(FP4_ERR_ARG, "null list");
+        const int32_t *src = L2[ch][t] + (p0 - base);
+        for (long long x = 0; x < p1 - p0; x++)
+          if (src[x] < 1 || src[x] > maxv) return fail(FP4_ERR_ARG, "list entry out of range");
+        HIPCHECK(hipMemcpyAsync(c->lists[t].as<int>() + (p0 - b.p0), src, sizeof(int) * (p1 - p0),
+                                hipMemcpyHostToDevice, c->stream));
+      }
+    }
+    if ((rc = rate3_batch(c, pl, b, false, g, w))) return rc;
+  }
+  write3_rates(c, pl, g, w, rate, r_plus, r_minus, w_plus, w_minus, true);
+  return FP4_OK;
+}
+
+int fp4_ind_3ph(fp4_ctx *c, const int32_t *N_plus, const int32_t *N_minus, int32_t *a2, int32_t *a3, double *G_plus,
+                int32_t *b2, int32_t *b3, double *G_minus, double *rate, double *r_plus, double *r_minus,
+                double *w_plus, double *w_minus) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  if (!N_plus || !N_minus) return fail(FP4_ERR_ARG, "null counts");
+  const size_t nm = (size_t)c->nb * c->nlist;
+  double *outs[5] = {rate, r_plus, r_minus, w_plus, w_minus};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  Plan pl;
+  all_modes3(c, pl);
+  if ((rc = count3_pass(c, pl, true))) return rc;
+  if ((rc = counts3_fit(pl)) || (rc = verify_counts3(pl, N_plus, N_minus))) return rc;
+  int32_t *const outl[2][2] = {{a2, a3}, {b2, b3}};
+  double *const outg[2] = {G_plus, G_minus};
+  std::vector<double> g(2 * nm, 0.0), w(2 * nm, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = emit3_batch(c, pl, b))) return rc;
+    if ((rc = rate3_batch(c, pl, b, true, g, w))) return rc;
+    if ((rc = copy3_batch(c, pl, b, outl, outg))) return rc;
+  }
+  // per-process Gamma are already in THz; Ind_driver sums them per mode
+  write3_rates(c, pl, g, w, rate, r_plus, r_minus, w_plus, w_minus, false);
+  return FP4_OK;
+}
+
+int fp4_vp_3ph(fp4_ctx *c, int32_t channel, int64_t n, const int32_t *idx, double *out) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  if (channel < 0 || channel > 1) return fail(FP4_ERR_ARG, "channel must be 0 or 1");
+  if (n < 0 || (n > 0 && (!idx || !out))) return fail(FP4_ERR_ARG, "bad arguments");
+  if (n == 0) return FP4_OK;
+  for (int64_t t = 0; t < n; t++) {
+    const int32_t *r = idx + 6 * t;
+    for (int a = 0; a < 3; a++)
+      if (r[a] < 1 || r[a] > c->nb) return fail(FP4_ERR_ARG, "branch index out of range in row %lld", (long long)t);
+    for (int a = 3; a < 6; a++)
+      if (r[a] < 1 || r[a] > c->nptk) return fail(FP4_ERR_ARG, "q index out of range in row %lld", (long long)t);
+  }
+  DevBuf di, dout;
+  if ((rc = di.ensure(sizeof(int) * 6 * n)) || (rc = dout.ensure(sizeof(double) * 2 * n))) {
+    di.release();
+    return rc;
+  }
+  Vp3Args a{};
+  a.m = mode_view(c);
+  a.f = ifc3_view(c);
+  a.ch = channel;
+  a.n = n;
+  a.idx = di.as<int>();
+  a.out = dout.as<double2>();
+  hipError_t e = hipMemcpyAsync(di.p, idx, sizeof(int) * 6 * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_vp3(a, c->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  di.release();
+  dout.release();
+  if (e != hipSuccess) return fail(FP4_ERR_DEVICE, "fp4_vp_3ph: %s", hipGetErrorString(e));
+  return FP4_OK;
+}
+
 static BteModes bte_modes(const fp4_ctx *c) {
   BteModes m;
   m.omega = c->energy.as<double>();

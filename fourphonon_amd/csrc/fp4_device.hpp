@@ -31,6 +31,7 @@ constexpr double kHbar = 1.05457172647e-22;
 constexpr float kClamp = 0.001f;           // processes.f90:2315
 constexpr float kGammaUnits = 3.37617087f; // processes.f90:3922
 constexpr float k1e22 = 1e22f;             // processes.f90:20
+constexpr float kGamma3Units = 5.60626442f; // processes.f90:433, 512, 1389, 1469
 
 struct ClusterRec {
   int atom[4];     // 0-based atoms of phonons 1..4
@@ -39,6 +40,16 @@ struct ClusterRec {
   int pl[3];
   int pad;
   double minv;     // 1/sqrt(m_i m_j m_k m_l), reference expression order
+};
+
+// Third-order cluster: atoms of phonons 1..3, phase multipliers of R_j, R_k
+// (q.R = 2 pi (sum_nu n_nu p[nu]) / L) and 1/sqrt(m_i m_j m_k).
+struct Cluster3Rec {
+  int atom[3];
+  int pj[3];
+  int pk[3];
+  int pad;
+  double minv;
 };
 
 struct ModeView {
@@ -61,6 +72,14 @@ struct ModeView {
 struct IfcView {
   const ClusterRec *cl;
   const double *phi;     // [Ntri][81], (u,t,s,r) with u fastest
+  const double2 *roots;  // [L] e^{2 pi i k / L}
+  int ntri;
+  int L;
+};
+
+struct Ifc3View {
+  const Cluster3Rec *cl;
+  const double *phi;     // [Ntri][27], (t,s,r) with t fastest
   const double2 *roots;  // [L] e^{2 pi i k / L}
   int ntri;
   int L;

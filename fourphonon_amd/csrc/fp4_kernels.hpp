@@ -70,4 +70,45 @@ struct VpArgs {
 };
 int launch_vp64(const VpArgs &a, hipStream_t st);
 
+// Three-phonon scan: items (channel, mode-in-subset, j) with j fastest,
+// second-phonon q window ii_lo0 .. ii_lo0 + nii - 1.
+struct Scan3Args {
+  ModeView m;
+  const int *modes;
+  int nsub;
+  int nii, ii_lo0;
+  long item_lo, item_hi;
+  int *cnt;                  // count pass: [items]
+  double *psum;              // count pass: [items]
+  const long long *offsets;  // emit pass: exclusive prefix of cnt
+  long long out_base;
+  int *out[2];               // emit pass: Indof2ndPhonon, Indof3rdPhonon
+};
+int launch_scan3(const Scan3Args &a, int mode, hipStream_t st);
+
+struct Rate3Args {
+  ModeView m;
+  Ifc3View f;
+  const int *lst[2];
+  const BlockDesc *blocks;
+  int nblocks;
+  double T;
+  int onlyharmonic;
+  int store_gamma;           // Ind_driver: per-process Gamma in THz
+  double *proc_gamma;
+  double *blk_gamma;
+  double *blk_wp;
+};
+int launch_rate3(const Rate3Args &a, hipStream_t st);
+
+struct Vp3Args {
+  ModeView m;
+  Ifc3View f;
+  int ch;
+  long n;
+  const int *idx;   // n rows of (i, j, k, q, q', q'') 1-based
+  double2 *out;
+};
+int launch_vp3(const Vp3Args &a, hipStream_t st);
+
 }  // namespace fp4

@@ -1,6 +1,7 @@
 ! Drop-in replacements for the four-phonon procedures of FourPhonon's
-! `module processes` (Src/processes.f90), running on an MI355X through the
-! fp4 C ABI (include/fp4.h) via iso_c_binding.
+! `module processes` (Src/processes.f90) and their three-phonon companions,
+! running on an MI355X through the fp4 C ABI (include/fp4.h) via
+! iso_c_binding.
 !
 ! Procedure names, argument order, kinds and array shapes are those of the
 ! reference:
@@ -10,6 +11,12 @@
 !   RTA_driver_4ph_GPU_using_Ind  Src/processes.f90:5315-5359
 !   Ind_driver_4ph                Src/processes.f90:3551-3589
 !   Vp_pp / Vp_pm / Vp_mm         Src/processes.f90:142-330
+!   NP_driver                     Src/processes.f90:983-995
+!   Ind_only_driver               Src/processes.f90:1198-1214
+!   RTA_driver                    Src/processes.f90:1676-1697
+!   RTA_driver_GPU_using_Ind      Src/processes.f90:1778-1808
+!   Ind_driver                    Src/processes.f90:524-561
+!   Vp_plus / Vp_minus            Src/processes.f90:25-139
 !   iteration0                    Src/iterations.f90:13-30
 !   iteration_4ph                 Src/iterations.f90:95-120
 !   TConduct                      Src/conductivity.f90:11-16
@@ -127,6 +134,52 @@ module fp4_processes
        integer(c_int32_t), intent(in) :: idx(*)
        real(c_double), intent(out) :: out(*)
      end function c_vp
+     integer(c_int) function c_set_ifc3(ctx, ntri, Phi, R_j, R_k, Ii, Ij, Ik) bind(C, name="fp4_ctx_set_ifc3")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ntri
+       real(c_double), intent(in) :: Phi(*), R_j(*), R_k(*)
+       integer(c_int32_t), intent(in) :: Ii(*), Ij(*), Ik(*)
+     end function c_set_ifc3
+     integer(c_int) function c_np3(ctx, Np, Pp, Nm, Pm) bind(C, name="fp4_np_3ph")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: Np(*), Nm(*)
+       real(c_double), intent(out) :: Pp(*), Pm(*)
+     end function c_np3
+     integer(c_int) function c_ind_only3(ctx, Np, Nm, a2, a3, b2, b3) bind(C, name="fp4_ind_only_3ph")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: Np(*), Nm(*)
+       integer(c_int32_t), intent(out) :: a2(*), a3(*), b2(*), b3(*)
+     end function c_ind_only3
+     integer(c_int) function c_rta3(ctx, r, rp, rm, wp, wm) bind(C, name="fp4_rta_3ph")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: r(*), rp(*), rm(*), wp(*), wm(*)
+     end function c_rta3
+     integer(c_int) function c_rta3_ind(ctx, Np, Nm, a2, b2, a3, b3, r, rp, rm, wp, wm) &
+          bind(C, name="fp4_rta_3ph_using_ind")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: Np(*), Nm(*), a2(*), b2(*), a3(*), b3(*)
+       real(c_double), intent(out) :: r(*), rp(*), rm(*), wp(*), wm(*)
+     end function c_rta3_ind
+     integer(c_int) function c_ind3(ctx, Np, Nm, a2, a3, gp, b2, b3, gm, r, rp, rm, wp, wm) bind(C, name="fp4_ind_3ph")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: Np(*), Nm(*)
+       integer(c_int32_t), intent(out) :: a2(*), a3(*), b2(*), b3(*)
+       real(c_double), intent(out) :: gp(*), gm(*), r(*), rp(*), rm(*), wp(*), wm(*)
+     end function c_ind3
+     integer(c_int) function c_vp3(ctx, channel, n, idx, out) bind(C, name="fp4_vp_3ph")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: channel
+       integer(c_int64_t), value :: n
+       integer(c_int32_t), intent(in) :: idx(*)
+       real(c_double), intent(out) :: out(*)
+     end function c_vp3
      integer(c_int) function c_set_symmetry(ctx, nsymm, ideq, symm, crot) bind(C, name="fp4_ctx_set_symmetry")
        import :: c_int, c_ptr, c_double, c_int32_t
        type(c_ptr), value :: ctx
@@ -190,6 +243,7 @@ module fp4_processes
   public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision
   public :: NP_driver_4ph, Ind_only_driver_4ph, RTA_driver_4ph, RTA_driver_4ph_GPU_using_Ind, Ind_driver_4ph
   public :: Vp_pp, Vp_pm, Vp_mm
+  public :: NP_driver, Ind_only_driver, RTA_driver, RTA_driver_GPU_using_Ind, Ind_driver, Vp_plus, Vp_minus
   public :: fp4_init_iterations, fp4_keep_operator, fp4_set_operator_4ph, fp4_solve_4ph
   public :: iteration0, iteration_4ph, TConduct
 
@@ -440,6 +494,126 @@ contains
     integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:), Index_l(:)
     Vp_mm = vp_one(2_c_int32_t, i, j, k, l, q, qprime, qdprime, qtprime)
   end function Vp_mm
+
+  ! ------------------------------------------------------ three phonons
+  subroutine upload_ifc3(Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    integer(kind=4), intent(in) :: Ntri
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    call check(c_set_ifc3(ctx, Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k), "fp4_ctx_set_ifc3")
+  end subroutine upload_ifc3
+
+  subroutine NP_driver(energy, velocity, Nlist, List, IJK, N_plus, Pspace_plus_total, N_minus, Pspace_minus_total)
+    real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
+    integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:)
+    integer(kind=4), intent(out) :: N_plus(:), N_minus(:)
+    real(kind=8), intent(out) :: Pspace_plus_total(:,:), Pspace_minus_total(:,:)
+    call upload_modes_noeig(energy, velocity, Nlist, List, IJK)
+    call check(c_np3(ctx, N_plus, Pspace_plus_total, N_minus, Pspace_minus_total), "NP_driver")
+  end subroutine NP_driver
+
+  subroutine Ind_only_driver(energy, velocity, Nlist, List, IJK, N_plus, N_minus, &
+       Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, Indof3rdPhonon_minus)
+    real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
+    integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:)
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:)
+    integer(kind=4), intent(inout) :: Indof2ndPhonon_plus(:), Indof3rdPhonon_plus(:)
+    integer(kind=4), intent(inout) :: Indof2ndPhonon_minus(:), Indof3rdPhonon_minus(:)
+    call upload_modes_noeig(energy, velocity, Nlist, List, IJK)
+    call check(c_ind_only3(ctx, N_plus, N_minus, Indof2ndPhonon_plus, Indof3rdPhonon_plus, &
+         Indof2ndPhonon_minus, Indof3rdPhonon_minus), "Ind_only_driver")
+  end subroutine Ind_only_driver
+
+  subroutine RTA_driver(energy, velocity, eigenvect, Nlist, List, IJK, Ntri, Phi, R_j, R_k, &
+       Index_i, Index_j, Index_k, rate_scatt, rate_scatt_plus, rate_scatt_minus, WP3_plus, WP3_minus)
+    real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
+    complex(kind=8), intent(in) :: eigenvect(:,:,:)
+    integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:), Ntri
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    real(kind=8), intent(out) :: rate_scatt(:,:), rate_scatt_plus(:,:), rate_scatt_minus(:,:), &
+         WP3_plus(:,:), WP3_minus(:,:)
+    call upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
+    call upload_ifc3(Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    call check(c_rta3(ctx, rate_scatt, rate_scatt_plus, rate_scatt_minus, WP3_plus, WP3_minus), "RTA_driver")
+  end subroutine RTA_driver
+
+  subroutine RTA_driver_GPU_using_Ind(energy, velocity, eigenvect, Nlist, List, IJK, Ntri, Phi, R_j, R_k, &
+       Index_i, Index_j, Index_k, N_plus, N_minus, Indof2ndPhonon_plus, Indof2ndPhonon_minus, &
+       Indof3rdPhonon_plus, Indof3rdPhonon_minus, rate_scatt, rate_scatt_plus, rate_scatt_minus, &
+       WP3_plus, WP3_minus)
+    real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
+    complex(kind=8), intent(in) :: eigenvect(:,:,:)
+    integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:), Ntri
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:)
+    integer(kind=4), intent(in) :: Indof2ndPhonon_plus(:), Indof2ndPhonon_minus(:)
+    integer(kind=4), intent(in) :: Indof3rdPhonon_plus(:), Indof3rdPhonon_minus(:)
+    real(kind=8), intent(out) :: rate_scatt(:,:), rate_scatt_plus(:,:), rate_scatt_minus(:,:), &
+         WP3_plus(:,:), WP3_minus(:,:)
+    call upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
+    call upload_ifc3(Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    call check(c_rta3_ind(ctx, N_plus, N_minus, Indof2ndPhonon_plus, Indof2ndPhonon_minus, &
+         Indof3rdPhonon_plus, Indof3rdPhonon_minus, rate_scatt, rate_scatt_plus, rate_scatt_minus, &
+         WP3_plus, WP3_minus), "RTA_driver_GPU_using_Ind")
+  end subroutine RTA_driver_GPU_using_Ind
+
+  subroutine Ind_driver(energy, velocity, eigenvect, Nlist, List, IJK, N_plus, N_minus, &
+       Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k, &
+       Indof2ndPhonon_plus, Indof3rdPhonon_plus, Gamma_plus, &
+       Indof2ndPhonon_minus, Indof3rdPhonon_minus, Gamma_minus, &
+       rate_scatt, rate_scatt_plus, rate_scatt_minus, WP3_plus, WP3_minus)
+    real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
+    complex(kind=8), intent(in) :: eigenvect(:,:,:)
+    integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:), Ntri
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:)
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    integer(kind=4), intent(out) :: Indof2ndPhonon_plus(:), Indof3rdPhonon_plus(:)
+    integer(kind=4), intent(out) :: Indof2ndPhonon_minus(:), Indof3rdPhonon_minus(:)
+    real(kind=8), intent(out) :: Gamma_plus(:), Gamma_minus(:)
+    real(kind=8), intent(out) :: rate_scatt(:,:), rate_scatt_plus(:,:), rate_scatt_minus(:,:), &
+         WP3_plus(:,:), WP3_minus(:,:)
+    call upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
+    call upload_ifc3(Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    call check(c_ind3(ctx, N_plus, N_minus, Indof2ndPhonon_plus, Indof3rdPhonon_plus, Gamma_plus, &
+         Indof2ndPhonon_minus, Indof3rdPhonon_minus, Gamma_minus, rate_scatt, rate_scatt_plus, &
+         rate_scatt_minus, WP3_plus, WP3_minus), "Ind_driver")
+  end subroutine Ind_driver
+
+  complex(kind=8) function vp3_one(channel, i, j, k, q, qprime, qdprime)
+    integer(c_int32_t), intent(in) :: channel
+    integer(kind=4), intent(in) :: i, j, k, q, qprime, qdprime
+    integer(c_int32_t) :: idx(6)
+    real(c_double) :: out(2)
+    idx = [i, j, k, q, qprime, qdprime]
+    call check(c_vp3(ctx, channel, 1_c_int64_t, idx, out), "fp4_vp_3ph")
+    vp3_one = cmplx(out(1), out(2), kind=8)
+  end function vp3_one
+
+  ! Vp_plus / Vp_minus use the modes and third-order clusters of the last
+  ! three-phonon driver call; their array arguments are accepted for
+  ! interface compatibility.
+  complex(kind=8) function Vp_plus(i, j, k, q, qprime, qdprime, realqprime, realqdprime, eigenvect, &
+       Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    integer(kind=4), intent(in) :: i, j, k, q, qprime, qdprime, Ntri
+    real(kind=8), intent(in) :: realqprime(3), realqdprime(3)
+    complex(kind=8), intent(in) :: eigenvect(:,:,:)
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    Vp_plus = vp3_one(0_c_int32_t, i, j, k, q, qprime, qdprime)
+  end function Vp_plus
+
+  complex(kind=8) function Vp_minus(i, j, k, q, qprime, qdprime, realqprime, realqdprime, eigenvect, &
+       Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
+    integer(kind=4), intent(in) :: i, j, k, q, qprime, qdprime, Ntri
+    real(kind=8), intent(in) :: realqprime(3), realqdprime(3)
+    complex(kind=8), intent(in) :: eigenvect(:,:,:)
+    real(kind=8), intent(in) :: Phi(:,:,:,:), R_j(:,:), R_k(:,:)
+    integer(kind=4), intent(in) :: Index_i(:), Index_j(:), Index_k(:)
+    Vp_minus = vp3_one(1_c_int32_t, i, j, k, q, qprime, qdprime)
+  end function Vp_minus
 
   ! ---------------------------------------------------------- iterations
   ! Globals read by Src/iterations.f90 and Src/conductivity.f90: the cell

@@ -33,10 +33,12 @@ ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR
 EXPORTS = [
     "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
-    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_ind_only_4ph_modes", "fp4_ctx_kernel_stats",
+    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_ind_only_4ph_modes",
+    "fp4_ctx_kernel_stats",
     "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape",
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
-    "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve",
+    "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
+    "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
 ]
 
 
@@ -114,6 +116,13 @@ class Context:
                                   _p(np.asfortranarray(f.R_l, np.float64)),
                                   *[_p(np.ascontiguousarray(x, np.int32))
                                     for x in (f.Index_i, f.Index_j, f.Index_k, f.Index_l)]))
+        f3 = getattr(case, "ifc3", None)
+        if f3 is not None:
+            _check(L.fp4_ctx_set_ifc3(h, C.c_int32(f3.Ntri), _p(np.asfortranarray(f3.Phi, np.float64)),
+                                      _p(np.asfortranarray(f3.R_j, np.float64)),
+                                      _p(np.asfortranarray(f3.R_k, np.float64)),
+                                      *[_p(np.ascontiguousarray(x, np.int32))
+                                        for x in (f3.Index_i, f3.Index_j, f3.Index_k)]))
         self.set_symmetry()
 
     def close(self):
@@ -141,6 +150,15 @@ class Context:
         N = [np.zeros(nm, np.int64) for _ in range(3)]
         P = [np.zeros((self.nb, self.nlist), order="F") for _ in range(3)]
         _check(load().fp4_np_4ph(self.h, _p(N[0]), _p(P[0]), _p(N[1]), _p(P[1]), _p(N[2]), _p(P[2])))
+        return N[0], P[0], N[1], P[1], N[2], P[2]
+
+    def NP_driver_4ph_modes(self, modes):
+        """Counts and phase space of a mode subset -> (N_pp, P_pp, N_pm, P_pm, N_mm, P_mm), each len(modes)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        N = [np.zeros(len(modes), np.int64) for _ in range(3)]
+        P = [np.zeros(len(modes)) for _ in range(3)]
+        _check(load().fp4_np_4ph_modes(self.h, C.c_int32(len(modes)), _p(modes), _p(N[0]), _p(P[0]), _p(N[1]),
+                                       _p(P[1]), _p(N[2]), _p(P[2])))
         return N[0], P[0], N[1], P[1], N[2], P[2]
 
     def Ind_only_driver_4ph(self, N_pp, N_pm, N_mm):
@@ -309,7 +327,7 @@ class Context:
         return {"F_n": F, "kappa": kappa, "kappa_iter": hist[:, :, :k + 1], "relchange": rel[:k + 1], "niter": k}
 
     # ------------------------------------------------------------ diagnostics
-    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE, KS_ITER = 0, 1, 2, 3, 4
+    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE, KS_ITER, KS_SCAN3, KS_RATE3 = 0, 1, 2, 3, 4, 5, 6
 
     def kernel_stats(self, kind):
         ms = C.c_double()
@@ -332,6 +350,70 @@ class Context:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"mfma": a.value, "tq": b.value, "ib": c.value}   # mfma: 0 VALU, 1 FP32, 2 BF16 hi/lo
+
+    # ------------------------------------------------------- three phonons
+    _RATE3 = ["rate_scatt", "rate_scatt_plus", "rate_scatt_minus", "WP3_plus", "WP3_minus"]
+
+    def NP_driver(self):
+        """-> (N_plus, P_plus, N_minus, P_minus); N int32 (Nlist*Nb), P (Nb,Nlist)."""
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int32) for _ in range(2)]
+        P = [np.zeros((self.nb, self.nlist), order="F") for _ in range(2)]
+        _check(load().fp4_np_3ph(self.h, _p(N[0]), _p(P[0]), _p(N[1]), _p(P[1])))
+        return N[0], P[0], N[1], P[1]
+
+    def Ind_only_driver(self, N_plus, N_minus):
+        """-> [[Indof2nd, Indof3rd] for +, -] (int32, reference encoding)."""
+        Ns = [np.ascontiguousarray(n, np.int32) for n in (N_plus, N_minus)]
+        outs = [[np.zeros(max(int(n.sum()), 1), np.int32) for _ in range(2)] for n in Ns]
+        _check(load().fp4_ind_only_3ph(self.h, *[_p(n) for n in Ns], *[_p(o) for ch in outs for o in ch]))
+        return [[o[:int(n.sum())] for o in ch] for ch, n in zip(outs, Ns)]
+
+    def _rate3_outputs(self):
+        return [np.zeros((self.nb, self.nlist), order="F") for _ in range(5)]
+
+    def RTA_driver(self):
+        out = self._rate3_outputs()
+        _check(load().fp4_rta_3ph(self.h, *[_p(o) for o in out]))
+        return dict(zip(self._RATE3, out))
+
+    def RTA_driver_modes(self, modes, ii_lo=1, ii_hi=None):
+        """RTA_driver over a mode subset and a q' window -> (dict, nproc)."""
+        out = self._rate3_outputs()
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        nproc = C.c_int64()
+        _check(load().fp4_rta_3ph_modes(self.h, C.c_int32(len(modes)), _p(modes), C.c_int32(ii_lo),
+                                        C.c_int32(self.nptk if ii_hi is None else ii_hi),
+                                        *[_p(o) for o in out], C.byref(nproc)))
+        return dict(zip(self._RATE3, out)), nproc.value
+
+    def RTA_driver_GPU_using_Ind(self, N_plus, N_minus, lists):
+        Ns = [np.ascontiguousarray(n, np.int32) for n in (N_plus, N_minus)]
+        L2 = [[np.ascontiguousarray(x, np.int32) if len(x) else np.zeros(1, np.int32) for x in ch] for ch in lists]
+        out = self._rate3_outputs()
+        # reference argument order: 2nd+, 2nd-, 3rd+, 3rd-
+        _check(load().fp4_rta_3ph_using_ind(self.h, _p(Ns[0]), _p(Ns[1]), _p(L2[0][0]), _p(L2[1][0]),
+                                            _p(L2[0][1]), _p(L2[1][1]), *[_p(o) for o in out]))
+        return dict(zip(self._RATE3, out))
+
+    def Ind_driver(self, N_plus, N_minus):
+        """-> (lists, [Gamma_plus, Gamma_minus], rates dict)."""
+        Ns = [np.ascontiguousarray(n, np.int32) for n in (N_plus, N_minus)]
+        tot = [int(n.sum()) for n in Ns]
+        lists = [[np.zeros(max(t, 1), np.int32) for _ in range(2)] for t in tot]
+        gam = [np.zeros(max(t, 1)) for t in tot]
+        out = self._rate3_outputs()
+        _check(load().fp4_ind_3ph(self.h, _p(Ns[0]), _p(Ns[1]), _p(lists[0][0]), _p(lists[0][1]), _p(gam[0]),
+                                  _p(lists[1][0]), _p(lists[1][1]), _p(gam[1]), *[_p(o) for o in out]))
+        return ([[x[:t] for x in ch] for ch, t in zip(lists, tot)], [g[:t] for g, t in zip(gam, tot)],
+                dict(zip(self._RATE3, out)))
+
+    def Vp3(self, channel, rows):
+        """Vp_plus (0) / Vp_minus (1) for rows of (i, j, k, q, q', q'') -> complex array."""
+        rows = np.ascontiguousarray(rows, np.int32).reshape(-1, 6)
+        out = np.zeros((len(rows), 2))
+        _check(load().fp4_vp_3ph(self.h, C.c_int32(channel), C.c_int64(len(rows)), _p(rows), _p(out)))
+        return out[:, 0] + 1j * out[:, 1]
 
     def set_batch_capacity(self, processes):
         _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))

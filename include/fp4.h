@@ -134,6 +134,13 @@ int fp4_ind_4ph(fp4_ctx *ctx, const int64_t *N_pp, const int64_t *N_pm, const in
    receives n complex values. */
 int fp4_vp_4ph(fp4_ctx *ctx, int32_t channel, int64_t n, const int32_t *idx, double *out);
 
+/* NP_driver_4ph restricted to a strictly increasing subset of modes (mm,
+   1-based): one MPI rank's share of the reference's chunk loop
+   (Src/processes.f90:2798-2803).  N_* and P_* receive nmodes entries each,
+   in the given order. */
+int fp4_np_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp,
+                     int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm);
+
 /* Ind_only_driver_4ph restricted to a strictly increasing subset of modes
    (mm, 1-based), for callers that split the mode range the way the
    reference's MPI chunks do (Src/processes.f90:3160-3203).  N_* receive the
@@ -145,6 +152,62 @@ int fp4_ind_only_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, i
                            int64_t *Indof4thPhonon_pp, int64_t *Indof2ndPhonon_pm, int64_t *Indof3rdPhonon_pm,
                            int64_t *Indof4thPhonon_pm, int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm,
                            int64_t *Indof4thPhonon_mm);
+
+/* ---- three-phonon path (Src/processes.f90:25-1836) ----------------------
+   Channel 0 = "+" (absorption), 1 = "-" (emission).  Counts and lists are
+   int32, as the reference declares them; a channel whose total would not fit
+   returns FP4_ERR_OVERFLOW.  The mode data are those of fp4_ctx_set_modes. */
+
+/* FORCE_CONSTANTS_3RD as read by read3fc (Src/input.f90:102-140):
+   Phi(3,3,3,Ntri) eV/A^3, R_j/R_k(3,Ntri) nm on lattice vectors, 1-based
+   Index_i/j/k(Ntri). */
+int fp4_ctx_set_ifc3(fp4_ctx *ctx, int32_t ntri, const double *Phi, const double *R_j, const double *R_k,
+                     const int32_t *Index_i, const int32_t *Index_j, const int32_t *Index_k);
+
+/* NP_driver (Src/processes.f90:983-1067): N_plus/N_minus(Nlist*Nbands),
+   Pspace_plus_total/Pspace_minus_total(Nbands,Nlist). */
+int fp4_np_3ph(fp4_ctx *ctx, int32_t *N_plus, double *Pspace_plus_total, int32_t *N_minus,
+               double *Pspace_minus_total);
+
+/* Ind_only_driver (Src/processes.f90:1198-1309): lists in the reference's
+   order (per mode: j outer, q' inner, k innermost), entries (q-1)*Nb+branch. */
+int fp4_ind_only_3ph(fp4_ctx *ctx, const int32_t *N_plus, const int32_t *N_minus, int32_t *Indof2ndPhonon_plus,
+                     int32_t *Indof3rdPhonon_plus, int32_t *Indof2ndPhonon_minus, int32_t *Indof3rdPhonon_minus);
+
+/* RTA_driver (Src/processes.f90:1676-1775): rate_scatt, rate_scatt_plus,
+   rate_scatt_minus, WP3_plus, WP3_minus, all (Nbands,Nlist), FP64. */
+int fp4_rta_3ph(fp4_ctx *ctx, double *rate_scatt, double *rate_scatt_plus, double *rate_scatt_minus,
+                double *WP3_plus, double *WP3_minus);
+
+/* RTA_driver over a strictly increasing mode subset (mm, 1-based) and a
+   second-phonon window ii_lo..ii_hi (1-based, inclusive), as one MPI rank's
+   chunk of the reference's loop (Src/processes.f90:1721-1765) restricted in
+   q'.  Entries of modes outside the subset are left untouched; nproc (may be
+   NULL) receives the number of processes visited. */
+int fp4_rta_3ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi,
+                      double *rate_scatt, double *rate_scatt_plus, double *rate_scatt_minus, double *WP3_plus,
+                      double *WP3_minus, int64_t *nproc);
+
+/* RTA_driver_GPU_using_Ind (Src/processes.f90:1778-1836), argument order of
+   the reference.  Uses the float32-promoted rate units of the CPU routines
+   (5.60626442 as a default real), like the other entry points. */
+int fp4_rta_3ph_using_ind(fp4_ctx *ctx, const int32_t *N_plus, const int32_t *N_minus,
+                          const int32_t *Indof2ndPhonon_plus, const int32_t *Indof2ndPhonon_minus,
+                          const int32_t *Indof3rdPhonon_plus, const int32_t *Indof3rdPhonon_minus,
+                          double *rate_scatt, double *rate_scatt_plus, double *rate_scatt_minus, double *WP3_plus,
+                          double *WP3_minus);
+
+/* Ind_driver (Src/processes.f90:524-707): lists as fp4_ind_only_3ph plus the
+   per-process Gamma_plus/Gamma_minus (THz) the iterative solver reads. */
+int fp4_ind_3ph(fp4_ctx *ctx, const int32_t *N_plus, const int32_t *N_minus, int32_t *Indof2ndPhonon_plus,
+                int32_t *Indof3rdPhonon_plus, double *Gamma_plus, int32_t *Indof2ndPhonon_minus,
+                int32_t *Indof3rdPhonon_minus, double *Gamma_minus, double *rate_scatt, double *rate_scatt_plus,
+                double *rate_scatt_minus, double *WP3_plus, double *WP3_minus);
+
+/* Vp_plus (channel 0) / Vp_minus (channel 1), Src/processes.f90:25-139, for
+   n rows of (i, j, k, q, q', q'') (1-based); out receives n complex values
+   (re, im). */
+int fp4_vp_3ph(fp4_ctx *ctx, int32_t channel, int64_t n, const int32_t *idx, double *out);
 
 /* ---- iterative solution (Src/iterations.f90, Src/conductivity.f90) ------
    F_n(Nb,nptk,3) and the scattering operator stay on the device between
@@ -226,7 +289,8 @@ int fp4_bte_solve(fp4_ctx *ctx, double V, const double *tau_zero, int32_t maxite
    each kernel family, measured with HIP events on the context's stream:
    kind 0 = prefilter count pass, 1 = list emission, 2 = FP64 per-process
    rate kernel, 3 = dense rate kernels, 4 = iteration_4ph sweeps (launches
-   counts (ll, i, kk) steps). */
+   counts (ll, i, kk) steps), 5 = three-phonon scan (count and list passes),
+   6 = three-phonon FP64 rate kernel. */
 int fp4_ctx_kernel_stats(fp4_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 int fp4_ctx_reset_stats(fp4_ctx *ctx);
 /* Diagnostics; no counterpart in the reference.  Work of the last dense-form

@@ -717,3 +717,357 @@ void orc_symmetrize_tensor(int nsymm_rot, const double *crotations, double *tens
   }
   for (int t = 0; t < 9; t++) tensor[t] = tmp[t] / nsymm_rot;
 }
+
+/* ==========================================================================
+ * Three-phonon path (processes.f90:25-139, 361-1775).  Channel 0 = "+"
+ * (absorption, q'' = q + q'), 1 = "-" (emission, q'' = q - q').  The window
+ * is |delta| <= 2 sigma with sigma from v(q',j) - v(q'',k) and no clamp;
+ * NP_* and RTA_* loop ii outer, j inner (:740-741, :1352-1353), Ind_only_*
+ * and Ind_* loop j outer, ii inner (:1105-1106, :398-399).
+ * ======================================================================== */
+typedef struct {
+  int ii, ss, j, k;
+  double omega, omegap, omegadp, delta, sigma;
+} orc3_hit;
+
+typedef void (*visit3_fn)(const orc_case *c, const orc3_hit *h, void *ud);
+
+static void test3(const orc_case *c, int ch, const int32_t *q, double omega, int ii, int j, visit3_fn f, void *ud) {
+  const int nb = c->nbands;
+  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
+  const int32_t *qp = c->IJK + 3 * (size_t)(ii - 1);
+  const double omegap = E(c, ii, j);
+  for (int k = 1; k <= nb; k++) {
+    int t0, t1, t2;
+    if (ch == 0) {
+      t0 = q[0] + qp[0]; t1 = q[1] + qp[1]; t2 = q[2] + qp[2];
+    } else {
+      t0 = q[0] - qp[0]; t1 = q[1] - qp[1]; t2 = q[2] - qp[2];
+    }
+    t0 = fmodulo(t0, n1); t1 = fmodulo(t1, n2); t2 = fmodulo(t2, n3);
+    const int ss = (t2 * n2 + t1) * n1 + t0 + 1;
+    const double omegadp = E(c, ss, k);
+    if (!((omegap != 0.0) && (omegadp != 0.0))) continue;
+    double dv[3];
+    for (int a = 1; a <= 3; a++) dv[a - 1] = VEL(c, ii, j, a) - VEL(c, ss, k, a);
+    const double sigma = c->scalebroad * orc_base_sigma(c, dv);
+    const double delta = (ch == 0) ? omega + omegap - omegadp : omega - omegap - omegadp;
+    if (fabs(delta) <= 2.0 * sigma) {
+      orc3_hit h;
+      h.ii = ii; h.ss = ss; h.j = j; h.k = k;
+      h.omega = omega; h.omegap = omegap; h.omegadp = omegadp;
+      h.delta = delta; h.sigma = sigma;
+      f(c, &h, ud);
+    }
+  }
+}
+
+/* order 0: ii outer, j inner; order 1: j outer, ii inner.  ii_lo..ii_hi
+   restricts the second phonon's q index. */
+static void enumerate3(const orc_case *c, int ch, int mm, int order, int ii_lo, int ii_hi, visit3_fn f, void *ud) {
+  const int nb = c->nbands, nptk = c->nptk;
+  const int i = fmodulo(mm - 1, nb) + 1;
+  const int qi = c->List[(mm - 1) / nb];
+  const int32_t *q = c->IJK + 3 * (size_t)(qi - 1);
+  const double omega = E(c, qi, i);
+  if (!(omega != 0.0)) return;
+  if (ii_lo < 1) ii_lo = 1;
+  if (ii_hi > nptk) ii_hi = nptk;
+  if (order == 0) {
+    for (int ii = ii_lo; ii <= ii_hi; ii++)
+      for (int j = 1; j <= nb; j++) test3(c, ch, q, omega, ii, j, f, ud);
+  } else {
+    for (int j = 1; j <= nb; j++)
+      for (int ii = ii_lo; ii <= ii_hi; ii++) test3(c, ch, q, omega, ii, j, f, ud);
+  }
+}
+
+typedef struct { int64_t n; double p; double den; } np3_acc;
+
+/* P += exp(-d**2/sigma**2)/(sigma*sqrt(Pi)*nptk**2*nbands**3), processes.f90:756-758
+   (nptk**2 and nbands**3 are integer products, converted once each) */
+static void visit_np3(const orc_case *c, const orc3_hit *h, void *ud) {
+  np3_acc *a = (np3_acc *)ud;
+  (void)c;
+  a->n = a->n + 1;
+  const double num = exp(-(h->delta * h->delta) / (h->sigma * h->sigma));
+  const double nb3 = (double)(c->nbands * c->nbands * c->nbands);
+  const double den = h->sigma * sqrt(PI_D) * a->den * nb3;
+  a->p = a->p + num / den;
+}
+
+void orc3_np_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P) {
+  np3_acc a;
+  a.n = 0;
+  a.p = 0.0;
+  a.den = (double)((int64_t)c->nptk * c->nptk);
+  enumerate3(c, ch, mm, 0, 1, c->nptk, visit_np3, &a);
+  *N = a.n;
+  *P = a.p;
+}
+
+typedef struct { int64_t n, cap; int32_t *i2, *i3; int nb; } list3_acc;
+
+static void visit_list3(const orc_case *c, const orc3_hit *h, void *ud) {
+  list3_acc *a = (list3_acc *)ud;
+  (void)c;
+  if (a->n < a->cap) {
+    a->i2[a->n] = (int32_t)((h->ii - 1) * a->nb + h->j);
+    a->i3[a->n] = (int32_t)((h->ss - 1) * a->nb + h->k);
+  }
+  a->n = a->n + 1;
+}
+
+int64_t orc3_ind_only_mode(const orc_case *c, int ch, int mm, int64_t cap, int32_t *i2, int32_t *i3) {
+  list3_acc a;
+  a.n = 0; a.cap = cap; a.i2 = i2; a.i3 = i3; a.nb = c->nbands;
+  enumerate3(c, ch, mm, 1, 1, c->nptk, visit_list3, &a);
+  return a.n;
+}
+
+/* Vp_plus / Vp_minus, processes.f90:25-139: prefactor
+   1/sqrt(m_i m_j m_k) * phexp(+-q'.R_j) * phexp(-q''.R_k); the 27-term sum
+   runs rr (phonon 3) outer, ss, tt (phonon 1) inner as
+   ((Phi e1) e2) conj(e3), with e2 conjugated for "-". */
+void orc3_vp(const orc_case *c, int ch, int i, int j, int k, int q, int qp, int qdp, const double rqp[3],
+             const double rqdp[3], double out[2]) {
+  cplx V = {0.0, 0.0};
+  for (int t = 0; t < c->ntri3; t++) {
+    const int Ii = c->Index3_i[t], Ij = c->Index3_j[t], Ik = c->Index3_k[t];
+    double mprod = c->masses[c->types[Ii - 1] - 1] * c->masses[c->types[Ij - 1] - 1];
+    mprod = mprod * c->masses[c->types[Ik - 1] - 1];
+    const double *rj = c->R3_j + 3 * (size_t)t, *rk = c->R3_k + 3 * (size_t)t;
+    double dj = 0.0, dk = 0.0;
+    for (int a = 0; a < 3; a++) dj = dj + rqp[a] * rj[a];
+    for (int a = 0; a < 3; a++) dk = dk + rqdp[a] * rk[a];
+    cplx pre = rmul(1.0 / sqrt(mprod), phexp(ch == 0 ? dj : -dj));
+    pre = cmul(pre, phexp(-dk));
+    cplx V0 = {0.0, 0.0};
+    const double *phi = c->Phi3 + 27 * (size_t)t;
+    for (int rr = 1; rr <= 3; rr++)
+      for (int ss = 1; ss <= 3; ss++)
+        for (int tt = 1; tt <= 3; tt++) {
+          const double f = phi[(tt - 1) + 3 * (ss - 1) + 9 * (rr - 1)];
+          cplx e1 = EV(c, q, i, tt + 3 * (Ii - 1));
+          cplx e2 = EV(c, qp, j, ss + 3 * (Ij - 1));
+          cplx e3 = conj_(EV(c, qdp, k, rr + 3 * (Ik - 1)));
+          if (ch == 1) e2 = conj_(e2);
+          cplx x = rmul(f, e1);
+          x = cmul(x, e2);
+          x = cmul(x, e3);
+          V0.re = V0.re + x.re;
+          V0.im = V0.im + x.im;
+        }
+    cplx y = cmul(pre, V0);
+    V.re = V.re + y.re;
+    V.im = V.im + y.im;
+  }
+  out[0] = V.re;
+  out[1] = V.im;
+}
+
+void orc3_vp_idx(const orc_case *c, int ch, int i, int j, int k, int q, int qp, int qdp, double out[2]) {
+  double a[3], b[3];
+  realq_of(c, qp, a);
+  realq_of(c, qdp, b);
+  orc3_vp(c, ch, i, j, k, q, qp, qdp, a, b, out);
+}
+
+/* float32-promoted 3ph rate units, processes.f90:433, 512, 1389, 1469 */
+static const float F_GAMMA3_UNITS = 5.60626442f;
+
+typedef struct {
+  int i, qi, ch, nb, store, onlyharmonic;
+  double gamma, wp3;
+  int64_t n, cap;
+  int32_t *i2, *i3;
+  double *g;
+} rta3_acc;
+
+/* WP3: "+" (f' - f'') and "-" (f' + f'' + 1), times
+   exp(-d**2/sigma**2)/sigma/sqrt(Pi)/(omega*omega'*omega''), processes.f90:1365-1367, 1444-1446 */
+static double wp3_of(const orc_case *c, int ch, const orc3_hit *h) {
+  const double fp = bose(c, h->omegap), fdp = bose(c, h->omegadp);
+  const double occ = (ch == 0) ? fp - fdp : fp + fdp + 1;
+  return occ * exp(-(h->delta * h->delta) / (h->sigma * h->sigma)) / h->sigma / sqrt(PI_D) /
+         (h->omega * h->omegap * h->omegadp);
+}
+
+static void visit_rta3(const orc_case *c, const orc3_hit *h, void *ud) {
+  rta3_acc *a = (rta3_acc *)ud;
+  const double w = wp3_of(c, a->ch, h);
+  a->wp3 = a->wp3 + w;
+  const double hbarp = HBAR_D * (double)F_1E22;
+  if (a->store) {
+    /* Ind_plus / Ind_minus: per-process Gamma, always with Vp (:418-433) */
+    double V[2];
+    orc3_vp_idx(c, a->ch, a->i, h->j, h->k, a->qi, h->ii, h->ss, V);
+    const double av = hypot(V[0], V[1]);
+    double g = hbarp * PI_D / 4.0 * w * (av * av);
+    g = g * (double)F_GAMMA3_UNITS * 1.e8 / c->nptk;
+    if (a->n < a->cap) {
+      a->i2[a->n] = (int32_t)((h->ii - 1) * a->nb + h->j);
+      a->i3[a->n] = (int32_t)((h->ss - 1) * a->nb + h->k);
+      a->g[a->n] = g;
+    }
+    a->n = a->n + 1;
+  } else if (!a->onlyharmonic) {
+    double V[2];
+    orc3_vp_idx(c, a->ch, a->i, h->j, h->k, a->qi, h->ii, h->ss, V);
+    const double av = hypot(V[0], V[1]);
+    a->gamma = a->gamma + hbarp * PI_D / 4.0 * w * (av * av);
+  }
+}
+
+/* RTA_plus / RTA_minus (processes.f90:1313-1471): Gamma in THz before the
+   driver's factor 1/2 for "-", WP3 after its 1/nptk (and 1/2 for "-"). */
+void orc3_rta_mode(const orc_case *c, int ch, int mm, int32_t ii_lo, int32_t ii_hi, double *gamma, double *wp3) {
+  rta3_acc a;
+  memset(&a, 0, sizeof(a));
+  a.i = fmodulo(mm - 1, c->nbands) + 1;
+  a.qi = c->List[(mm - 1) / c->nbands];
+  a.ch = ch;
+  a.nb = c->nbands;
+  a.onlyharmonic = c->onlyharmonic;
+  const double omega = E(c, a.qi, a.i);
+  enumerate3(c, ch, mm, 0, ii_lo, ii_hi, visit_rta3, &a);
+  if (omega != 0.0) a.wp3 = (ch == 0) ? a.wp3 / c->nptk : a.wp3 * 5.e-1 / c->nptk;
+  *gamma = a.gamma * (double)F_GAMMA3_UNITS * 1.e8 / c->nptk;
+  *wp3 = a.wp3;
+}
+
+/* Ind_plus / Ind_minus (processes.f90:361-521): list in j-outer order and
+   per-process Gamma (THz). */
+int64_t orc3_ind_mode(const orc_case *c, int ch, int mm, int64_t cap, int32_t *i2, int32_t *i3, double *gamma,
+                      double *wp3) {
+  rta3_acc a;
+  memset(&a, 0, sizeof(a));
+  a.i = fmodulo(mm - 1, c->nbands) + 1;
+  a.qi = c->List[(mm - 1) / c->nbands];
+  a.ch = ch;
+  a.nb = c->nbands;
+  a.store = 1;
+  a.cap = cap;
+  a.i2 = i2; a.i3 = i3; a.g = gamma;
+  const double omega = E(c, a.qi, a.i);
+  enumerate3(c, ch, mm, 1, 1, c->nptk, visit_rta3, &a);
+  if (omega != 0.0) a.wp3 = (ch == 0) ? a.wp3 / c->nptk : a.wp3 * 5.e-1 / c->nptk;
+  *wp3 = a.wp3;
+  return a.n;
+}
+
+/* NP_driver, processes.f90:983-1067 */
+void orc3_np_driver(const orc_case *c, int nthreads, int32_t *N_plus, double *P_plus, int32_t *N_minus,
+                    double *P_minus) {
+  const int nm = c->nbands * c->nlist;
+  set_threads(nthreads);
+  for (int m = 0; m < nm; m++) {
+    N_plus[m] = N_minus[m] = 0;
+    P_plus[m] = P_minus[m] = 0.0;
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (E(c, c->List[ll - 1], i) <= c->omega_max) {
+      int64_t n;
+      orc3_np_mode(c, 0, mm, &n, &P_plus[mm - 1]);
+      N_plus[mm - 1] = (int32_t)n;
+      orc3_np_mode(c, 1, mm, &n, &P_minus[mm - 1]);
+      N_minus[mm - 1] = (int32_t)n;
+    }
+  }
+}
+
+/* Ind_only_driver, processes.f90:1198-1309.  Returns 0, or -1 when a list is
+   longer than its count. */
+int orc3_ind_only_driver(const orc_case *c, int nthreads, const int32_t *N_plus, const int32_t *N_minus,
+                         int32_t *i2_plus, int32_t *i3_plus, int32_t *i2_minus, int32_t *i3_minus) {
+  const int nm = c->nbands * c->nlist;
+  const int32_t *Ns[2] = {N_plus, N_minus};
+  int32_t *out[2][2] = {{i2_plus, i3_plus}, {i2_minus, i3_minus}};
+  int64_t *acc[2];
+  int bad = 0;
+  for (int ch = 0; ch < 2; ch++) {
+    acc[ch] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nm + 1));
+    acc[ch][0] = 0;
+    for (int m = 0; m < nm; m++) acc[ch][m + 1] = acc[ch][m] + Ns[ch][m];
+    for (int64_t t = 0; t < acc[ch][nm]; t++) out[ch][0][t] = out[ch][1][t] = 0;
+  }
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    for (int ch = 0; ch < 2; ch++) {
+      const int64_t n = Ns[ch][mm - 1];
+      if (n == 0) continue;
+      const int64_t off = acc[ch][mm - 1];
+      if (orc3_ind_only_mode(c, ch, mm, n, out[ch][0] + off, out[ch][1] + off) != n) bad |= 1;
+    }
+  }
+  for (int ch = 0; ch < 2; ch++) free(acc[ch]);
+  return bad ? -1 : 0;
+}
+
+/* RTA_driver, processes.f90:1676-1775, restricted to a list of modes and a
+   q' window; outputs are (Nb,Nlist) arrays that the caller zeroes. */
+void orc3_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi,
+                     double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus) {
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int t = 0; t < nmodes; t++) {
+    const int mm = modes[t];
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    double g, w;
+    orc3_rta_mode(c, 0, mm, ii_lo, ii_hi, &g, &w);
+    rate_plus[mm - 1] = g; wp3_plus[mm - 1] = w;
+    orc3_rta_mode(c, 1, mm, ii_lo, ii_hi, &g, &w);
+    rate_minus[mm - 1] = g * 5.e-1; wp3_minus[mm - 1] = w;
+  }
+  for (int t = 0; t < nmodes; t++) {
+    const int m = modes[t] - 1;
+    rate[m] = rate_plus[m] + rate_minus[m];
+  }
+}
+
+/* Ind_driver, processes.f90:524-707 */
+int orc3_ind_driver(const orc_case *c, int nthreads, const int32_t *N_plus, const int32_t *N_minus, int32_t *i2_plus,
+                    int32_t *i3_plus, double *g_plus, int32_t *i2_minus, int32_t *i3_minus, double *g_minus,
+                    double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus) {
+  const int nm = c->nbands * c->nlist;
+  const int32_t *Ns[2] = {N_plus, N_minus};
+  int32_t *out[2][2] = {{i2_plus, i3_plus}, {i2_minus, i3_minus}};
+  double *gs[2] = {g_plus, g_minus};
+  double *rates[2] = {rate_plus, rate_minus};
+  double *wps[2] = {wp3_plus, wp3_minus};
+  int64_t *acc[2];
+  int bad = 0;
+  for (int ch = 0; ch < 2; ch++) {
+    acc[ch] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nm + 1));
+    acc[ch][0] = 0;
+    for (int m = 0; m < nm; m++) acc[ch][m + 1] = acc[ch][m] + Ns[ch][m];
+    for (int m = 0; m < nm; m++) rates[ch][m] = wps[ch][m] = 0.0;
+  }
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int mm = 1; mm <= nm; mm++) {
+    for (int ch = 0; ch < 2; ch++) {
+      const int64_t n = Ns[ch][mm - 1];
+      if (n == 0) continue;
+      const int64_t off = acc[ch][mm - 1];
+      double w;
+      if (orc3_ind_mode(c, ch, mm, n, out[ch][0] + off, out[ch][1] + off, gs[ch] + off, &w) != n) bad |= 1;
+      wps[ch][mm - 1] = w;
+      double s = 0.0;
+      for (int64_t t = 0; t < n; t++) s = s + gs[ch][off + t];
+      rates[ch][mm - 1] = (ch == 0) ? s : s * 5.e-1;
+    }
+  }
+  for (int m = 0; m < nm; m++) rate[m] = rate_plus[m] + rate_minus[m];
+  for (int ch = 0; ch < 2; ch++) free(acc[ch]);
+  return bad ? -1 : 0;
+}

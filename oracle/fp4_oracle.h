@@ -1,5 +1,6 @@
 /*
- * CPU restatement of FourPhonon's four-phonon hot path -- TEST INFRASTRUCTURE.
+ * CPU restatement of FourPhonon's four-phonon hot path and its three-phonon
+ * companion -- TEST INFRASTRUCTURE.
  *
  * This library is the checker for the HIP implementation.  Only the test
  * suite, the smoke check and the benchmark's CPU-baseline leg may load it.
@@ -43,6 +44,11 @@ typedef struct orc_case {
   const double *Phi;           /* Phi(3,3,3,3,Ntri) eV/A^4 */
   const double *R_j, *R_k, *R_l; /* (3,Ntri) nm */
   const int32_t *Index_i, *Index_j, *Index_k, *Index_l;
+  /* third-order clusters (FORCE_CONSTANTS_3RD as read by read3fc) */
+  int32_t ntri3;
+  const double *Phi3;          /* Phi(3,3,3,Ntri) eV/A^3 */
+  const double *R3_j, *R3_k;   /* (3,Ntri) nm */
+  const int32_t *Index3_i, *Index3_j, *Index3_k;
 } orc_case;
 
 double orc_base_sigma(const orc_case *c, const double v[3]);
@@ -108,6 +114,26 @@ void orc_tconduct(int nptk, int nbands, double T, double V, const double *omega,
                   const double *velocity, const double *F_n, double *kappa);
 void orc_symmetrize_F(int nptk, int nbands, const double *symmetrizers, double *F_n);
 void orc_symmetrize_tensor(int nsymm_rot, const double *crotations, double *tensor);
+
+/* Three-phonon path.  Channels: 0 = "+" (absorption), 1 = "-" (emission).
+   Counts and lists are int32 as in the reference. */
+void orc3_np_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P);
+int64_t orc3_ind_only_mode(const orc_case *c, int ch, int mm, int64_t cap, int32_t *i2, int32_t *i3);
+void orc3_vp(const orc_case *c, int ch, int i, int j, int k, int q, int qp, int qdp, const double rqp[3],
+             const double rqdp[3], double out[2]);
+void orc3_vp_idx(const orc_case *c, int ch, int i, int j, int k, int q, int qp, int qdp, double out[2]);
+void orc3_rta_mode(const orc_case *c, int ch, int mm, int32_t ii_lo, int32_t ii_hi, double *gamma, double *wp3);
+int64_t orc3_ind_mode(const orc_case *c, int ch, int mm, int64_t cap, int32_t *i2, int32_t *i3, double *gamma,
+                      double *wp3);
+void orc3_np_driver(const orc_case *c, int nthreads, int32_t *N_plus, double *P_plus, int32_t *N_minus,
+                    double *P_minus);
+int orc3_ind_only_driver(const orc_case *c, int nthreads, const int32_t *N_plus, const int32_t *N_minus,
+                         int32_t *i2_plus, int32_t *i3_plus, int32_t *i2_minus, int32_t *i3_minus);
+void orc3_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi,
+                     double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus);
+int orc3_ind_driver(const orc_case *c, int nthreads, const int32_t *N_plus, const int32_t *N_minus, int32_t *i2_plus,
+                    int32_t *i3_plus, double *g_plus, int32_t *i2_minus, int32_t *i3_minus, double *g_minus,
+                    double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus);
 
 #ifdef __cplusplus
 }

@@ -3,8 +3,8 @@
 Only the tests, ``__graft_entry__.smoke()`` and the benchmark's CPU-baseline
 leg import this module.  It wraps ``libfp4oracle.so`` (built from
 ``fp4_oracle.c`` by ``oracle/Makefile``), the FP64 CPU restatement of the
-reference's four-phonon path; see ``fp4_oracle.c`` for the reference lines
-each routine follows.
+reference's four-phonon path and its three-phonon companion; see
+``fp4_oracle.c`` for the reference lines each routine follows.
 """
 from __future__ import annotations
 
@@ -29,6 +29,8 @@ class OrcCase(C.Structure):
         ("eigenvect", C.c_void_p), ("IJK", C.c_void_p), ("List", C.c_void_p), ("ntri", C.c_int32),
         ("Phi", C.c_void_p), ("R_j", C.c_void_p), ("R_k", C.c_void_p), ("R_l", C.c_void_p),
         ("Index_i", C.c_void_p), ("Index_j", C.c_void_p), ("Index_k", C.c_void_p), ("Index_l", C.c_void_p),
+        ("ntri3", C.c_int32), ("Phi3", C.c_void_p), ("R3_j", C.c_void_p), ("R3_k", C.c_void_p),
+        ("Index3_i", C.c_void_p), ("Index3_j", C.c_void_p), ("Index3_k", C.c_void_p),
     ]
 
 
@@ -75,6 +77,14 @@ class Oracle:
             Index_i=np.ascontiguousarray(f.Index_i, np.int32), Index_j=np.ascontiguousarray(f.Index_j, np.int32),
             Index_k=np.ascontiguousarray(f.Index_k, np.int32), Index_l=np.ascontiguousarray(f.Index_l, np.int32),
         )
+        f3 = getattr(case, "ifc3", None)
+        if f3 is not None:
+            keep.update(
+                Phi3=np.asfortranarray(f3.Phi, np.float64),
+                R3_j=np.asfortranarray(f3.R_j, np.float64), R3_k=np.asfortranarray(f3.R_k, np.float64),
+                Index3_i=np.ascontiguousarray(f3.Index_i, np.int32),
+                Index3_j=np.ascontiguousarray(f3.Index_j, np.int32),
+                Index3_k=np.ascontiguousarray(f3.Index_k, np.int32))
         self._keep = keep
         c = OrcCase()
         c.ngrid[:] = [int(x) for x in m.ngrid]
@@ -85,6 +95,7 @@ class Oracle:
         for k, v in keep.items():
             setattr(c, k, _ptr(v))
         c.ntri = f.Ntri
+        c.ntri3 = f3.Ntri if f3 is not None else 0
         self.c = c
         self.L = lib()
 
@@ -167,6 +178,76 @@ class Oracle:
         names = ["rate_4ph", "rate_pp", "rate_pm", "rate_mm", "wp4_pp", "wp4_pm", "wp4_mm"]
         return ([[o[:t] for o in ch] for ch, t in zip(lists, tot)], [g[:t] for g, t in zip(gam, tot)],
                 dict(zip(names, rates)))
+
+
+    # ------------------------------------------------------- three phonons
+    def _need3(self):
+        if self.c.ntri3 <= 0:
+            raise ValueError("case has no third-order constants")
+
+    def np3_mode(self, ch, mm):
+        n = C.c_int64()
+        p = C.c_double()
+        self.L.orc3_np_mode(C.byref(self.c), C.c_int(ch), C.c_int(mm), C.byref(n), C.byref(p))
+        return n.value, p.value
+
+    def ind_only3_mode(self, ch, mm, cap=None):
+        if cap is None:
+            cap = self.np3_mode(ch, mm)[0]
+        out = [np.zeros(max(cap, 1), np.int32) for _ in range(2)]
+        self.L.orc3_ind_only_mode.restype = C.c_int64
+        n = self.L.orc3_ind_only_mode(C.byref(self.c), C.c_int(ch), C.c_int(mm), C.c_int64(cap),
+                                      *[_ptr(o) for o in out])
+        return [o[:n] for o in out]
+
+    def vp3(self, ch, i, j, k, q, qp, qdp):
+        self._need3()
+        out = np.zeros(2)
+        self.L.orc3_vp_idx(C.byref(self.c), C.c_int(ch), *[C.c_int(int(x)) for x in (i, j, k, q, qp, qdp)],
+                           _ptr(out))
+        return complex(out[0], out[1])
+
+    def np3_driver(self, nthreads=0):
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int32) for _ in range(2)]
+        P = [np.zeros(nm, np.float64) for _ in range(2)]
+        self.L.orc3_np_driver(C.byref(self.c), C.c_int(nthreads), _ptr(N[0]), _ptr(P[0]), _ptr(N[1]), _ptr(P[1]))
+        return N, P
+
+    def ind_only3_driver(self, N, nthreads=0):
+        N = [np.ascontiguousarray(n, np.int32) for n in N]
+        outs = [[np.zeros(max(int(n.sum()), 1), np.int32) for _ in range(2)] for n in N]
+        rc = self.L.orc3_ind_only_driver(C.byref(self.c), C.c_int(nthreads), *[_ptr(n) for n in N],
+                                         *[_ptr(o) for ch in outs for o in ch])
+        if rc != 0:
+            raise RuntimeError("oracle list longer than its count")
+        return [[o[:int(n.sum())] for o in ch] for ch, n in zip(outs, N)]
+
+    def rta3_driver(self, modes=None, ii_lo=1, ii_hi=None, nthreads=0):
+        self._need3()
+        nm = self.nmodes
+        modes = np.arange(1, nm + 1, dtype=np.int32) if modes is None else np.ascontiguousarray(modes, np.int32)
+        out = [np.zeros(nm, np.float64) for _ in range(5)]
+        self.L.orc3_rta_driver(C.byref(self.c), C.c_int(nthreads), C.c_int(len(modes)), _ptr(modes),
+                               C.c_int32(ii_lo), C.c_int32(self.nptk if ii_hi is None else ii_hi),
+                               *[_ptr(o) for o in out])
+        return dict(zip(["rate", "rate_plus", "rate_minus", "wp3_plus", "wp3_minus"], out))
+
+    def ind3_driver(self, N, nthreads=0):
+        self._need3()
+        nm = self.nmodes
+        N = [np.ascontiguousarray(n, np.int32) for n in N]
+        tot = [int(n.sum()) for n in N]
+        lists = [[np.zeros(max(t, 1), np.int32) for _ in range(2)] for t in tot]
+        gam = [np.zeros(max(t, 1), np.float64) for t in tot]
+        rates = [np.zeros(nm, np.float64) for _ in range(5)]
+        rc = self.L.orc3_ind_driver(C.byref(self.c), C.c_int(nthreads), _ptr(N[0]), _ptr(N[1]),
+                                    _ptr(lists[0][0]), _ptr(lists[0][1]), _ptr(gam[0]),
+                                    _ptr(lists[1][0]), _ptr(lists[1][1]), _ptr(gam[1]), *[_ptr(r) for r in rates])
+        if rc != 0:
+            raise RuntimeError("oracle list longer than its count")
+        return ([[o[:t] for o in ch] for ch, t in zip(lists, tot)], [g[:t] for g, t in zip(gam, tot)],
+                dict(zip(["rate", "rate_plus", "rate_minus", "wp3_plus", "wp3_minus"], rates)))
 
 
 def iteration0(case, tau_zero):

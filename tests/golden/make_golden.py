@@ -2,7 +2,7 @@
 
 Run in the development container (the oracle never runs on the GPU box):
 
-    python tests/golden/make_golden.py cfg1 cfg2 cfg3
+    python tests/golden/make_golden.py cfg1 cfg2 cfg3 cfg4
 
 Each part writes tests/golden/<part>.npz holding inputs-by-seed (the crystal
 is regenerated from its seed by fourphonon_amd.crystal) and the oracle's
@@ -105,6 +105,69 @@ def part_cfg3():
     np.savez_compressed(os.path.join(OUT, "cfg3.npz"), ngrid=np.array([12, 12, 8]), shell=2, workload=3,
                         modes=modes, N=N, P=P, H=H, rate_modes=rmodes, ii_lo=lo, ii_hi=hi,
                         **{k: v[rmodes - 1] for k, v in r.items()})
+
+
+CFG4_HASH_CAP = 30_000_000     # lists longer than this per mode are counted, not hashed
+
+
+def part_cfg4():
+    """The 24-crystal batch at its drawn meshes: three modes per crystal for
+    counts, phase space and list hashes, one mode per crystal for rates over a
+    one-point q' window (ii = 2)."""
+    from concurrent.futures import ThreadPoolExecutor
+    cases = []
+    for c in range(crystal.BATCH_SIZE):
+        cases.append(crystal.batch_case(c))
+        m = cases[-1].modes
+        log(f"cfg4: crystal {c} {cases[-1].structure.name} mesh {list(m.ngrid)} Nb {m.nbands} Nlist {m.Nlist}")
+    oracles = [Oracle(cs) for cs in cases]
+    modes = []
+    for o, cs in zip(oracles, cases):
+        nb, nm = cs.nbands, o.nmodes
+        modes.append(np.array([nb + 4, nm // 2 + 1, nm], np.int32))
+    rmode = np.array([m[1] for m in modes], np.int32)
+
+    def counts_job(arg):
+        c, t = arg
+        o, mm = oracles[c], int(modes[c][t])
+        res = []
+        for ch in range(3):
+            n, p = o.np_mode(ch, mm)
+            h = list_hash(*o.ind_only_mode(ch, mm, cap=n)) if n <= CFG4_HASH_CAP else 0
+            res.append((n, p, h, n <= CFG4_HASH_CAP))
+        return c, t, res
+
+    def rate_job(c):
+        r = oracles[c].rta_driver(modes=rmode[c:c + 1], ii_lo=2, ii_hi=2, nthreads=1)
+        return c, {k: v[rmode[c] - 1] for k, v in r.items()}
+
+    nc = crystal.BATCH_SIZE
+    N = np.zeros((nc, 3, 3), np.int64)
+    P = np.zeros((nc, 3, 3))
+    H = np.zeros((nc, 3, 3), np.uint64)
+    hashed = np.zeros((nc, 3, 3), bool)
+    t0 = time.time()
+    jobs = [(c, t) for c in range(nc) for t in range(3)]
+    jobs.sort(key=lambda ct: -cases[ct[0]].nptk ** 2 * cases[ct[0]].nbands ** 3)
+    with ThreadPoolExecutor(THREADS) as ex:
+        for c, t, res in ex.map(counts_job, jobs):
+            for ch, (n, p, h, ok) in enumerate(res):
+                N[c, ch, t], P[c, ch, t], H[c, ch, t], hashed[c, ch, t] = n, p, h, ok
+            log(f"cfg4: counts crystal {c} mode {modes[c][t]} N {[r[0] for r in res]}")
+    log("cfg4: counts took", time.time() - t0)
+    rates = {}
+    t0 = time.time()
+    with ThreadPoolExecutor(THREADS) as ex:
+        for c, r in ex.map(rate_job, range(nc)):
+            for k, v in r.items():
+                rates.setdefault(k, np.zeros(nc))[c] = v
+            log(f"cfg4: rate crystal {c} mode {rmode[c]}: {r['rate_4ph']:.6e}")
+    log("cfg4: rates took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "cfg4.npz"), workload=4,
+                        ngrid=np.array([list(cs.modes.ngrid) for cs in cases]),
+                        nbands=np.array([cs.nbands for cs in cases]), shell=1,
+                        modes=np.array(modes), N=N, P=P, H=H, hashed=hashed,
+                        rate_modes=rmode, ii_lo=2, ii_hi=2, **rates)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,114 @@
+"""The 24-crystal batch (fcc / zincblende / rocksalt-like cells of 2, 4, 6
+and 8 atoms, Nb = 6 ... 24, meshes 6^3 ... 14^3) against the oracle.
+
+Bars: counts and lists bit-exact; FP64 rates within 1e-10 relative; the
+dense FP32 and BF16 hi/lo forms (cells of up to 4 atoms) within 1e-3
+relative per mode of the FP64 form.  Cells of 6 and 8 atoms run the FP64
+form only: the dense kernel takes at most 4 atoms per cell.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fourphonon_amd import crystal
+from fourphonon_amd import runtime as rt
+from tests.golden.listhash import list_hash
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+RTOL = 1e-10
+
+
+def banner(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+
+
+def assert_close(got, want, rtol=RTOL, what=""):
+    got, want = np.asarray(got, float).ravel(), np.asarray(want, float).ravel()
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want)
+    bad = err > rtol * np.maximum(np.abs(want), 1e-3 * scale)
+    assert not bad.any(), f"{what}: max rel err {np.max(err / np.maximum(np.abs(want), 1e-300)):.3e}"
+
+
+RATE_KEYS = (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
+             ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusplus", "wp4_pp"), ("WP4_plusminus", "wp4_pm"),
+             ("WP4_minusminus", "wp4_mm"))
+
+# one crystal per cell size, every kind at least once; Nb 24 runs a 3^3 mesh
+LIVE = [(4, (4, 4, 4)), (1, (4, 4, 4)), (10, (3, 3, 3)), (7, (3, 3, 3))]
+
+
+@pytest.mark.parametrize("c,mesh", LIVE, ids=[f"crystal{c}" for c, _ in LIVE])
+def test_batch_crystal_small_mesh(c, mesh):
+    """All modes: counts, phase space and lists bit-exact; FP64 rates of a mode subset."""
+    from oracle.oracle import Oracle
+    case = crystal.batch_case(c, ngrid=mesh)
+    banner(f"batch crystal {c} {case.structure.name} Nb {case.nbands} mesh {mesh}")
+    ctx = rt.Context(case)
+    o = Oracle(case)
+    try:
+        N = ctx.NP_driver_4ph()
+        No, Po = o.np_driver(nthreads=8)
+        for ch in range(3):
+            assert np.array_equal(N[2 * ch], No[ch]), f"counts channel {ch}"
+            assert_close(N[2 * ch + 1].ravel(order="F"), Po[ch], 1e-12, f"P channel {ch}")
+        assert sum(int(No[ch].sum()) for ch in range(3)) > 0
+        got = ctx.Ind_only_driver_4ph(*No)
+        want = o.ind_only_driver(No, nthreads=8)
+        for ch in range(3):
+            for t in range(3):
+                assert np.array_equal(got[ch][t], want[ch][t]), (ch, t)
+        modes = np.arange(1, ctx.nmodes + 1)[case.nbands // 2::max(1, ctx.nmodes // 12)]
+        r, _ = ctx.RTA_driver_4ph_modes(modes, rt.PREC_FP64, ii_lo=2, ii_hi=3)
+        w = o.rta_driver(modes=modes, ii_lo=2, ii_hi=3, nthreads=8)
+        for g, k in RATE_KEYS:
+            assert_close(r[g].ravel(order="F")[modes - 1], w[k][modes - 1], RTOL, g)
+        assert np.abs(w["rate_4ph"][modes - 1]).max() > 0
+    finally:
+        ctx.close()
+
+
+def _golden():
+    path = os.path.join(GOLDEN, "cfg4.npz")
+    if not os.path.exists(path):
+        pytest.skip("cfg4.npz not generated")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("c", range(crystal.BATCH_SIZE))
+def test_batch_golden(c):
+    """Each crystal at its own mesh: counts, phase space and list hashes of three
+    modes, FP64 rates of one mode over one q' point, and the dense forms there."""
+    g = _golden()
+    case = crystal.batch_case(c)
+    assert list(case.modes.ngrid) == list(g["ngrid"][c]) and case.nbands == int(g["nbands"][c])
+    banner(f"golden batch crystal {c} {case.structure.name} Nb {case.nbands} mesh {list(case.modes.ngrid)}")
+    ctx = rt.Context(case)
+    try:
+        modes = g["modes"][c]
+        N = ctx.NP_driver_4ph_modes(modes)
+        for ch in range(3):
+            assert np.array_equal(N[2 * ch], g["N"][c, ch]), f"counts channel {ch}"
+            assert_close(N[2 * ch + 1], g["P"][c, ch], RTOL, f"P channel {ch}")
+        for t, mm in enumerate(modes):
+            if not g["hashed"][c, :, t].all():
+                continue
+            Np, Npm, Nmm, lists = ctx.Ind_only_driver_4ph_modes([mm], cap=int(g["N"][c, :, t].max()))
+            for ch in range(3):
+                assert list_hash(*lists[ch]) == int(g["H"][c, ch, t]), (ch, int(mm))
+        rm = int(g["rate_modes"][c])
+        lo, hi = int(g["ii_lo"]), int(g["ii_hi"])
+        r, n64 = ctx.RTA_driver_4ph_modes([rm], rt.PREC_FP64, ii_lo=lo, ii_hi=hi)
+        for key, wk in RATE_KEYS:
+            assert_close(r[key].ravel(order="F")[rm - 1], g[wk][c], RTOL, key)
+        if case.structure.natoms <= 4:
+            want = g["rate_4ph"][c]
+            for prec in (rt.PREC_FP32, rt.PREC_BF16X3):
+                rd, nd = ctx.RTA_driver_4ph_modes([rm], prec, ii_lo=lo, ii_hi=hi)
+                got = rd["rate_scatt_4ph"].ravel(order="F")[rm - 1]
+                assert nd == n64
+                assert abs(got - want) <= 1e-3 * abs(want), (prec, got, want)
+    finally:
+        ctx.close()
