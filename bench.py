@@ -257,6 +257,25 @@ def main():
             "kappa_rel_err": float(abs(np.trace(kappa_rta(case, ao)) - np.trace(k64)) / abs(np.trace(k64)))}
         log(f"{other} form {no / to:.3e} proc/s")
 
+    # ---- three-phonon companion on the same crystal (2nd-shell FC3): counts
+    # and FP64 per-process rates, all modes; reported beside the headline
+    if case.ifc3 is not None:
+        ctx.reset_stats()
+        t1 = time.perf_counter()
+        Np3, _, Nm3, _ = ctx.NP_driver()
+        t_np3 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        ctx.RTA_driver()
+        t_r3 = time.perf_counter() - t1
+        ms_r3, _ = ctx.kernel_stats(rt.Context.KS_RATE3)
+        n3 = int(Np3.sum() + Nm3.sum())
+        f3 = 442.0 * case.ifc3.Ntri * n3
+        out["three_phonon"] = {"processes": n3, "ntri3": int(case.ifc3.Ntri), "np_seconds": t_np3,
+                               "rta_seconds": t_r3, "value": n3 / t_r3, "unit": "processes/s",
+                               "rate_kernel_ms": ms_r3,
+                               "rate_kernel_frac_fp64_peak": f3 / (ms_r3 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        log(f"3ph: {n3} processes, counts {t_np3:.2f}s, FP64 rates {t_r3:.2f}s")
+
     # ---- CPU baseline: the oracle on the host cores, bounded sample
     if not args.no_cpu:
         try:

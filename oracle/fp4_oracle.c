@@ -1,5 +1,6 @@
 /*
- * CPU restatement of FourPhonon's four-phonon hot path -- TEST INFRASTRUCTURE.
+ * CPU restatement of FourPhonon's four-phonon hot path and its three-phonon
+ * companion -- TEST INFRASTRUCTURE.
  *
  * Only the tests, the smoke check and the benchmark's CPU-baseline leg use
  * this file; it is the checker, never the thing measured on the GPU or
@@ -21,19 +22,29 @@
  *   TConduct             conductivity.f90:11-37
  *   symmetrisers on F_n  ShengBTE.f90:1288-1290
  *   symmetrize_tensor    config.f90:503-518
+ *   3ph Vp_plus/minus    processes.f90:25-139
+ *   3ph NP_plus/minus    processes.f90:712-823, NP_driver 983-1067
+ *   3ph Ind_only_*       processes.f90:1074-1192, Ind_only_driver 1198-1309
+ *   3ph RTA_plus/minus   processes.f90:1313-1471, RTA_driver 1676-1775
+ *   3ph Ind_plus/minus   processes.f90:361-521, Ind_driver 524-707
  *
  * Float32-promoted literals of the reference (unsuffixed reals are default
- * REAL): 0.001 (sigma clamp), 3.37617087 (4ph rate units), 1e22 (hbarp),
- * 1e21 (kappa prefactor).  1.25, 0.5, 2. and 8.d0 are exact.
+ * REAL): 0.001 (sigma clamp), 3.37617087 (4ph rate units), 5.60626442 (3ph
+ * rate units), 1e22 (hbarp), 1e21 (kappa prefactor).  1.25, 0.5, 2. and 8.d0
+ * are exact.
  *
  * Pinning.  The reference cannot be compiled in this image, so every
  * routine here is checked by an independent route (brute-force enumeration,
- * closed forms, invariants, dense linear algebra; tests/test_oracle*.py).
+ * closed forms, invariants, dense linear algebra; tests/test_oracle*.py),
+ * and the count / phase-space drivers of both orders also against the
+ * reference's own Si example outputs (statistical agreement, ten printed
+ * digits; tests/test_reference_example.py).
  * Parity unpinned: which candidates exactly at the window edge the
  * reference's compiled code keeps (this file follows its expression order and
- * rounding, but only a run of the reference could confirm ties), and the
- * reference's example count files, which need the harmonic front end to
- * reproduce.
+ * rounding, but only a run of the reference could confirm ties), a bitwise
+ * match of the example count files (their omega and v are printed to ten
+ * digits), and the example rate files, which need the harmonic front end
+ * (eigenvectors) and, for four phonons, constants the examples do not ship.
  */
 #include "fp4_oracle.h"
 

@@ -121,3 +121,49 @@ def test_bad_list_entry_is_rejected(ctx):
     L4[1][2][0] = case.modes.nptk * case.modes.nbands + 1
     with pytest.raises(rt.FP4Error):
         c.bte_set_operator(N4, L4, G4)
+
+
+@pytest.mark.parametrize("mesh", [(4, 4, 4)], ids=["4cube"])
+def test_kappa_3ph_plus_4ph_end_to_end(mesh):
+    """The diamond workload with both scattering orders, every stage on the
+    device: 3ph and 4ph counts, Ind_driver lists and per-process Gamma, the
+    total rate (3ph + 4ph, Src/ShengBTE.f90:1220) and tau0, the RTA kappa and
+    the iterative solution with the joint operator.  The oracle runs the same
+    pipeline from its own lists and Gamma."""
+    from oracle.oracle import Oracle, bte_solve, iteration0, tconduct
+    case = crystal.config_case(1, ngrid=mesh)
+    banner(f"3ph+4ph kappa diamond {mesh}")
+    c = rt.Context(case)
+    o = Oracle(case)
+    try:
+        n3 = c.NP_driver()
+        N3 = [n3[0], n3[2]]
+        L3, G3, R3 = c.Ind_driver(*N3)
+        n4 = c.NP_driver_4ph()
+        N4 = [n4[0], n4[2], n4[4]]
+        L4, G4, R4 = c.Ind_driver_4ph(*N4)
+        rate = R3["rate_scatt"] + R4["rate_scatt_4ph"]
+        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+
+        No3, _ = o.np3_driver(nthreads=8)
+        wl3, wg3, wr3 = o.ind3_driver(No3, nthreads=8)
+        No4, _ = o.np_driver(nthreads=8)
+        wl4, wg4, wr4 = o.ind_driver(No4, nthreads=8)
+        wrate = (wr3["rate"] + wr4["rate_4ph"]).reshape(rate.shape, order="F")
+        assert rel(rate, wrate) <= 1e-10
+        wtau = np.asfortranarray(np.where(wrate > 0, 1.0 / np.where(wrate > 0, wrate, 1.0), 0.0))
+
+        k_rta = c.TConduct(c.iteration0(tau)).sum(axis=0)
+        w_rta = tconduct(case, iteration0(case, wtau)).sum(axis=0)
+        assert rel(k_rta, w_rta) <= 1e-10
+
+        c.bte_set_operator(N4, L4, G4, N3, L3, G3)
+        got = c.bte_solve(tau, maxiter=60)
+        want = bte_solve(case, No4, wl4, wg4, wtau, maxiter=60, N3=No3, lists3=wl3, gammas3=wg3)
+        print(f"niter {got['niter']} (oracle {want['niter']}), RTA kappa_xx {k_rta[0, 0]:.6e}, "
+              f"converged kappa_xx {got['kappa_iter'][0, 0, -1]:.6e}")
+        assert got["niter"] == want["niter"] and got["niter"] > 1
+        assert rel(got["kappa_iter"], want["kappa_iter"]) <= 1e-9
+        assert rel(got["F_n"], want["F_n"]) <= 1e-9
+    finally:
+        c.close()
