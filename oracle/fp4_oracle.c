@@ -36,15 +36,14 @@
  * Pinning.  The reference cannot be compiled in this image, so every
  * routine here is checked by an independent route (brute-force enumeration,
  * closed forms, invariants, dense linear algebra; tests/test_oracle*.py),
- * and the count / phase-space drivers of both orders also against the
- * reference's own Si example outputs (statistical agreement, ten printed
+ * and the three-phonon drivers also against the reference's own Si example
+ * run (counts equal to its totals, phase space and rates to its ten printed
  * digits; tests/test_reference_example.py).
  * Parity unpinned: which candidates exactly at the window edge the
  * reference's compiled code keeps (this file follows its expression order and
- * rounding, but only a run of the reference could confirm ties), a bitwise
- * match of the example count files (their omega and v are printed to ten
- * digits), and the example rate files, which need the harmonic front end
- * (eigenvectors) and, for four phonons, constants the examples do not ship.
+ * rounding, but only a run of the reference could confirm ties), and the
+ * four-phonon matrix elements and rates against a reference run (the
+ * examples ship no fourth-order constants).
  */
 #include "fp4_oracle.h"
 

@@ -1,70 +1,87 @@
-"""The device count, phase-space and weighted-phase-space drivers (three and
-four phonons) against the reference's own Si example run at 16x16x16
-(tests/golden/si_example.npz).  Agreement is statistical for the reasons
-given in tests/test_reference_example.py: channel totals within 0.5 %,
-per-mode medians within 1 % (counts, phase space) and a few percent
-(weighted phase space, printed with five digits)."""
+"""The device drivers against the reference's own Si example run at
+16x16x16 (tests/golden/si_example.npz): 4ph counts per mode
+(BTE.Numprocess_4ph) and phase space (BTE.P4_*), 3ph counts, phase space and
+RTA rates (BTE.P3_*, T300K/BTE.w_3ph*), and the weighted phase space
+(T300K/BTE.WP3_*, BTE.WP4_*), with omega, v and e from the harmonic front
+end.  Bars are the printing precision of the files (ten digits; WP five),
+as in tests/test_reference_example.py."""
 import numpy as np
 import pytest
 
 from fourphonon_amd import runtime as rt
-from tests.si_example import load, si_case
+from tests.si_example import degenerate_sets, load, si_case
 
 pytestmark = pytest.mark.gpu
 
 
-def _rel(a, b):
-    return abs(float(a) - float(b)) / abs(float(b))
-
-
-def _median_rel(got, want):
+def _rel(got, want):
+    got, want = np.atleast_1d(np.asarray(got, float)), np.atleast_1d(np.asarray(want, float))
     nz = want != 0
-    return float(np.median(np.abs(got[nz] - want[nz]) / np.abs(want[nz])))
+    if not nz.any():
+        return float(np.abs(got).max()) if got.size else 0.0
+    return float(np.max(np.abs(got[nz] - want[nz]) / np.abs(want[nz])))
 
 
 @pytest.fixture(scope="module")
 def si():
     g = load()
+    case = si_case(g)
     print("\n[gpu] Si example 16x16x16 context", flush=True)
-    c = rt.Context(si_case(g))
-    yield g, c
+    c = rt.Context(case)
+    yield g, case, c
     c.close()
 
 
 def test_four_phonon_counts_and_phase_space(si):
-    g, c = si
+    """Counts per mode equal the reference's except for candidates that sit
+    within rounding of the window edge: our omega comes from a different
+    Hermitian solver than the reference's LAPACK, and agrees with it to a few
+    ulps only, so a handful of the ~1e10 candidates per channel flip
+    (measured: none in ++, 27 processes over 29 of 870 modes in +-)."""
+    g, case, c = si
     N = c.NP_driver_4ph()
+    worst = []
     for ch in range(3):
-        got, want = N[2 * ch], g["N4"][ch]
-        P, Pw = N[2 * ch + 1].ravel(order="F"), g["P4"][ch].ravel(order="F")
-        print(f"4ph channel {ch}: counts {int(got.sum())} vs {int(want.sum())}, "
-              f"median rel {_median_rel(got, want):.2e}; P4 total rel {_rel(P.sum(), Pw.sum()):.2e}")
-        assert _rel(got.sum(), want.sum()) <= 5e-3
-        assert _rel(P.sum(), Pw.sum()) <= 5e-3
-        assert _median_rel(got, want) <= 1e-2
-        assert _median_rel(P, Pw) <= 1e-2
+        got, want = N[2 * ch].astype(np.int64), g["N4"][ch]
+        d = np.abs(got - want)
+        print(f"4ph channel {ch}: {int(got.sum())} vs {int(want.sum())}, {int((d > 0).sum())} modes differ, "
+              f"max {int(d.max())}; P4 max rel {_rel(N[2 * ch + 1].ravel(order='F'), g['P4'][ch].ravel(order='F')):.2e}")
+        worst.append((int(d.max()), int((d > 0).sum()), abs(int(got.sum()) - int(want.sum())) / int(want.sum())))
+        # phase space: the printing precision where no candidate flips (++:
+        # 5e-10); flips, also ones that cancel in the count, move a mode's
+        # P4 by up to 8e-7 in +-
+        assert _rel(N[2 * ch + 1].ravel(order="F"), g["P4"][ch].ravel(order="F")) <= 2e-6, ch
+    for ch, (dmax, nmodes, tot) in enumerate(worst):
+        assert dmax <= 16 and nmodes <= 60 and tot <= 1e-9, (ch, dmax, nmodes, tot)
 
 
-def test_three_phonon_counts_and_phase_space(si):
-    g, c = si
+def test_three_phonon_counts_phase_space_and_rates(si):
+    g, case, c = si
     N = c.NP_driver()
     for ch in range(2):
-        assert _rel(N[2 * ch].sum(), g["Ntotal3"][ch]) <= 2e-3
-        P, Pw = N[2 * ch + 1].ravel(order="F"), g["P3"][ch].ravel(order="F")
-        assert _rel(P.sum(), Pw.sum()) <= 2e-3
-        assert _median_rel(P, Pw) <= 3e-2
+        assert int(N[2 * ch].sum()) == int(g["Ntotal3"][ch])
+        assert _rel(N[2 * ch + 1].ravel(order="F"), g["P3"][ch].ravel(order="F")) <= 1e-8
+    r = c.RTA_driver()
+    sets = degenerate_sets(case)
+    single = np.array([s[0] for s in sets if len(s) == 1])
+    for k, key in enumerate(("rate_scatt_plus", "rate_scatt_minus", "rate_scatt")):
+        got, ref = r[key].ravel(order="F"), g["W3"][k]
+        err = _rel(got[single - 1], ref[single - 1])
+        print(f"{key}: non-degenerate modes max rel {err:.2e}")
+        assert err <= 1e-8, key
+        for s in sets:
+            if len(s) > 1:
+                assert _rel(got[s - 1].sum(), ref[s - 1].sum()) <= 1e-6, (key, s)
+    for ch, key in enumerate(("WP3_plus", "WP3_minus")):
+        assert _rel(r[key].ravel(order="F"), g["WP3"][ch]) <= 2e-4, key
 
 
-def test_weighted_phase_space(si):
-    g, c = si
+def test_four_phonon_weighted_phase_space(si):
+    g, case, c = si
     modes = np.arange(1, c.nmodes + 1)[::10]
     r, n = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
     assert n > 0
     for ch, k in enumerate(("WP4_plusplus", "WP4_plusminus", "WP4_minusminus")):
-        got, want = r[k].ravel(order="F")[modes - 1], g["WP4"][ch][modes - 1]
-        print(f"WP4 channel {ch}: total rel {_rel(got.sum(), want.sum()):.2e}, median rel {_median_rel(got, want):.2e}")
-        assert _rel(got.sum(), want.sum()) <= 3e-2 and _median_rel(got, want) <= 3e-2
-    r3 = c.RTA_driver()
-    for ch, k in enumerate(("WP3_plus", "WP3_minus")):
-        got, want = r3[k].ravel(order="F"), g["WP3"][ch]
-        assert _rel(got.sum(), want.sum()) <= 5e-2 and _median_rel(got, want) <= 1e-1
+        err = _rel(r[k].ravel(order="F")[modes - 1], g["WP4"][ch][modes - 1])
+        print(f"{k}: max rel {err:.2e}")
+        assert err <= 2e-4, k
