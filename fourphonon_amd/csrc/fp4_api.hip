@@ -1312,6 +1312,169 @@ int fp4_vp_3ph(fp4_ctx *c, int32_t channel, int64_t n, const int32_t *idx, doubl
   return FP4_OK;
 }
 
+// ============================================================ harmonic front end
+// Image list of phonon_espresso (Src/phonon_routines.f90:440-537): for each
+// atom pair, the supercell images t (bohr) inside the Wigner-Seitz cell of
+// the supercell with their weights 1/nreq, and the constants of each image.
+namespace {
+struct HarmonicImages {
+  std::vector<int> start;
+  std::vector<double> t, fc;
+};
+
+int espresso_images(const fp4_harmonic *h, const double *fc_in, HarmonicImages &out, std::vector<double> &mm) {
+  const int nat = h->natoms, s1 = h->scell[0], s2 = h->scell[1], s3 = h->scell[2];
+  const size_t ncell = (size_t)s1 * s2 * s3;
+  const float f_bohr2nm = 0.052917721092f, f_mf1 = 1.8218779f, f_mf2 = 6.022e-4f;
+  const double bohr2nm = (double)f_bohr2nm, massfactor = (double)(f_mf1 * f_mf2), tol = (double)1e-6f;
+  // fc_s(ipol,jpol,iat,jat,t1,t2,t3), Fortran order
+  auto at = [&](int a, int b, int i, int j, size_t cell) {
+    return (size_t)a + 3 * ((size_t)b + 3 * ((size_t)i + nat * ((size_t)j + nat * cell)));
+  };
+  std::vector<double> fc(fc_in, fc_in + 9 * (size_t)nat * nat * ncell);
+  // momentum conservation on the on-site block, Src/phonon_routines.f90:416-423
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int i = 0; i < nat; i++) {
+        double s = 0.0;
+        for (size_t cell = 0; cell < ncell; cell++)
+          for (int j = 0; j < nat; j++) s = s + fc[at(a, b, i, j, cell)];
+        fc[at(a, b, i, i, 0)] = fc[at(a, b, i, i, 0)] - s;
+      }
+  double cell_r[3][3], r[64][3];   // rows a_i in bohr; atom positions in bohr
+  for (int i = 0; i < 3; i++)
+    for (int c = 0; c < 3; c++) cell_r[i][c] = h->lattvec[c + 3 * i] / bohr2nm;
+  for (int i = 0; i < nat; i++)
+    for (int c = 0; c < 3; c++) {
+      double x = 0.0;
+      for (int k = 0; k < 3; k++) x = x + h->lattvec[c + 3 * k] * h->positions[k + 3 * i];
+      r[i][c] = x / bohr2nm;
+    }
+  double rws[124][4];
+  int nr = 0;
+  for (int m1 = -2; m1 <= 2; m1++)
+    for (int m2 = -2; m2 <= 2; m2++)
+      for (int m3 = -2; m3 <= 2; m3++) {
+        if (!m1 && !m2 && !m3) continue;
+        double d = 0.0;
+        for (int c = 0; c < 3; c++) {
+          rws[nr][c + 1] = cell_r[0][c] * s1 * m1 + cell_r[1][c] * s2 * m2 + cell_r[2][c] * s3 * m3;
+          d = d + rws[nr][c + 1] * rws[nr][c + 1];
+        }
+        rws[nr][0] = 0.5 * d;
+        nr++;
+      }
+  out.start.assign(1, 0);
+  out.t.clear();
+  out.fc.clear();
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < nat; j++) {
+      for (int m1 = -2 * s1; m1 <= 2 * s1; m1++)
+        for (int m2 = -2 * s2; m2 <= 2 * s2; m2++)
+          for (int m3 = -2 * s3; m3 <= 2 * s3; m3++) {
+            double t[3], rw[3];
+            for (int c = 0; c < 3; c++) {
+              t[c] = m1 * cell_r[0][c] + m2 * cell_r[1][c] + m3 * cell_r[2][c];
+              rw[c] = t[c] + (r[i][c] - r[j][c]);
+            }
+            int nreq = 1;
+            bool outside = false;
+            for (int k = 0; k < nr; k++) {
+              const double ck = rw[0] * rws[k][1] + rw[1] * rws[k][2] + rw[2] * rws[k][3] - rws[k][0];
+              if (ck > tol) outside = true;
+              else if (std::fabs(ck) < tol) nreq++;
+            }
+            if (outside) continue;
+            const double w = 1.0 / (double)nreq;
+            const size_t cell = (size_t)(((m1 % s1) + s1) % s1) +
+                                (size_t)s1 * ((size_t)(((m2 % s2) + s2) % s2) + (size_t)s2 * (((m3 % s3) + s3) % s3));
+            for (int c = 0; c < 3; c++) out.t.push_back(t[c]);
+            for (int a = 0; a < 3; a++)
+              for (int b = 0; b < 3; b++) out.fc.push_back(fc[at(a, b, i, j, cell)] * w);
+          }
+      out.start.push_back((int)(out.t.size() / 3));
+    }
+  mm.assign((size_t)nat * nat, 0.0);
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < nat; j++) {
+      const double mi = h->masses[h->types[i] - 1] / massfactor, mj = h->masses[h->types[j] - 1] / massfactor;
+      mm[(size_t)i * nat + j] = (i == j) ? mi : std::sqrt(mi * mj);
+    }
+  return 0;
+}
+}  // namespace
+
+int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double *symmetrizers, double *omega,
+                          double *eigenvect, double *velocity) {
+  if (!h || !fc || !omega || !eigenvect || !velocity || !h->positions || !h->masses || !h->types)
+    return fail(FP4_ERR_ARG, "null argument");
+  const int nat = h->natoms, nb = 3 * nat;
+  if (nat < 1 || nat > 64) return fail(FP4_ERR_ARG, "natoms out of range");
+  if (!nb_supported(nb)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d not built", nb);
+  for (int c = 0; c < 3; c++)
+    if (h->ngrid[c] < 1 || h->scell[c] < 1) return fail(FP4_ERR_ARG, "bad ngrid or scell");
+  for (int i = 0; i < nat; i++)
+    if (h->types[i] < 1 || h->types[i] > h->nelements) return fail(FP4_ERR_ARG, "types out of range");
+  const int nptk = h->ngrid[0] * h->ngrid[1] * h->ngrid[2];
+  HarmonicImages im;
+  std::vector<double> mm;
+  espresso_images(h, fc, im, mm);
+  double rl[9];
+  {
+    // rlattvec = 2 pi inv(lattvec)^T, columns b_i
+    const double *A = h->lattvec;
+    const double det = A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+                       A[6] * (A[1] * A[5] - A[4] * A[2]);
+    if (!(std::fabs(det) > 0.0)) return fail(FP4_ERR_ARG, "singular lattice");
+    const double cof[9] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8], A[3] * A[7] - A[4] * A[6],
+                           A[2] * A[7] - A[1] * A[8], A[0] * A[8] - A[2] * A[6], A[1] * A[6] - A[0] * A[7],
+                           A[1] * A[5] - A[2] * A[4], A[2] * A[3] - A[0] * A[5], A[0] * A[4] - A[1] * A[3]};
+    // b_i . a_j = 2 pi delta_ij: b_i = 2 pi (a_j x a_k) / det
+    for (int i = 0; i < 3; i++)
+      for (int c = 0; c < 3; c++) rl[c + 3 * i] = 2.0 * M_PI * cof[c + 3 * i] / det;
+  }
+  HIPCHECK(hipSetDevice(h->device));
+  DevBuf d_start, d_t, d_fc, d_mm, d_symm, d_om, d_eig, d_vel;
+  int rc;
+  if ((rc = d_start.ensure(sizeof(int) * im.start.size())) || (rc = d_t.ensure(sizeof(double) * im.t.size())) ||
+      (rc = d_fc.ensure(sizeof(double) * im.fc.size())) || (rc = d_mm.ensure(sizeof(double) * mm.size())) ||
+      (rc = d_om.ensure(sizeof(double) * (size_t)nptk * nb)) ||
+      (rc = d_eig.ensure(sizeof(double2) * (size_t)nptk * nb * nb)) ||
+      (rc = d_vel.ensure(sizeof(double) * (size_t)nptk * nb * 3)) ||
+      (symmetrizers && (rc = d_symm.ensure(sizeof(double) * 9 * (size_t)nptk))))
+    return rc;
+  HIPCHECK(hipMemcpy(d_start.p, im.start.data(), sizeof(int) * im.start.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_t.p, im.t.data(), sizeof(double) * im.t.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_fc.p, im.fc.data(), sizeof(double) * im.fc.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_mm.p, mm.data(), sizeof(double) * mm.size(), hipMemcpyHostToDevice));
+  if (symmetrizers)
+    HIPCHECK(hipMemcpy(d_symm.p, symmetrizers, sizeof(double) * 9 * (size_t)nptk, hipMemcpyHostToDevice));
+  EigenDMArgs a{};
+  a.nptk = nptk;
+  a.nb = nb;
+  a.nat = nat;
+  a.n1 = h->ngrid[0];
+  a.n2 = h->ngrid[1];
+  a.n3 = h->ngrid[2];
+  for (int t = 0; t < 9; t++) a.b[t] = rl[t];
+  a.bohr2nm = (double)0.052917721092f;
+  a.tothz = (double)20670.687f;
+  a.img_start = d_start.as<int>();
+  a.img_t = d_t.as<double>();
+  a.img_fc = d_fc.as<double>();
+  a.mm = d_mm.as<double>();
+  a.symm = symmetrizers ? d_symm.as<double>() : nullptr;
+  a.omega = d_om.as<double>();
+  a.eig = d_eig.as<double2>();
+  a.vel = d_vel.as<double>();
+  if (launch_eigen_dm(a, nullptr)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", nb);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(omega, d_om.p, sizeof(double) * (size_t)nptk * nb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(eigenvect, d_eig.p, sizeof(double2) * (size_t)nptk * nb * nb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(velocity, d_vel.p, sizeof(double) * (size_t)nptk * nb * 3, hipMemcpyDeviceToHost));
+  return FP4_OK;
+}
+
 static BteModes bte_modes(const fp4_ctx *c) {
   BteModes m;
   m.omega = c->energy.as<double>();

@@ -111,4 +111,22 @@ struct Vp3Args {
 };
 int launch_vp3(const Vp3Args &a, hipStream_t st);
 
+// Harmonic front end (eigenDM, Quantum Espresso constants): one wavefront
+// per grid point.
+struct EigenDMArgs {
+  int nptk, nb, nat;
+  int n1, n2, n3;
+  double b[9];              // rlattvec(3,3) column-major, nm^-1
+  double bohr2nm, tothz;
+  const int *img_start;     // [nat*nat + 1] image ranges per atom pair (iat*nat + jat)
+  const double *img_t;      // [nimg][3] image vectors, bohr
+  const double *img_fc;     // [nimg][9] Phi(alpha,beta) * Wigner-Seitz weight, alpha slow
+  const double *mm;         // [nat*nat] sqrt(m_i m_j) (m_i on the diagonal), Ry mass units
+  const double *symm;       // [nptk][9] symmetrizers(:,:,q) column-major, or null
+  double *omega;            // omega(nptk,Nb)
+  double2 *eig;             // eigenvect(nptk,Nb,Nb)
+  double *vel;              // velocity(nptk,Nb,3)
+};
+int launch_eigen_dm(const EigenDMArgs &a, hipStream_t st);
+
 }  // namespace fp4

@@ -39,6 +39,7 @@ EXPORTS = [
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
+    "fp4_eigen_dm_espresso",
 ]
 
 
@@ -83,6 +84,39 @@ def selftest_mfma(device: int = 0) -> float:
     err = C.c_double()
     _check(load().fp4_selftest_mfma(C.c_int32(device), C.byref(err)))
     return err.value
+
+
+class Harmonic(C.Structure):
+    _fields_ = [("ngrid", C.c_int32 * 3), ("natoms", C.c_int32), ("nelements", C.c_int32), ("scell", C.c_int32 * 3),
+                ("lattvec", C.c_double * 9), ("positions", C.c_void_p), ("masses", C.c_void_p),
+                ("types", C.c_void_p), ("device", C.c_int32)]
+
+
+def eigen_dm_espresso(structure, fc, scell, ngrid, symmetrizers=None, device: int = 0):
+    """Device eigenDM for espresso 2nd-order constants fc(3,3,nat,nat,s1,s2,s3)
+    -> (omega (nptk,Nb), eigenvect (nptk,Nb,Nb), velocity (nptk,Nb,3)), F-order."""
+    st = structure
+    nat = st.natoms
+    nb = 3 * nat
+    nptk = int(np.prod(ngrid))
+    pos = np.asfortranarray(np.asarray(st.positions, np.float64).T)          # positions(3,natoms)
+    masses = np.ascontiguousarray(st.masses, np.float64)
+    types = np.ascontiguousarray(st.types, np.int32)
+    fcf = np.asfortranarray(fc, np.float64)
+    h = Harmonic()
+    h.ngrid[:] = [int(x) for x in ngrid]
+    h.natoms, h.nelements = nat, len(masses)
+    h.scell[:] = [int(x) for x in scell]
+    h.lattvec[:] = list(np.asarray(st.lattvec, np.float64).ravel(order="F"))
+    h.positions, h.masses, h.types = _p(pos), _p(masses), _p(types)
+    h.device = device
+    om = np.zeros((nptk, nb), order="F")
+    eig = np.zeros((nptk, nb, nb), np.complex128, order="F")
+    vel = np.zeros((nptk, nb, 3), order="F")
+    sym = None if symmetrizers is None else np.asfortranarray(symmetrizers, np.float64)
+    _check(load().fp4_eigen_dm_espresso(C.byref(h), _p(fcf), _p(sym) if sym is not None else None,
+                                        _p(om), _p(eig), _p(vel)))
+    return om, eig, vel
 
 
 class Context:

@@ -209,6 +209,31 @@ int fp4_ind_3ph(fp4_ctx *ctx, const int32_t *N_plus, const int32_t *N_minus, int
    (re, im). */
 int fp4_vp_3ph(fp4_ctx *ctx, int32_t channel, int64_t n, const int32_t *idx, double *out);
 
+/* ---- harmonic front end (Src/phonon_routines.f90) -----------------------
+   eigenDM for Quantum Espresso 2nd-order constants (phonon_espresso,
+   Src/phonon_routines.f90:12-85, 311-677), on the device: omega(nptk,Nb)
+   rad/ps, eigenvect(nptk,Nb,Nb) complex (component 3*(atom-1)+alpha last)
+   and velocity(nptk,Nb,3) km/s on the Gamma-centred grid, velocities through
+   the q-point symmetrizers(3,3,nptk) (may be NULL), acoustic modes at Gamma
+   exactly zero.  fc is fc_s(3,3,natoms,natoms,scell(1),scell(2),scell(3)) in
+   Ry/bohr^2 as read from espresso.ifc2.  The non-analytic (polar) term is
+   not built: it is zero for zero Born charges, the only case accepted.
+   Eigenvectors of degenerate modes are one orthonormal basis of their
+   subspace.  Needs no context. */
+typedef struct fp4_harmonic {
+  int32_t ngrid[3];
+  int32_t natoms;
+  int32_t nelements;
+  int32_t scell[3];
+  double lattvec[9];          /* lattvec(3,3) in nm, column a_i = lattvec(:,i) */
+  const double *positions;    /* positions(3,natoms), fractional */
+  const double *masses;       /* masses(nelements), amu */
+  const int32_t *types;       /* types(natoms), 1-based */
+  int32_t device;
+} fp4_harmonic;
+int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double *symmetrizers, double *omega,
+                          double *eigenvect, double *velocity);
+
 /* ---- iterative solution (Src/iterations.f90, Src/conductivity.f90) ------
    F_n(Nb,nptk,3) and the scattering operator stay on the device between
    calls; the functions below take and return the reference's host arrays. */
