@@ -4,8 +4,9 @@ Workload: the synthetic diamond crystal (a = 5.43 A, m = 28.085 amu, spring
 FC2, seeded second-shell fourth-order IFCs) on a 16x16x16 Gamma-centred mesh
 (4096 q, 6 branches, 145 irreducible q, 870 modes), RTA at 300 K,
 scalebroad 1.  A step is one RTA_driver_4ph pass over all 870 modes
-(all three channels) in the dense FP32 form; value = processes/s of that
-pass, with every input resident in HBM before the timed region.
+(all three channels) in the dense MFMA form (BF16 hi/lo by default, FP32 with
+--precision fp32); value = processes/s of that pass, with every input
+resident in HBM before the timed region.
 
 Also reported: the FP64 per-process form (the reference-faithful kernel) and
 the dense form on the same stratified sample (every 29th mode plus the modes
@@ -17,6 +18,9 @@ bounded sample.
 
 Library calls are synchronous (they return after their stream has drained),
 so a wall clock around the steps brackets exactly the device work.
+
+--workload iterative20 instead runs the diamond 20x20x20 iterative workload
+(4ph RTA + 3ph iterative kappa at 100/300/700 K) and prints its own line.
 """
 import argparse
 import json
@@ -124,6 +128,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="MFMA form of the timed steps: FP32 MFMA or BF16 hi/lo MFMA (FP32 accumulate)")
+    ap.add_argument("--workload", choices=["rta16", "iterative20"], default="rta16",
+                    help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
+                         "4ph RTA + 3ph iterative kappa at 100/300/700 K")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -132,6 +139,9 @@ def main():
         sys.exit(2)
     if args.copy_bytes > 0:
         copy_calibration(args.copy_bytes)
+    if args.workload == "iterative20":
+        run_iterative20(args)
+        return
 
     from fourphonon_amd import crystal
     from fourphonon_amd import runtime as rt
@@ -294,6 +304,61 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def run_iterative20(args):
+    """Diamond 20x20x20 with 1st-shell FC3/FC4, the reference's default mode:
+    4ph RTA rates (MFMA form), 3ph lists and Gamma on the device, tau0 from
+    both, the 3ph iterative solution to eps = 1e-5 on the device, at 100, 300
+    and 700 K.  Prints one JSON line with the times and kappa per T."""
+    from fourphonon_amd import crystal
+    from fourphonon_amd import runtime as rt
+
+    t0 = time.time()
+    case = crystal.config_case(5)
+    m = case.modes
+    log(f"case built in {time.time() - t0:.1f}s: nptk={m.nptk} Nlist={m.Nlist} Ntri4={case.ifc4.Ntri} "
+        f"Ntri3={case.ifc3.Ntri}")
+    ctx = rt.Context(case)
+    nm = ctx.nmodes
+    all_modes = np.arange(1, nm + 1, dtype=np.int32)
+    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    N4 = [np.zeros(nm, np.int64) for _ in range(3)]
+    L4 = [[np.zeros(1, np.int64)] * 3 for _ in range(3)]
+    G4 = [np.zeros(1)] * 3
+    t1 = time.perf_counter()
+    n3 = ctx.NP_driver()
+    t_np3 = time.perf_counter() - t1
+    N3 = [n3[0], n3[2]]
+    per_T = []
+    for T in (100.0, 300.0, 700.0):
+        ctx.set_temperature(T)
+        t1 = time.perf_counter()
+        r4, n4 = ctx.RTA_driver_4ph_modes(all_modes, prec)
+        t4 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        L3, G3, R3 = ctx.Ind_driver(*N3)
+        t3 = time.perf_counter() - t1
+        rate = R3["rate_scatt"] + r4["rate_scatt_4ph"]
+        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+        t1 = time.perf_counter()
+        ctx.bte_set_operator(N4, L4, G4, N3, L3, G3)
+        res = ctx.bte_solve(tau, maxiter=1000)
+        ts = time.perf_counter() - t1
+        k = res["kappa_iter"]
+        rec = {"T": T, "processes_4ph": int(n4), "processes_3ph": int(sum(int(x.sum()) for x in N3)),
+               "rta_4ph_seconds": t4, "processes_4ph_per_s": n4 / t4, "ind_3ph_seconds": t3,
+               "solve_seconds": ts, "niter": int(res["niter"]), "final_relchange": float(res["relchange"][-1]),
+               "kappa_rta": float(np.trace(k[:, :, 0]) / 3), "kappa": float(np.trace(k[:, :, -1]) / 3)}
+        per_T.append(rec)
+        log(f"T={T:.0f} K: 4ph {t4:.1f}s ({n4} processes), 3ph Ind {t3:.2f}s, solve {ts:.2f}s "
+            f"in {rec['niter']} iterations, kappa {rec['kappa']:.4f} W/m/K (RTA {rec['kappa_rta']:.4f})")
+    ctx.close()
+    total = sum(r["rta_4ph_seconds"] + r["ind_3ph_seconds"] + r["solve_seconds"] for r in per_T) + t_np3
+    print(json.dumps({"workload": "diamond 20x20x20, 1st-shell FC3/FC4, 4ph RTA + 3ph iterative kappa "
+                                  "(reference default mode), eps 1e-5",
+                      "unit": "W/(m K)", "precision_4ph": args.precision, "np_3ph_seconds": t_np3,
+                      "total_seconds": total, "temperatures": per_T}), flush=True)
 
 
 if __name__ == "__main__":

@@ -18,6 +18,7 @@
 !   Ind_driver                    Src/processes.f90:524-561
 !   Vp_plus / Vp_minus            Src/processes.f90:25-139
 !   iteration0                    Src/iterations.f90:13-30
+!   iteration                     Src/iterations.f90:34-93
 !   iteration_4ph                 Src/iterations.f90:95-120
 !   TConduct                      Src/conductivity.f90:11-16
 ! The reference reads the crystal and run parameters from `module config`
@@ -245,7 +246,7 @@ module fp4_processes
   public :: Vp_pp, Vp_pm, Vp_mm
   public :: NP_driver, Ind_only_driver, RTA_driver, RTA_driver_GPU_using_Ind, Ind_driver, Vp_plus, Vp_minus
   public :: fp4_init_iterations, fp4_keep_operator, fp4_set_operator_4ph, fp4_solve_4ph
-  public :: iteration0, iteration_4ph, TConduct
+  public :: iteration0, iteration, iteration_4ph, TConduct
 
 contains
 
@@ -710,6 +711,35 @@ contains
     end if
     call check(c_bte_iterate(ctx, tau_zero, F_n), "iteration_4ph")
   end subroutine iteration_4ph
+
+  ! The three-phonon iteration of the reference's default mode
+  ! (four_phonon_iteration = .false., Src/iterations.f90:34-93): the same
+  ! in-place update with empty four-phonon lists; 4ph scattering enters only
+  ! through tau_zero.
+  subroutine iteration(Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, Ntotal_minus, &
+       Indof2ndPhonon_plus, Indof3rdPhonon_plus, Indof2ndPhonon_minus, Indof3rdPhonon_minus, omega, &
+       velocity, Gamma_plus, Gamma_minus, tau_zero, F_n)
+    integer(kind=4), intent(in) :: Nlist, Nequi(:), ALLEquiList(:,:), TypeofSymmetry(:,:)
+    integer(kind=4), intent(in) :: N_plus(:), N_minus(:), Ntotal_plus, Ntotal_minus
+    integer(kind=4), intent(in) :: Indof2ndPhonon_plus(:), Indof3rdPhonon_plus(:)
+    integer(kind=4), intent(in) :: Indof2ndPhonon_minus(:), Indof3rdPhonon_minus(:)
+    real(kind=8), intent(in) :: omega(:,:), velocity(:,:,:)
+    real(kind=8), intent(in) :: Gamma_plus(:), Gamma_minus(:), tau_zero(:,:)
+    real(kind=8), intent(inout) :: F_n(:,:,:)
+    integer(kind=8), allocatable :: zn(:), zl(:)
+    real(kind=8), allocatable :: zg(:)
+    if (.not. (keep_operator_ .and. operator_valid_)) then
+       allocate(zn(size(N_plus)), zl(1), zg(1))
+       zn = 0
+       zl = 1
+       zg = 0.d0
+       call fp4_set_operator_4ph(Nlist, Nequi, ALLEquiList, TypeofSymmetry, N_plus, N_minus, Ntotal_plus, &
+            Ntotal_minus, zn, zn, zn, 0_8, 0_8, 0_8, Indof2ndPhonon_plus, Indof3rdPhonon_plus, &
+            Indof2ndPhonon_minus, Indof3rdPhonon_minus, zl, zl, zl, zl, zl, zl, zl, zl, zl, &
+            Gamma_plus, Gamma_minus, zg, zg, zg)
+    end if
+    call check(c_bte_iterate(ctx, tau_zero, F_n), "iteration")
+  end subroutine iteration
 
   ! TConduct at the context's T and the V of fp4_init_iterations.
   subroutine TConduct(omega, velocity, F_n, ThConductivity, ThConductivityMode)

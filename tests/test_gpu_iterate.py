@@ -167,3 +167,46 @@ def test_kappa_3ph_plus_4ph_end_to_end(mesh):
         assert rel(got["F_n"], want["F_n"]) <= 1e-9
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_kappa_default_mode_4ph_rta_3ph_iterative(prec):
+    """The reference's default mode (four_phonon_iteration = .false.,
+    Src/ShengBTE.f90:1273-1284): tau0 from the 3ph + 4ph rates, the iteration
+    over the 3ph operator only (Src/iterations.f90:34-93).  4ph rates from the
+    FP64 form (kappa within 1e-9 of the oracle) or the BF16 hi/lo MFMA form
+    (within 1e-4)."""
+    from oracle.oracle import Oracle, bte_solve
+    case = crystal.config_case(1, ngrid=(4, 4, 4))
+    banner(f"default mode: 4ph RTA ({prec}) + 3ph iterative, diamond 4x4x4")
+    c = rt.Context(case)
+    o = Oracle(case)
+    try:
+        n3 = c.NP_driver()
+        N3 = [n3[0], n3[2]]
+        L3, G3, R3 = c.Ind_driver(*N3)
+        r4 = c.RTA_driver_4ph(rt.PREC_FP64 if prec == "fp64" else rt.PREC_BF16X3)["rate_scatt_4ph"]
+        rate = R3["rate_scatt"] + r4
+        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+        nm = c.nmodes
+        N4 = [np.zeros(nm, np.int64) for _ in range(3)]
+        L4 = [[np.zeros(1, np.int64)] * 3 for _ in range(3)]
+        G4 = [np.zeros(1)] * 3
+        c.bte_set_operator(N4, L4, G4, N3, L3, G3)
+        got = c.bte_solve(tau, maxiter=60)
+
+        No3, _ = o.np3_driver(nthreads=8)
+        wl3, wg3, wr3 = o.ind3_driver(No3, nthreads=8)
+        wr4 = o.rta_driver(nthreads=8)["rate_4ph"]
+        wrate = (wr3["rate"] + wr4).reshape(rate.shape, order="F")
+        wtau = np.asfortranarray(np.where(wrate > 0, 1.0 / np.where(wrate > 0, wrate, 1.0), 0.0))
+        want = bte_solve(case, N4, L4, G4, wtau, maxiter=60, N3=No3, lists3=wl3, gammas3=wg3)
+        kg, kw = got["kappa_iter"][:, :, -1], want["kappa_iter"][:, :, -1]
+        err = float(np.abs(kg - kw).max() / np.abs(kw).max())
+        print(f"niter {got['niter']} (oracle {want['niter']}), kappa_xx {kg[0, 0]:.6e}, rel err {err:.2e}")
+        if prec == "fp64":
+            assert got["niter"] == want["niter"] and err <= 1e-9
+        else:
+            assert abs(got["niter"] - want["niter"]) <= 1 and err <= 1e-4
+    finally:
+        c.close()
