@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfp4.so")
 SOURCES = ["fp4_api.hip", "fp4_prefilter.hip", "fp4_rate64.hip", "fp4_dense.hip", "fp4_iterate.hip", "fp4_3ph.hip",
-           "fp4_harmonic.hip"]
+           "fp4_harmonic.hip", "fp4_sample.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -72,7 +72,10 @@ struct DevBuf {
   }
 };
 
-enum { KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_ITER = 4, KS_SCAN3 = 5, KS_RATE3 = 6, KS_N = 7 };
+enum {
+  KS_PREFILTER = 0, KS_EMIT = 1, KS_RATE64 = 2, KS_DENSE = 3, KS_ITER = 4, KS_SCAN3 = 5, KS_RATE3 = 6, KS_SAMPLE = 7,
+  KS_N = 8
+};
 
 }  // namespace
 
@@ -1035,6 +1038,137 @@ int fp4_ind_only_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int
     HIPCHECK(hipStreamSynchronize(c->stream));
   }
   return FP4_OK;
+}
+
+// ---- sampling estimators (four phonons) -----------------------------------
+// Segments are (channel, mode): one item each.
+static int sample_check(const fp4_ctx *c, int32_t nsample) {
+  (void)c;
+  if (nsample < 1) return fail(FP4_ERR_ARG, "nsample must be positive (got %d)", nsample);
+  return 0;
+}
+
+static long long sample_total(const fp4_ctx *c) {
+  // Nbands*nptk*Nbands*nptk*Nbands in int64, processes.f90:2497, 4836
+  return (long long)c->nb * c->nptk * c->nb * c->nptk * c->nb;
+}
+
+static SampleArgs sample_args(const fp4_ctx *c, const Plan &pl, int32_t nsample, uint64_t seed) {
+  SampleArgs a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = (int)pl.modes.size();
+  a.nsample = nsample;
+  a.seed = seed;
+  return a;
+}
+
+static int sample_count_pass(fp4_ctx *c, Plan &pl, int32_t nsample, uint64_t seed) {
+  const int nsub = (int)pl.modes.size();
+  pl.ii_lo0 = 0;
+  pl.nii = 1;
+  pl.items = 3L * nsub;
+  const int nseg = 3 * nsub;
+  int rc;
+  if ((rc = c->modes_sub.ensure(sizeof(int) * nsub))) return rc;
+  if ((rc = c->cnt.ensure(sizeof(int) * nseg))) return rc;
+  if ((rc = c->psum.ensure(sizeof(double) * nseg))) return rc;
+  if ((rc = c->segN.ensure(sizeof(long long) * nseg))) return rc;
+  if ((rc = c->segP.ensure(sizeof(double) * nseg))) return rc;
+  HIPCHECK(hipMemcpyAsync(c->modes_sub.p, pl.modes.data(), sizeof(int) * nsub, hipMemcpyHostToDevice, c->stream));
+  SampleArgs a = sample_args(c, pl, nsample, seed);
+  a.seg_lo = 0;
+  a.seg_hi = nseg;
+  a.cnt = c->cnt.as<int>();
+  a.psum = c->psum.as<double>();
+  if ((rc = timed(c, KS_SAMPLE, [&] {
+         return launch_sample4(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+       })))
+    return rc;
+  return finish_counts(c, pl, nseg, true);
+}
+
+static int sample_emit_batch(fp4_ctx *c, const Plan &pl, const Batch &b, int32_t nsample, uint64_t seed) {
+  int rc;
+  const long long n = b.p1 - b.p0;
+  for (int t = 0; t < 3; t++)
+    if ((rc = c->lists[t].ensure(sizeof(int) * (size_t)std::max(n, 1LL)))) return rc;
+  SampleArgs a = sample_args(c, pl, nsample, seed);
+  a.seg_lo = (int)b.item_lo;
+  a.seg_hi = (int)b.item_hi;
+  a.offsets = c->offsets.as<long long>();
+  a.out_base = b.p0;
+  for (int t = 0; t < 3; t++) a.out32[t] = c->lists[t].as<int>();
+  return timed(c, KS_SAMPLE, [&] { return launch_sample4(a, 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+}
+
+int fp4_np_4ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, int64_t *N_pp, double *P_pp, int64_t *N_pm,
+                      double *P_pm, int64_t *N_mm, double *P_mm) {
+  int rc;
+  if ((rc = check_ready(c, false)) || (rc = sample_check(c, nsample))) return rc;
+  if (!N_pp || !P_pp || !N_pm || !P_pm || !N_mm || !P_mm) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  all_modes(c, pl);
+  if ((rc = sample_count_pass(c, pl, nsample, seed))) return rc;
+  const int nm = (int)pl.modes.size();
+  const double total = (double)sample_total(c), num = (double)nsample;
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  double *P[3] = {P_pp, P_pm, P_mm};
+  for (int ch = 0; ch < 3; ch++)
+    for (int m = 0; m < nm; m++) {
+      // INT(REAL(N,8)/num*total, 8) and P*total/num, processes.f90:2562-2564
+      N[ch][m] = (int64_t)((double)pl.segN[ch * nm + m] / num * total);
+      P[ch][m] = pl.segP[ch * nm + m] * total / num;
+    }
+  return FP4_OK;
+}
+
+int fp4_rta_4ph_sample_modes(fp4_ctx *c, int32_t nsample, uint64_t seed, int32_t nmodes, const int32_t *modes,
+                             double *rate_4ph, double *r_pp, double *r_pm, double *r_mm, double *w_pp, double *w_pm,
+                             double *w_mm, int64_t *nhits) {
+  int rc;
+  if ((rc = check_ready(c, true)) || (rc = sample_check(c, nsample))) return rc;
+  if (!rate_4ph || !r_pp || !r_pm || !r_mm || !w_pp || !w_pm || !w_mm) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  if ((rc = sample_count_pass(c, pl, nsample, seed))) return rc;
+  const int nseg = 3 * nmodes;
+  std::vector<double> g(nseg, 0.0), w(nseg, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = sample_emit_batch(c, pl, b, nsample, seed))) return rc;
+    if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
+  }
+  write_rates(c, pl, g, w, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm, true);
+  // Gamma/num*total and WP4*total/num, processes.f90:4922-4923
+  const double total = (double)sample_total(c), num = (double)nsample;
+  double *R[3] = {r_pp, r_pm, r_mm};
+  double *W[3] = {w_pp, w_pm, w_mm};
+  for (int mi = 0; mi < nmodes; mi++) {
+    const int m = pl.modes[mi] - 1;
+    for (int ch = 0; ch < 3; ch++) {
+      R[ch][m] = R[ch][m] / num * total;
+      W[ch][m] = W[ch][m] * total / num;
+    }
+    rate_4ph[m] = r_pp[m] + r_pm[m] + r_mm[m];
+  }
+  if (nhits) *nhits = std::accumulate(pl.segN.begin(), pl.segN.end(), 0LL);
+  return FP4_OK;
+}
+
+int fp4_rta_4ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, double *rate_4ph, double *r_pp, double *r_pm,
+                       double *r_mm, double *w_pp, double *w_pm, double *w_mm) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  std::vector<int> modes((size_t)c->nb * c->nlist);
+  std::iota(modes.begin(), modes.end(), 1);
+  const size_t nm = modes.size();
+  double *outs[7] = {rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  return fp4_rta_4ph_sample_modes(c, nsample, seed, (int)nm, modes.data(), rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm,
+                                  w_mm, nullptr);
 }
 
 // ---- iterative solution ---------------------------------------------------

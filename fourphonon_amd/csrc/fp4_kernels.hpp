@@ -30,6 +30,25 @@ void launch_reduce_counts(const int *cnt, const double *psum, int nseg, int nii,
                           hipStream_t st);
 void launch_widen(const int *cnt, long long *out, long n, hipStream_t st);
 
+// Sampling estimators: segments are (channel, mode-in-subset), ch-major.
+struct SampleArgs {
+  ModeView m;
+  const int *modes;          // mm (1-based) of the subset
+  int nsub;
+  long long nsample;         // draws per segment
+  unsigned long long seed;
+  int seg_lo, seg_hi;        // segments of this launch
+  // count pass
+  int *cnt;                  // [3*nsub] hits
+  double *psum;              // [3*nsub] phase-space sums
+  // emit pass
+  const long long *offsets;  // exclusive prefix of cnt, 3*nsub+1 entries
+  long long out_base;        // offsets[seg_lo]
+  int *out32[3];
+};
+
+int launch_sample4(const SampleArgs &a, int mode, hipStream_t st);
+
 // A block of at most 256 consecutive processes of one (channel, mode).
 struct BlockDesc {
   long long start;   // first process, index into the batch's list arrays

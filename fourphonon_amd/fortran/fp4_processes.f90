@@ -97,6 +97,22 @@ module fp4_processes
        integer(c_int64_t), intent(out) :: Npp(*), Npm(*), Nmm(*)
        real(c_double), intent(out) :: Ppp(*), Ppm(*), Pmm(*)
      end function c_np
+     integer(c_int) function c_np_sample(ctx, ns, seed, Npp, Ppp, Npm, Ppm, Nmm, Pmm) bind(C, name="fp4_np_4ph_sample")
+       import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ns
+       integer(c_int64_t), value :: seed
+       integer(c_int64_t), intent(out) :: Npp(*), Npm(*), Nmm(*)
+       real(c_double), intent(out) :: Ppp(*), Ppm(*), Pmm(*)
+     end function c_np_sample
+     integer(c_int) function c_rta_sample(ctx, ns, seed, r4, rpp, rpm, rmm, wpp, wpm, wmm) &
+          bind(C, name="fp4_rta_4ph_sample")
+       import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ns
+       integer(c_int64_t), value :: seed
+       real(c_double), intent(out) :: r4(*), rpp(*), rpm(*), rmm(*), wpp(*), wpm(*), wmm(*)
+     end function c_rta_sample
      integer(c_int) function c_ind_only(ctx, Npp, Npm, Nmm, a2, a3, a4, b2, b3, b4, c2, c3, c4) &
           bind(C, name="fp4_ind_only_4ph")
        import :: c_int, c_ptr, c_int64_t
@@ -235,13 +251,18 @@ module fp4_processes
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int32_t), save :: precision = FP4_PREC_FP64_DENSE
+  ! num_sample_process_4ph and num_sample_process_4ph_phase_space
+  ! (Src/config.f90:23, default -1 = no sampling) and the seed of the
+  ! sampling streams
+  integer(kind=4), save :: num_sample_4ph_ = -1, num_sample_4ph_ps_ = -1
+  integer(c_int64_t), save :: sample_seed_ = 1
   integer(kind=4), save :: nptk_ = 0, nbands_ = 0
   real(c_double), save :: volume_ = 0.d0
   logical, save :: keep_operator_ = .false., operator_valid_ = .false.
   real(c_double), allocatable, target, save :: masses_(:)
   integer(c_int32_t), allocatable, target, save :: types_(:)
 
-  public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision
+  public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision, fp4_set_sampling
   public :: NP_driver_4ph, Ind_only_driver_4ph, RTA_driver_4ph, RTA_driver_4ph_GPU_using_Ind, Ind_driver_4ph
   public :: Vp_pp, Vp_pm, Vp_mm
   public :: NP_driver, Ind_only_driver, RTA_driver, RTA_driver_GPU_using_Ind, Ind_driver, Vp_plus, Vp_minus
@@ -317,6 +338,18 @@ contains
     precision = p
   end subroutine fp4_set_precision
 
+  ! The reference's num_sample_process_4ph / num_sample_process_4ph_phase_space
+  ! (-1: the full drivers; n > 0: n sampled candidates per channel and mode)
+  ! and the seed of the sampling streams (the reference reseeds from the
+  ! clock; here a run is reproducible for a given seed).
+  subroutine fp4_set_sampling(num_sample_process_4ph, num_sample_process_4ph_phase_space, seed)
+    integer(kind=4), intent(in) :: num_sample_process_4ph, num_sample_process_4ph_phase_space
+    integer(kind=8), intent(in) :: seed
+    num_sample_4ph_ = num_sample_process_4ph
+    num_sample_4ph_ps_ = num_sample_process_4ph_phase_space
+    sample_seed_ = seed
+  end subroutine fp4_set_sampling
+
   subroutine upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
     real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
     complex(kind=8), intent(in) :: eigenvect(:,:,:)
@@ -350,8 +383,13 @@ contains
     integer(kind=8), intent(out) :: N_plusplus(:), N_plusminus(:), N_minusminus(:)
     real(kind=8), intent(out) :: Pspace_plusplus_total(:,:), Pspace_plusminus_total(:,:), Pspace_minusminus_total(:,:)
     call upload_modes_noeig(energy, velocity, Nlist, List, IJK)
-    call check(c_np(ctx, N_plusplus, Pspace_plusplus_total, N_plusminus, Pspace_plusminus_total, &
-         N_minusminus, Pspace_minusminus_total), "NP_driver_4ph")
+    if (num_sample_4ph_ps_ == -1) then
+       call check(c_np(ctx, N_plusplus, Pspace_plusplus_total, N_plusminus, Pspace_plusminus_total, &
+            N_minusminus, Pspace_minusminus_total), "NP_driver_4ph")
+    else
+       call check(c_np_sample(ctx, num_sample_4ph_ps_, sample_seed_, N_plusplus, Pspace_plusplus_total, &
+            N_plusminus, Pspace_plusminus_total, N_minusminus, Pspace_minusminus_total), "NP_driver_4ph (sampled)")
+    end if
   end subroutine NP_driver_4ph
 
   subroutine Ind_only_driver_4ph(energy, velocity, Nlist, List, IJK, N_plusplus, N_plusminus, N_minusminus, &
@@ -385,8 +423,14 @@ contains
          rate_scatt_minusminus(:,:), WP4_plusplus(:,:), WP4_plusminus(:,:), WP4_minusminus(:,:)
     call upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
     call upload_ifc4(Ntri, Phi, R_j, R_k, R_l, Index_i, Index_j, Index_k, Index_l)
-    call check(c_rta(ctx, precision, rate_scatt_4ph, rate_scatt_plusplus, rate_scatt_plusminus, &
-         rate_scatt_minusminus, WP4_plusplus, WP4_plusminus, WP4_minusminus), "RTA_driver_4ph")
+    if (num_sample_4ph_ == -1) then
+       call check(c_rta(ctx, precision, rate_scatt_4ph, rate_scatt_plusplus, rate_scatt_plusminus, &
+            rate_scatt_minusminus, WP4_plusplus, WP4_plusminus, WP4_minusminus), "RTA_driver_4ph")
+    else
+       call check(c_rta_sample(ctx, num_sample_4ph_, sample_seed_, rate_scatt_4ph, rate_scatt_plusplus, &
+            rate_scatt_plusminus, rate_scatt_minusminus, WP4_plusplus, WP4_plusminus, WP4_minusminus), &
+            "RTA_driver_4ph (sampled)")
+    end if
   end subroutine RTA_driver_4ph
 
   subroutine RTA_driver_4ph_GPU_using_Ind(energy, velocity, eigenvect, Nlist, List, IJK, &

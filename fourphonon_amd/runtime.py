@@ -39,7 +39,7 @@ EXPORTS = [
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
-    "fp4_eigen_dm_espresso",
+    "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
 ]
 
 
@@ -238,6 +238,28 @@ class Context:
                                         *[_p(o) for o in out], C.byref(nproc)))
         return self._named(out), nproc.value
 
+    # ------------------------------------------------ sampling estimators
+    def NP_driver_4ph_sample(self, nsample, seed):
+        """Sampled NP_driver_4ph -> (N_pp, P_pp, N_pm, P_pm, N_mm, P_mm) estimates."""
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int64) for _ in range(3)]
+        P = [np.zeros((self.nb, self.nlist), order="F") for _ in range(3)]
+        _check(load().fp4_np_4ph_sample(self.h, C.c_int32(nsample), C.c_uint64(seed), _p(N[0]), _p(P[0]), _p(N[1]),
+                                        _p(P[1]), _p(N[2]), _p(P[2])))
+        return N[0], P[0], N[1], P[1], N[2], P[2]
+
+    def RTA_driver_4ph_sample(self, nsample, seed, modes=None):
+        """Sampled RTA_driver_4ph (FP64 per-process form) -> (rates dict, sampled processes evaluated)."""
+        out = self._rate_outputs()
+        if modes is None:
+            _check(load().fp4_rta_4ph_sample(self.h, C.c_int32(nsample), C.c_uint64(seed), *[_p(o) for o in out]))
+            return self._named(out), None
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        n = C.c_int64()
+        _check(load().fp4_rta_4ph_sample_modes(self.h, C.c_int32(nsample), C.c_uint64(seed), C.c_int32(len(modes)),
+                                               _p(modes), *[_p(o) for o in out], C.byref(n)))
+        return self._named(out), n.value
+
     def RTA_driver_4ph_GPU_using_Ind(self, N_pp, N_pm, N_mm, lists):
         Ns = [np.ascontiguousarray(n, np.int64) for n in (N_pp, N_pm, N_mm)]
         L3 = [[np.ascontiguousarray(x, np.int64) if len(x) else np.zeros(1, np.int64) for x in ch] for ch in lists]
@@ -361,7 +383,7 @@ class Context:
         return {"F_n": F, "kappa": kappa, "kappa_iter": hist[:, :, :k + 1], "relchange": rel[:k + 1], "niter": k}
 
     # ------------------------------------------------------------ diagnostics
-    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE, KS_ITER, KS_SCAN3, KS_RATE3 = 0, 1, 2, 3, 4, 5, 6
+    KS_PREFILTER, KS_EMIT, KS_RATE64, KS_DENSE, KS_ITER, KS_SCAN3, KS_RATE3, KS_SAMPLE = 0, 1, 2, 3, 4, 5, 6, 7
 
     def kernel_stats(self, kind):
         ms = C.c_double()

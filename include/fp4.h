@@ -153,6 +153,33 @@ int fp4_ind_only_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, i
                            int64_t *Indof4thPhonon_pm, int64_t *Indof2ndPhonon_mm, int64_t *Indof3rdPhonon_mm,
                            int64_t *Indof4thPhonon_mm);
 
+/* ---- sampling estimators (Src/processes.f90:2480-2746, 4811-5171) -------
+   The branches NP_driver_4ph and RTA_driver_4ph take when
+   num_sample_process_4ph_phase_space / num_sample_process_4ph is not -1
+   (Src/processes.f90:2807-2826, 5257-5290).  Per (channel, mode) nsample
+   candidates are drawn uniformly from the Nb*nptk*Nb*nptk*Nb box; those in
+   the part the full enumeration visits that pass its tests and exact
+   window are summed, and the sums are scaled by total/nsample as the
+   reference does (N = INT(REAL(hits)/nsample*total), P*total/nsample,
+   Gamma/nsample*total, WP4*total/nsample).  The estimates are unbiased for
+   the full drivers' outputs.  Random numbers come from counter-based
+   SplitMix64 streams keyed by (seed, channel, mode), so a result is a
+   function of (seed, nsample) and the inputs; the reference reseeds from the
+   clock on every call, draws num+1 samples and reads out of bounds for the
+   first one -- neither is reproduced. */
+int fp4_np_4ph_sample(fp4_ctx *ctx, int32_t nsample, uint64_t seed, int64_t *N_pp, double *P_pp, int64_t *N_pm,
+                      double *P_pm, int64_t *N_mm, double *P_mm);
+/* Rates in the FP64 per-process form over the sampled processes; outputs
+   as fp4_rta_4ph. */
+int fp4_rta_4ph_sample(fp4_ctx *ctx, int32_t nsample, uint64_t seed, double *rate_4ph, double *rate_pp,
+                       double *rate_pm, double *rate_mm, double *WP4_pp, double *WP4_pm, double *WP4_mm);
+/* As fp4_rta_4ph_sample for a strictly increasing subset of modes; outputs
+   are (Nb,Nlist) arrays of which the subset's entries are written; *nhits
+   (unless NULL) receives the number of sampled processes evaluated. */
+int fp4_rta_4ph_sample_modes(fp4_ctx *ctx, int32_t nsample, uint64_t seed, int32_t nmodes, const int32_t *modes,
+                             double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm, double *WP4_pp,
+                             double *WP4_pm, double *WP4_mm, int64_t *nhits);
+
 /* ---- three-phonon path (Src/processes.f90:25-1836) ----------------------
    Channel 0 = "+" (absorption), 1 = "-" (emission).  Counts and lists are
    int32, as the reference declares them; a channel whose total would not fit
@@ -315,7 +342,7 @@ int fp4_bte_solve(fp4_ctx *ctx, double V, const double *tau_zero, int32_t maxite
    kind 0 = prefilter count pass, 1 = list emission, 2 = FP64 per-process
    rate kernel, 3 = dense rate kernels, 4 = iteration_4ph sweeps (launches
    counts (ll, i, kk) steps), 5 = three-phonon scan (count and list passes),
-   6 = three-phonon FP64 rate kernel. */
+   6 = three-phonon FP64 rate kernel, 7 = sampling passes. */
 int fp4_ctx_kernel_stats(fp4_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 int fp4_ctx_reset_stats(fp4_ctx *ctx);
 /* Diagnostics; no counterpart in the reference.  Work of the last dense-form

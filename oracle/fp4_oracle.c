@@ -17,6 +17,8 @@
  *   Ind_only_*           processes.f90:2854-3074, Ind_only_driver_4ph 3080-3246
  *   Ind_* (iterative)    processes.f90:3254-3545, Ind_driver_4ph 3551-3822
  *   RTA_*                processes.f90:3828-4129, RTA_driver_4ph 5174-5311
+ *   *_sample             processes.f90:2480-2746, 4811-5171 (sampling
+ *                        estimators; the random stream is this file's own)
  *   iteration0           iterations.f90:13-30
  *   iteration_4ph        iterations.f90:95-215 (3ph and 4ph terms)
  *   TConduct             conductivity.f90:11-37
@@ -153,68 +155,81 @@ typedef struct {
 
 typedef void (*visit_fn)(const orc_case *c, const orc_hit *h, void *ud);
 
+/* The tests of one candidate (ii, jj, ss; j, k, l) of mode (i, qi) with
+   frequency omega in channel ch: nonzero frequencies, the +- pair
+   exclusions and the window |delta| <= sigma, then the 0.001 clamp.  Fills
+   h and returns 1 for a listed process. */
+static int test4(const orc_case *c, int ch, int i, int qi, double omega, int ii, int jj, int ss, int j, int k, int l,
+                 orc_hit *h) {
+  const double op = E(c, ii, j), odp = E(c, jj, k), otp = E(c, ss, l);
+  if (!((op != 0.0) && (odp != 0.0) && (otp != 0.0))) return 0;
+  double dv[3];
+  for (int a = 1; a <= 3; a++) {
+    if (ch == 0)
+      dv[a - 1] = -VEL(c, jj, k, a) + VEL(c, ss, l, a);
+    else
+      dv[a - 1] = VEL(c, jj, k, a) - VEL(c, ss, l, a);
+  }
+  double sigma = c->scalebroad * orc_base_sigma(c, dv);
+  if (ch == 1) {
+    if (!((qi != jj || i != k) && (qi != ss || i != l) && (ii != jj || j != k) && (ii != ss || j != l))) return 0;
+  }
+  double delta;
+  if (ch == 0)
+    delta = omega + op + odp - otp;
+  else if (ch == 1)
+    delta = omega + op - odp - otp;
+  else
+    delta = omega - op - odp - otp;
+  if (!(fabs(delta) <= sigma)) return 0;
+  if (sigma <= (double)F_CLAMP) sigma = 1.0 / sqrt(PI_D);
+  h->omega = omega;
+  h->ii = ii; h->jj = jj; h->ss = ss; h->j = j; h->k = k; h->l = l;
+  h->omegap = op; h->omegadp = odp; h->omegatp = otp;
+  h->delta = delta;
+  h->sigma = sigma;
+  return 1;
+}
+
+/* ss of channel ch for grid points qi, ii, jj (1-based), Index_N of
+   processes.f90:2277-2298 */
+static int third_partner(const orc_case *c, int ch, int qi, int ii, int jj) {
+  const int32_t *q = c->IJK + 3 * (size_t)(qi - 1);
+  const int32_t *qp = c->IJK + 3 * (size_t)(ii - 1);
+  const int32_t *qdp = c->IJK + 3 * (size_t)(jj - 1);
+  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
+  int t0, t1, t2;
+  if (ch == 0) {
+    t0 = q[0] + qp[0] + qdp[0]; t1 = q[1] + qp[1] + qdp[1]; t2 = q[2] + qp[2] + qdp[2];
+  } else if (ch == 1) {
+    t0 = q[0] + qp[0] - qdp[0]; t1 = q[1] + qp[1] - qdp[1]; t2 = q[2] + qp[2] - qdp[2];
+  } else {
+    t0 = q[0] - qp[0] - qdp[0]; t1 = q[1] - qp[1] - qdp[1]; t2 = q[2] - qp[2] - qdp[2];
+  }
+  t0 = fmodulo(t0, n1); t1 = fmodulo(t1, n2); t2 = fmodulo(t2, n3);
+  return (t2 * n2 + t1) * n1 + t0 + 1;
+}
+
 static void enumerate(const orc_case *c, int ch, int mm, int ii_lo, int ii_hi, visit_fn f, void *ud) {
   const int nb = c->nbands, nptk = c->nptk;
-  const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
   const int i = fmodulo(mm - 1, nb) + 1;
   const int ll = (mm - 1) / nb + 1;
   const int qi = c->List[ll - 1];
-  const int32_t *q = c->IJK + 3 * (size_t)(qi - 1);
+  const double omega = E(c, qi, i);
   orc_hit h;
-  h.omega = E(c, qi, i);
-  if (!(h.omega != 0.0)) return;
+  if (!(omega != 0.0)) return;
   if (ii_lo < 1) ii_lo = 1;
   if (ii_hi > nptk) ii_hi = nptk;
   for (int ii = ii_lo; ii <= ii_hi; ii++) {
-    const int32_t *qp = c->IJK + 3 * (size_t)(ii - 1);
     for (int jj = (ch == 1 ? 1 : ii); jj <= nptk; jj++) {
-      const int32_t *qdp = c->IJK + 3 * (size_t)(jj - 1);
-      int t0, t1, t2;
-      if (ch == 0) {
-        t0 = q[0] + qp[0] + qdp[0]; t1 = q[1] + qp[1] + qdp[1]; t2 = q[2] + qp[2] + qdp[2];
-      } else if (ch == 1) {
-        t0 = q[0] + qp[0] - qdp[0]; t1 = q[1] + qp[1] - qdp[1]; t2 = q[2] + qp[2] - qdp[2];
-      } else {
-        t0 = q[0] - qp[0] - qdp[0]; t1 = q[1] - qp[1] - qdp[1]; t2 = q[2] - qp[2] - qdp[2];
-      }
-      t0 = fmodulo(t0, n1); t1 = fmodulo(t1, n2); t2 = fmodulo(t2, n3);
-      const int ss = (t2 * n2 + t1) * n1 + t0 + 1;
+      const int ss = third_partner(c, ch, qi, ii, jj);
       if (ch == 0 ? !(ss >= 1) : !(ss >= jj)) continue;
       for (int j = 1; j <= nb; j++) {
         const int kstart = (ch != 1 && jj == ii) ? j : 1;
         for (int k = kstart; k <= nb; k++) {
           const int lstart = (ch != 0 && ss == jj) ? k : 1;
-          for (int l = lstart; l <= nb; l++) {
-            const double op = E(c, ii, j), odp = E(c, jj, k), otp = E(c, ss, l);
-            if (!((op != 0.0) && (odp != 0.0) && (otp != 0.0))) continue;
-            double dv[3];
-            for (int a = 1; a <= 3; a++) {
-              if (ch == 0)
-                dv[a - 1] = -VEL(c, jj, k, a) + VEL(c, ss, l, a);
-              else
-                dv[a - 1] = VEL(c, jj, k, a) - VEL(c, ss, l, a);
-            }
-            double sigma = c->scalebroad * orc_base_sigma(c, dv);
-            if (ch == 1) {
-              if (!((qi != jj || i != k) && (qi != ss || i != l) && (ii != jj || j != k) && (ii != ss || j != l)))
-                continue;
-            }
-            double delta;
-            if (ch == 0)
-              delta = h.omega + op + odp - otp;
-            else if (ch == 1)
-              delta = h.omega + op - odp - otp;
-            else
-              delta = h.omega - op - odp - otp;
-            if (fabs(delta) <= sigma) {
-              if (sigma <= (double)F_CLAMP) sigma = 1.0 / sqrt(PI_D);
-              h.ii = ii; h.jj = jj; h.ss = ss; h.j = j; h.k = k; h.l = l;
-              h.omegap = op; h.omegadp = odp; h.omegatp = otp;
-              h.delta = delta;
-              h.sigma = sigma;
-              f(c, &h, ud);
-            }
-          }
+          for (int l = lstart; l <= nb; l++)
+            if (test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, &h)) f(c, &h, ud);
         }
       }
     }
@@ -502,6 +517,142 @@ void orc_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *
     const int m = modes[t] - 1;
     rate_4ph[m] = rate_pp[m] + rate_pm[m] + rate_mm[m];
   }
+}
+
+/* ---------------------------------------------------- sampling estimators
+ * processes.f90:2480-2746 (NP_*_sample) and 4811-5171 (RTA_*_sample),
+ * dispatched at 2807-2826 and 5257-5290.  Candidates (ii, j, jj, k, l) are
+ * drawn uniformly from the Nb*nptk*Nb*nptk*Nb box and kept when they lie in
+ * the part the full enumeration visits (jj >= ii, k >= j on the diagonal,
+ * for ++ and --; ss >= jj, l >= k on the diagonal, for +- and --) and pass
+ * test4; sums are scaled by total/nsample in the reference's order.
+ * Random numbers: SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) evaluated
+ * at counter positions: segment key mix64(seed + G*(3*(mm-1)+ch+1)), draw t
+ * of sample s mix64(key + G*(5*s+t+1)), an index below n as the high 64 bits
+ * of draw*n.  Exactly nsample samples are drawn (the reference draws num+1,
+ * the first from out-of-bounds entries of its random array).
+ * ------------------------------------------------------------------------ */
+#define ORC_GOLDEN 0x9E3779B97F4A7C15ull
+
+static uint64_t orc_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static int orc_below(uint64_t u, int n) { return (int)(((unsigned __int128)u * (uint64_t)n) >> 64); }
+
+/* Raw sums of one (channel, mode): hits, phase space, Gamma and WP4 (before
+   the unit and sampling factors); rates only when want_rates. */
+void orc_sample_mode(const orc_case *c, int ch, int mm, int32_t nsample, uint64_t seed, int want_rates,
+                     int64_t *hits, double *P, double *gamma, double *wp4) {
+  const int nb = c->nbands, nptk = c->nptk;
+  const int i = fmodulo(mm - 1, nb) + 1;
+  const int qi = c->List[(mm - 1) / nb];
+  const double omega = E(c, qi, i);
+  np_acc na;
+  const double n = (double)nptk;
+  na.n = 0;
+  na.p = 0.0;
+  na.dn3 = n * n * n;
+  na.dnb4 = (double)(nb * nb * nb * nb);
+  rta_acc ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.i = i;
+  ra.qi = qi;
+  ra.ch = ch;
+  ra.nb = nb;
+  if (omega != 0.0) {
+    const uint64_t key = orc_mix64(seed + ORC_GOLDEN * (uint64_t)(3 * (mm - 1) + ch + 1));
+    for (int64_t s = 0; s < nsample; s++) {
+      const uint64_t base = key + ORC_GOLDEN * (uint64_t)(5 * s + 1);
+      const int ii = 1 + orc_below(orc_mix64(base), nptk);
+      const int j = 1 + orc_below(orc_mix64(base + ORC_GOLDEN), nb);
+      const int jj = 1 + orc_below(orc_mix64(base + 2 * ORC_GOLDEN), nptk);
+      const int k = 1 + orc_below(orc_mix64(base + 3 * ORC_GOLDEN), nb);
+      const int l = 1 + orc_below(orc_mix64(base + 4 * ORC_GOLDEN), nb);
+      const int ss = third_partner(c, ch, qi, ii, jj);
+      if (ch != 1 && !(jj >= ii && (jj != ii || k >= j))) continue;
+      if (ch != 0 && !(ss >= jj && (ss != jj || l >= k))) continue;
+      orc_hit h;
+      if (!test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, &h)) continue;
+      visit_np(c, &h, &na);
+      if (want_rates) visit_rta(c, &h, &ra);
+    }
+  }
+  *hits = na.n;
+  *P = na.p;
+  if (gamma) *gamma = ra.gamma;
+  if (wp4) *wp4 = ra.wp4;
+}
+
+static double orc_sample_total(const orc_case *c) {
+  return (double)((int64_t)c->nbands * c->nptk * c->nbands * c->nptk * c->nbands);
+}
+
+/* NP_driver_4ph with num_sample_process_4ph_phase_space = nsample */
+void orc_np_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int64_t *N_pp,
+                          double *P_pp, int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm) {
+  const int nm = c->nbands * c->nlist;
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  double *P[3] = {P_pp, P_pm, P_mm};
+  const double total = orc_sample_total(c), num = (double)nsample;
+  set_threads(nthreads);
+  for (int m = 0; m < nm; m++)
+    for (int ch = 0; ch < 3; ch++) {
+      N[ch][m] = 0;
+      P[ch][m] = 0.0;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    for (int ch = 0; ch < 3; ch++) {
+      int64_t hits;
+      double p;
+      orc_sample_mode(c, ch, mm, nsample, seed, 0, &hits, &p, NULL, NULL);
+      /* processes.f90:2562-2564 */
+      N[ch][mm - 1] = (int64_t)((double)hits / num * total);
+      P[ch][mm - 1] = p * total / num;
+    }
+  }
+}
+
+/* RTA_driver_4ph with num_sample_process_4ph = nsample, over a mode list;
+   hits (unless NULL) receives the number of sampled processes evaluated. */
+void orc_rta_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int nmodes,
+                           const int32_t *modes, double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm,
+                           double *wp4_pp, double *wp4_pm, double *wp4_mm, int64_t *hits) {
+  double *R[3] = {rate_pp, rate_pm, rate_mm};
+  double *W[3] = {wp4_pp, wp4_pm, wp4_mm};
+  const double total = orc_sample_total(c), num = (double)nsample;
+  int64_t nh = 0;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nh)
+  for (int t = 0; t < nmodes; t++) {
+    const int mm = modes[t];
+    const int i = fmodulo(mm - 1, c->nbands) + 1;
+    const int ll = (mm - 1) / c->nbands + 1;
+    const double omega = E(c, c->List[ll - 1], i);
+    if (!(omega <= c->omega_max)) continue;
+    for (int ch = 0; ch < 3; ch++) {
+      int64_t h;
+      double p, g, w;
+      orc_sample_mode(c, ch, mm, nsample, seed, 1, &h, &p, &g, &w);
+      nh += h;
+      /* processes.f90:4918-4923 */
+      if (omega != 0.0) w = w / c->nptk / c->nptk;
+      g = g * (double)F_GAMMA_UNITS * 1.e9 / c->nptk / c->nptk;
+      R[ch][mm - 1] = g / num * total;
+      W[ch][mm - 1] = w * total / num;
+    }
+  }
+  for (int t = 0; t < nmodes; t++) {
+    const int m = modes[t] - 1;
+    rate_4ph[m] = rate_pp[m] + rate_pm[m] + rate_mm[m];
+  }
+  if (hits) *hits = nh;
 }
 
 /* processes.f90:3551-3822 (the momentum test of Ind_* selects the same ss as

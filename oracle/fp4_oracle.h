@@ -84,6 +84,15 @@ void orc_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *
                     int32_t ii_lo, int32_t ii_hi,
                     double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm,
                     double *wp4_pp, double *wp4_pm, double *wp4_mm);
+void orc_sample_mode(const orc_case *c, int ch, int mm, int32_t nsample, uint64_t seed, int want_rates,
+                     int64_t *hits, double *P, double *gamma, double *wp4);
+void orc_np_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed,
+                          int64_t *N_pp, double *P_pp, int64_t *N_pm, double *P_pm,
+                          int64_t *N_mm, double *P_mm);
+void orc_rta_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed,
+                           int nmodes, const int32_t *modes, double *rate_4ph, double *rate_pp,
+                           double *rate_pm, double *rate_mm, double *wp4_pp, double *wp4_pm,
+                           double *wp4_mm, int64_t *hits);
 int orc_ind_driver(const orc_case *c, int nthreads, const int64_t *N_pp,
                    const int64_t *N_pm, const int64_t *N_mm,
                    int64_t *i2_pp, int64_t *i3_pp, int64_t *i4_pp,

@@ -164,6 +164,32 @@ class Oracle:
         names = ["rate_4ph", "rate_pp", "rate_pm", "rate_mm", "wp4_pp", "wp4_pm", "wp4_mm"]
         return dict(zip(names, out))
 
+    # ---------------------------------------------------- sampling estimators
+    def sample_mode(self, ch, mm, nsample, seed, want_rates=False):
+        """Raw sums of one (channel, mode) -> (hits, P, Gamma, WP4) before unit and sampling factors."""
+        h, p, g, w = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        self.L.orc_sample_mode(C.byref(self.c), C.c_int(ch), C.c_int(mm), C.c_int32(nsample), C.c_uint64(seed),
+                               C.c_int(int(want_rates)), C.byref(h), C.byref(p), C.byref(g), C.byref(w))
+        return h.value, p.value, g.value, w.value
+
+    def np_sample_driver(self, nsample, seed, nthreads=0):
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int64) for _ in range(3)]
+        P = [np.zeros(nm, np.float64) for _ in range(3)]
+        self.L.orc_np_sample_driver(C.byref(self.c), C.c_int(nthreads), C.c_int32(nsample), C.c_uint64(seed),
+                                    _ptr(N[0]), _ptr(P[0]), _ptr(N[1]), _ptr(P[1]), _ptr(N[2]), _ptr(P[2]))
+        return N, P
+
+    def rta_sample_driver(self, nsample, seed, modes=None, nthreads=0):
+        nm = self.nmodes
+        modes = np.arange(1, nm + 1, dtype=np.int32) if modes is None else np.ascontiguousarray(modes, np.int32)
+        out = [np.zeros(nm, np.float64) for _ in range(7)]
+        hits = C.c_int64()
+        self.L.orc_rta_sample_driver(C.byref(self.c), C.c_int(nthreads), C.c_int32(nsample), C.c_uint64(seed),
+                                     C.c_int(len(modes)), _ptr(modes), *[_ptr(o) for o in out], C.byref(hits))
+        names = ["rate_4ph", "rate_pp", "rate_pm", "rate_mm", "wp4_pp", "wp4_pm", "wp4_mm"]
+        return dict(zip(names, out)), hits.value
+
     def ind_driver(self, N, nthreads=0):
         nm = self.nmodes
         tot = [int(n.sum()) for n in N]
