@@ -20,7 +20,9 @@ Library calls are synchronous (they return after their stream has drained),
 so a wall clock around the steps brackets exactly the device work.
 
 --workload iterative20 instead runs the diamond 20x20x20 iterative workload
-(4ph RTA + 3ph iterative kappa at 100/300/700 K) and prints its own line.
+(4ph RTA + 3ph iterative kappa at 100/300/700 K) and prints its own line;
+--workload sample16 runs the sampling estimator of RTA_driver_4ph on the
+16x16x16 crystal against the full rates.
 """
 import argparse
 import json
@@ -128,9 +130,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="MFMA form of the timed steps: FP32 MFMA or BF16 hi/lo MFMA (FP32 accumulate)")
-    ap.add_argument("--workload", choices=["rta16", "iterative20"], default="rta16",
+    ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16"], default="rta16",
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
-                         "4ph RTA + 3ph iterative kappa at 100/300/700 K")
+                         "4ph RTA + 3ph iterative kappa at 100/300/700 K; sample16: the sampling estimator on "
+                         "diamond 16^3 against the full rates")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -141,6 +144,9 @@ def main():
         copy_calibration(args.copy_bytes)
     if args.workload == "iterative20":
         run_iterative20(args)
+        return
+    if args.workload == "sample16":
+        run_sample16(args)
         return
 
     from fourphonon_amd import crystal
@@ -304,6 +310,44 @@ def main():
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def run_sample16(args):
+    """Diamond 16x16x16, 2nd-shell FC4 (the metric's crystal): the sampling
+    estimator of RTA_driver_4ph (num_sample_process_4ph) at several sample
+    counts against the full dense-form rates.  Prints one JSON line with the
+    time, the sampled processes evaluated and the per-mode rate and RTA kappa
+    errors per sample count."""
+    from fourphonon_amd import crystal
+    from fourphonon_amd import runtime as rt
+
+    case = crystal.config_case(2)
+    ctx = rt.Context(case)
+    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    t1 = time.perf_counter()
+    full = ctx.RTA_driver_4ph(prec)["rate_scatt_4ph"].ravel(order="F")
+    t_full = time.perf_counter() - t1
+    k_full = np.trace(kappa_rta(case, full)) / 3
+    log(f"full dense rates in {t_full:.2f}s, RTA kappa {k_full:.4f}")
+    ok = full > 0
+    recs = []
+    for ns in (10000, 100000, 1000000):
+        t1 = time.perf_counter()
+        r, _ = ctx.RTA_driver_4ph_sample(ns, 1)
+        ts = time.perf_counter() - t1
+        est = r["rate_scatt_4ph"].ravel(order="F")
+        err = np.abs(est[ok] - full[ok]) / full[ok]
+        k = np.trace(kappa_rta(case, est)) / 3
+        ms, _ = ctx.kernel_stats(rt.Context.KS_SAMPLE)
+        rec = {"nsample": ns, "seconds": ts, "rate_rel_err_median": float(np.median(err)),
+               "rate_rel_err_p90": float(np.percentile(err, 90)), "kappa_rel_err": float(abs(k - k_full) / k_full)}
+        recs.append(rec)
+        log(f"nsample {ns}: {ts:.3f}s, per-mode rate error median {rec['rate_rel_err_median']:.2e} "
+            f"p90 {rec['rate_rel_err_p90']:.2e}, kappa error {rec['kappa_rel_err']:.2e}")
+    ctx.close()
+    print(json.dumps({"workload": "diamond 16x16x16, 2nd-shell FC4, sampled 4ph RTA rates vs the full dense form",
+                      "precision_full": args.precision, "full_seconds": t_full, "kappa_full": float(k_full),
+                      "samples": recs}), flush=True)
 
 
 def run_iterative20(args):

@@ -267,6 +267,128 @@ int launch_scan3(const Scan3Args &a, int mode, hipStream_t st) {
   return 0;
 }
 
+// Sampling estimators of NP_driver / RTA_driver (Src/processes.f90:828-979,
+// 1474-1673): per (channel, mode) segment nsample draws of (q', j, k), each
+// uniform over the Nb*nptk*Nb candidates of the full loops (index
+// ((ii0*Nb + j)*Nb + k), decoded as the reference does), kept when they pass
+// the full path's tests.  Draw s: mix64(key + kGolden*(s+1)) with
+// key = mix64(seed + kGolden*(2*(mm-1)+ch+1)).  Count pass: hits and
+// phase-space sums per thread in sample order, fixed-tree block sums; emit
+// pass: hits in sample order as int32 list entries for k3_rate64.
+template <int NB>
+__device__ __forceinline__ bool sample3_one(const ModeView &m, int ch, double w, const int *q,
+                                            unsigned long long key, long long s, int &ii0, int &j0, int &ss0,
+                                            int &k0, double &delta, double &sigma) {
+  const int r = below(mix64(key + kGolden * (unsigned long long)(s + 1)), NB * m.nptk * NB);
+  k0 = r % NB;
+  j0 = (r / NB) % NB;
+  ii0 = r / (NB * NB);
+  int qp[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) qp[t] = m.ijk[3 * ii0 + t];
+  ss0 = partner3(m, ch, q, qp);
+  const double op = E(m, ii0, j0), odp = E(m, ss0, k0);
+  if (!((op != 0.0) && (odp != 0.0))) return false;
+  sigma = sigma3(m, ii0, j0, ss0, k0);
+  delta = delta3(ch, w, op, odp);
+  return fabs(delta) <= 2.0 * sigma;
+}
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(256) k3_sample(SampleArgs a) {
+  const ModeView &m = a.m;
+  const int seg = blockIdx.x + a.seg_lo;
+  if (seg >= a.seg_hi) return;
+  const int ch = seg / a.nsub, mi = seg % a.nsub;
+  const int mm = a.modes[mi];
+  const int i0 = (mm - 1) % NB;
+  const int qi0 = m.list[(mm - 1) / NB] - 1;
+  const double w = E(m, qi0, i0);
+  int q[3];
+#pragma unroll
+  for (int t = 0; t < 3; t++) q[t] = m.ijk[3 * qi0 + t];
+  const bool active = (w != 0.0) && (w <= m.omega_max);
+  const unsigned long long key = mix64(a.seed + kGolden * (unsigned long long)(2 * (mm - 1) + ch + 1));
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ double sp[4];
+  __shared__ int sc[4];
+  if (MODE == 0) {
+    int cnt = 0;
+    double psum = 0.0;
+    const double den0 = (double)((long long)m.nptk * m.nptk);
+    const double nb3 = (double)(NB * NB * NB);
+    if (active) {
+      for (long long s = tid; s < a.nsample; s += 256) {
+        int ii0, j0, ss0, k0;
+        double delta, sigma;
+        if (!sample3_one<NB>(m, ch, w, q, key, s, ii0, j0, ss0, k0, delta, sigma)) continue;
+        cnt += 1;
+        const double num = exp(-(delta * delta) / (sigma * sigma));
+        const double den = sigma * sqrt(kPi) * den0 * nb3;
+        psum = psum + num / den;
+      }
+    }
+    cnt = wave_sum_i(cnt);
+    psum = wave_sum(psum);
+    if (lane == 0) {
+      sc[wv] = cnt;
+      sp[wv] = psum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      a.cnt[seg] = ((sc[0] + sc[1]) + sc[2]) + sc[3];
+      a.psum[seg] = ((sp[0] + sp[1]) + sp[2]) + sp[3];
+    }
+  } else {
+    if (!active) return;
+    long long p = a.offsets[seg] - a.out_base;
+    const long long end = a.offsets[seg + 1] - a.out_base;   // the count pass's total for this segment
+    for (long long s0 = 0; s0 < a.nsample; s0 += 256) {
+      const long long s = s0 + tid;
+      int ii0 = 0, j0 = 0, ss0 = 0, k0 = 0;
+      double delta, sigma;
+      const bool hit = s < a.nsample && sample3_one<NB>(m, ch, w, q, key, s, ii0, j0, ss0, k0, delta, sigma);
+      const int f = hit ? 1 : 0;
+      const int off = wave_excl_scan(f, lane);
+      const int tot = wave_sum_i(f);
+      __syncthreads();   // the previous chunk has read sc
+      if (lane == 0) sc[wv] = tot;
+      __syncthreads();
+      int before = 0;
+      for (int t = 0; t < wv; t++) before += sc[t];
+      const long long at = p + before + off;
+      if (hit && at < end) {
+        a.out32[0][at] = ii0 * NB + j0 + 1;
+        a.out32[1][at] = ss0 * NB + k0 + 1;
+      }
+      p += ((sc[0] + sc[1]) + sc[2]) + sc[3];
+    }
+  }
+}
+
+template <int NB>
+static void sample3_nb(const SampleArgs &a, int mode, hipStream_t st) {
+  const int n = a.seg_hi - a.seg_lo;
+  if (n <= 0) return;
+  if (mode == 0)
+    hipLaunchKernelGGL((k3_sample<NB, 0>), dim3(n), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k3_sample<NB, 1>), dim3(n), dim3(256), 0, st, a);
+}
+
+int launch_sample3(const SampleArgs &a, int mode, hipStream_t st) {
+  switch (a.m.nb) {
+    case 3: sample3_nb<3>(a, mode, st); break;
+    case 6: sample3_nb<6>(a, mode, st); break;
+    case 9: sample3_nb<9>(a, mode, st); break;
+    case 12: sample3_nb<12>(a, mode, st); break;
+    case 18: sample3_nb<18>(a, mode, st); break;
+    case 24: sample3_nb<24>(a, mode, st); break;
+    default: return -1;
+  }
+  return 0;
+}
+
 int launch_rate3(const Rate3Args &a, hipStream_t st) {
   if (a.nblocks <= 0) return 0;
   const dim3 g((unsigned)a.nblocks), b(256);

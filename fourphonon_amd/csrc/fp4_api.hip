@@ -1063,12 +1063,17 @@ static SampleArgs sample_args(const fp4_ctx *c, const Plan &pl, int32_t nsample,
   return a;
 }
 
-static int sample_count_pass(fp4_ctx *c, Plan &pl, int32_t nsample, uint64_t seed) {
+// nch = 3: four-phonon channels (++, +-, --); nch = 2: three-phonon (+, -).
+static int sample_launch(const SampleArgs &a, int nch, int mode, hipStream_t st) {
+  return nch == 3 ? launch_sample4(a, mode, st) : launch_sample3(a, mode, st);
+}
+
+static int sample_count_pass(fp4_ctx *c, Plan &pl, int32_t nsample, uint64_t seed, int nch) {
   const int nsub = (int)pl.modes.size();
   pl.ii_lo0 = 0;
   pl.nii = 1;
-  pl.items = 3L * nsub;
-  const int nseg = 3 * nsub;
+  pl.items = (long)nch * nsub;
+  const int nseg = nch * nsub;
   int rc;
   if ((rc = c->modes_sub.ensure(sizeof(int) * nsub))) return rc;
   if ((rc = c->cnt.ensure(sizeof(int) * nseg))) return rc;
@@ -1082,24 +1087,25 @@ static int sample_count_pass(fp4_ctx *c, Plan &pl, int32_t nsample, uint64_t see
   a.cnt = c->cnt.as<int>();
   a.psum = c->psum.as<double>();
   if ((rc = timed(c, KS_SAMPLE, [&] {
-         return launch_sample4(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+         return sample_launch(a, nch, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
        })))
     return rc;
   return finish_counts(c, pl, nseg, true);
 }
 
-static int sample_emit_batch(fp4_ctx *c, const Plan &pl, const Batch &b, int32_t nsample, uint64_t seed) {
+static int sample_emit_batch(fp4_ctx *c, const Plan &pl, const Batch &b, int32_t nsample, uint64_t seed, int nch) {
   int rc;
   const long long n = b.p1 - b.p0;
-  for (int t = 0; t < 3; t++)
+  for (int t = 0; t < nch; t++)
     if ((rc = c->lists[t].ensure(sizeof(int) * (size_t)std::max(n, 1LL)))) return rc;
   SampleArgs a = sample_args(c, pl, nsample, seed);
   a.seg_lo = (int)b.item_lo;
   a.seg_hi = (int)b.item_hi;
   a.offsets = c->offsets.as<long long>();
   a.out_base = b.p0;
-  for (int t = 0; t < 3; t++) a.out32[t] = c->lists[t].as<int>();
-  return timed(c, KS_SAMPLE, [&] { return launch_sample4(a, 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
+  for (int t = 0; t < nch; t++) a.out32[t] = c->lists[t].as<int>();
+  return timed(c, KS_SAMPLE,
+               [&] { return sample_launch(a, nch, 1, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "nb") : 0; });
 }
 
 int fp4_np_4ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, int64_t *N_pp, double *P_pp, int64_t *N_pm,
@@ -1109,7 +1115,7 @@ int fp4_np_4ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, int64_t *N_pp,
   if (!N_pp || !P_pp || !N_pm || !P_pm || !N_mm || !P_mm) return fail(FP4_ERR_ARG, "null output");
   Plan pl;
   all_modes(c, pl);
-  if ((rc = sample_count_pass(c, pl, nsample, seed))) return rc;
+  if ((rc = sample_count_pass(c, pl, nsample, seed, 3))) return rc;
   const int nm = (int)pl.modes.size();
   const double total = (double)sample_total(c), num = (double)nsample;
   int64_t *N[3] = {N_pp, N_pm, N_mm};
@@ -1131,11 +1137,11 @@ int fp4_rta_4ph_sample_modes(fp4_ctx *c, int32_t nsample, uint64_t seed, int32_t
   if (!rate_4ph || !r_pp || !r_pm || !r_mm || !w_pp || !w_pm || !w_mm) return fail(FP4_ERR_ARG, "null output");
   Plan pl;
   if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
-  if ((rc = sample_count_pass(c, pl, nsample, seed))) return rc;
+  if ((rc = sample_count_pass(c, pl, nsample, seed, 3))) return rc;
   const int nseg = 3 * nmodes;
   std::vector<double> g(nseg, 0.0), w(nseg, 0.0);
   for (const Batch &b : plan_batches(pl, c->list_cap)) {
-    if ((rc = sample_emit_batch(c, pl, b, nsample, seed))) return rc;
+    if ((rc = sample_emit_batch(c, pl, b, nsample, seed, 3))) return rc;
     if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
   }
   write_rates(c, pl, g, w, rate_4ph, r_pp, r_pm, r_mm, w_pp, w_pm, w_mm, true);
@@ -1171,7 +1177,6 @@ int fp4_rta_4ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, double *rate_
                                   w_mm, nullptr);
 }
 
-// ---- iterative solution ---------------------------------------------------
 // ============================================================ three phonons
 int fp4_ctx_set_ifc3(fp4_ctx *c, int32_t ntri, const double *Phi, const double *R_j, const double *R_k,
                      const int32_t *Ii, const int32_t *Ij, const int32_t *Ik) {
@@ -1446,6 +1451,86 @@ int fp4_vp_3ph(fp4_ctx *c, int32_t channel, int64_t n, const int32_t *idx, doubl
   return FP4_OK;
 }
 
+// ---- three-phonon sampling estimators -------------------------------------
+// Scaling in the reference's left-to-right
+// order: N = INT(REAL(hits)/num*Nb*nptk*Nb, 4), P*Nb*nptk*Nb/num
+// (processes.f90:898-899); Gamma*Nb*nptk*Nb/num then *5.60626442*1e8/nptk,
+// WP3/nptk (WP3*0.5/nptk for "-") then *Nb*nptk*Nb/num (:1559-1570,
+// :1660-1671); the "-" rate halved in the driver (:1759).
+int fp4_np_3ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, int32_t *N_plus, double *P_plus, int32_t *N_minus,
+                      double *P_minus) {
+  int rc;
+  if ((rc = check_ready3(c, false)) || (rc = sample_check(c, nsample))) return rc;
+  if (!N_plus || !P_plus || !N_minus || !P_minus) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  all_modes3(c, pl);
+  if ((rc = sample_count_pass(c, pl, nsample, seed, 2))) return rc;
+  const int nm = (int)pl.modes.size(), nb = c->nb, nptk = c->nptk;
+  const double num = (double)nsample;
+  int32_t *N[2] = {N_plus, N_minus};
+  double *P[2] = {P_plus, P_minus};
+  for (int ch = 0; ch < 2; ch++)
+    for (int m = 0; m < nm; m++) {
+      const double n = (double)pl.segN[ch * nm + m] / num * nb * nptk * nb;
+      if (n > 2147483647.0)
+        return fail(FP4_ERR_OVERFLOW, "channel %d mode %d: count estimate %.3e exceeds int32", ch, m + 1, n);
+      N[ch][m] = (int32_t)n;
+      P[ch][m] = pl.segP[ch * nm + m] * nb * nptk * nb / num;
+    }
+  return FP4_OK;
+}
+
+int fp4_rta_3ph_sample_modes(fp4_ctx *c, int32_t nsample, uint64_t seed, int32_t nmodes, const int32_t *modes,
+                             double *rate, double *r_plus, double *r_minus, double *w_plus, double *w_minus,
+                             int64_t *nhits) {
+  int rc;
+  if ((rc = check_ready3(c, true)) || (rc = sample_check(c, nsample))) return rc;
+  if (!rate || !r_plus || !r_minus || !w_plus || !w_minus) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  if ((rc = sample_count_pass(c, pl, nsample, seed, 2))) return rc;
+  std::vector<double> g(2 * (size_t)nmodes, 0.0), w(2 * (size_t)nmodes, 0.0);
+  for (const Batch &b : plan_batches(pl, c->list_cap)) {
+    if ((rc = sample_emit_batch(c, pl, b, nsample, seed, 2))) return rc;
+    if ((rc = rate3_batch(c, pl, b, false, g, w))) return rc;
+  }
+  const int nb = c->nb, nptk = c->nptk;
+  const double num = (double)nsample;
+  double *R[2] = {r_plus, r_minus};
+  double *W[2] = {w_plus, w_minus};
+  for (int mi = 0; mi < nmodes; mi++) {
+    const int m = pl.modes[mi] - 1;
+    for (int ch = 0; ch < 2; ch++) {
+      double gs = g[ch * nmodes + mi], ws = w[ch * nmodes + mi];
+      ws = (ch == 0) ? ws / nptk : ws * 5.e-1 / nptk;
+      gs = gs * nb * nptk * nb / num;
+      ws = ws * nb * nptk * nb / num;
+      gs = gs * (double)kGamma3Units * 1.e8 / nptk;
+      R[ch][m] = (ch == 0) ? gs : gs * 5.e-1;
+      W[ch][m] = ws;
+    }
+    rate[m] = r_plus[m] + r_minus[m];
+  }
+  if (nhits) *nhits = std::accumulate(pl.segN.begin(), pl.segN.end(), 0LL);
+  return FP4_OK;
+}
+
+int fp4_rta_3ph_sample(fp4_ctx *c, int32_t nsample, uint64_t seed, double *rate, double *r_plus, double *r_minus,
+                       double *w_plus, double *w_minus) {
+  int rc;
+  if ((rc = check_ready3(c, true))) return rc;
+  std::vector<int> modes((size_t)c->nb * c->nlist);
+  std::iota(modes.begin(), modes.end(), 1);
+  const size_t nm = modes.size();
+  double *outs[5] = {rate, r_plus, r_minus, w_plus, w_minus};
+  for (double *o : outs) {
+    if (!o) return fail(FP4_ERR_ARG, "null output");
+    std::fill(o, o + nm, 0.0);
+  }
+  return fp4_rta_3ph_sample_modes(c, nsample, seed, (int)nm, modes.data(), rate, r_plus, r_minus, w_plus, w_minus,
+                                  nullptr);
+}
+
 // ============================================================ harmonic front end
 // Image list of phonon_espresso (Src/phonon_routines.f90:440-537): for each
 // atom pair, the supercell images t (bohr) inside the Wigner-Seitz cell of
@@ -1609,6 +1694,7 @@ int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double 
   return FP4_OK;
 }
 
+// ---- iterative solution ---------------------------------------------------
 static BteModes bte_modes(const fp4_ctx *c) {
   BteModes m;
   m.omega = c->energy.as<double>();

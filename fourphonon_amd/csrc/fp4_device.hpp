@@ -200,6 +200,21 @@ __device__ __forceinline__ double wp4_of(int ch, double w, double wp, double wdp
   return wp4_given(ch, w, wp, wdp, wtp, bose(wp, T), bose(wdp, T), bose(wtp, T), delta, sigma);
 }
 
+// Counter-based random numbers of the sampling estimators: the SplitMix64
+// output function (Steele, Lea & Flood, OOPSLA 2014) applied to
+// key + kGolden*position, and an index below n as the high 64 bits of u*n.
+constexpr unsigned long long kGolden = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ int below(unsigned long long u, int n) {
+  return (int)__umul64hi(u, (unsigned long long)n);
+}
+
 // Deterministic wave-wide sums (fixed butterfly order).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

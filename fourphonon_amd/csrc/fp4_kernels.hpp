@@ -48,6 +48,9 @@ struct SampleArgs {
 };
 
 int launch_sample4(const SampleArgs &a, int mode, hipStream_t st);
+// three phonons: segments (channel, mode-in-subset) with 2 channels; the
+// emit pass writes out32[0..1]
+int launch_sample3(const SampleArgs &a, int mode, hipStream_t st);
 
 // A block of at most 256 consecutive processes of one (channel, mode).
 struct BlockDesc {

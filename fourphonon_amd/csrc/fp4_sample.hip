@@ -36,18 +36,6 @@
 
 namespace fp4 {
 
-constexpr unsigned long long kGolden = 0x9E3779B97F4A7C15ull;
-
-__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ int below(unsigned long long u, int n) {
-  return (int)__umul64hi(u, (unsigned long long)n);
-}
-
 struct SampleHit {
   int ii0, j, jj0, k, ss0, l;
   double delta;

@@ -164,6 +164,21 @@ module fp4_processes
        integer(c_int32_t), intent(out) :: Np(*), Nm(*)
        real(c_double), intent(out) :: Pp(*), Pm(*)
      end function c_np3
+     integer(c_int) function c_np3_sample(ctx, ns, seed, Np, Pp, Nm, Pm) bind(C, name="fp4_np_3ph_sample")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ns
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), intent(out) :: Np(*), Nm(*)
+       real(c_double), intent(out) :: Pp(*), Pm(*)
+     end function c_np3_sample
+     integer(c_int) function c_rta3_sample(ctx, ns, seed, r, rp, rm, wp, wm) bind(C, name="fp4_rta_3ph_sample")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ns
+       integer(c_int64_t), value :: seed
+       real(c_double), intent(out) :: r(*), rp(*), rm(*), wp(*), wm(*)
+     end function c_rta3_sample
      integer(c_int) function c_ind_only3(ctx, Np, Nm, a2, a3, b2, b3) bind(C, name="fp4_ind_only_3ph")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
@@ -251,9 +266,10 @@ module fp4_processes
 
   type(c_ptr), save :: ctx = c_null_ptr
   integer(c_int32_t), save :: precision = FP4_PREC_FP64_DENSE
-  ! num_sample_process_4ph and num_sample_process_4ph_phase_space
+  ! num_sample_process_3ph[_phase_space] and num_sample_process_4ph[_phase_space]
   ! (Src/config.f90:23, default -1 = no sampling) and the seed of the
   ! sampling streams
+  integer(kind=4), save :: num_sample_3ph_ = -1, num_sample_3ph_ps_ = -1
   integer(kind=4), save :: num_sample_4ph_ = -1, num_sample_4ph_ps_ = -1
   integer(c_int64_t), save :: sample_seed_ = 1
   integer(kind=4), save :: nptk_ = 0, nbands_ = 0
@@ -338,13 +354,17 @@ contains
     precision = p
   end subroutine fp4_set_precision
 
-  ! The reference's num_sample_process_4ph / num_sample_process_4ph_phase_space
-  ! (-1: the full drivers; n > 0: n sampled candidates per channel and mode)
+  ! The reference's four sampling switches, in the order of Src/config.f90:23
+  ! (-1: the full drivers; n > 0: n sampled candidates per channel and mode),
   ! and the seed of the sampling streams (the reference reseeds from the
   ! clock; here a run is reproducible for a given seed).
-  subroutine fp4_set_sampling(num_sample_process_4ph, num_sample_process_4ph_phase_space, seed)
+  subroutine fp4_set_sampling(num_sample_process_3ph, num_sample_process_3ph_phase_space, &
+       num_sample_process_4ph, num_sample_process_4ph_phase_space, seed)
+    integer(kind=4), intent(in) :: num_sample_process_3ph, num_sample_process_3ph_phase_space
     integer(kind=4), intent(in) :: num_sample_process_4ph, num_sample_process_4ph_phase_space
     integer(kind=8), intent(in) :: seed
+    num_sample_3ph_ = num_sample_process_3ph
+    num_sample_3ph_ps_ = num_sample_process_3ph_phase_space
     num_sample_4ph_ = num_sample_process_4ph
     num_sample_4ph_ps_ = num_sample_process_4ph_phase_space
     sample_seed_ = seed
@@ -554,7 +574,12 @@ contains
     integer(kind=4), intent(out) :: N_plus(:), N_minus(:)
     real(kind=8), intent(out) :: Pspace_plus_total(:,:), Pspace_minus_total(:,:)
     call upload_modes_noeig(energy, velocity, Nlist, List, IJK)
-    call check(c_np3(ctx, N_plus, Pspace_plus_total, N_minus, Pspace_minus_total), "NP_driver")
+    if (num_sample_3ph_ps_ == -1) then
+       call check(c_np3(ctx, N_plus, Pspace_plus_total, N_minus, Pspace_minus_total), "NP_driver")
+    else
+       call check(c_np3_sample(ctx, num_sample_3ph_ps_, sample_seed_, N_plus, Pspace_plus_total, N_minus, &
+            Pspace_minus_total), "NP_driver (sampled)")
+    end if
   end subroutine NP_driver
 
   subroutine Ind_only_driver(energy, velocity, Nlist, List, IJK, N_plus, N_minus, &
@@ -580,7 +605,12 @@ contains
          WP3_plus(:,:), WP3_minus(:,:)
     call upload_modes(energy, velocity, eigenvect, Nlist, List, IJK)
     call upload_ifc3(Ntri, Phi, R_j, R_k, Index_i, Index_j, Index_k)
-    call check(c_rta3(ctx, rate_scatt, rate_scatt_plus, rate_scatt_minus, WP3_plus, WP3_minus), "RTA_driver")
+    if (num_sample_3ph_ == -1) then
+       call check(c_rta3(ctx, rate_scatt, rate_scatt_plus, rate_scatt_minus, WP3_plus, WP3_minus), "RTA_driver")
+    else
+       call check(c_rta3_sample(ctx, num_sample_3ph_, sample_seed_, rate_scatt, rate_scatt_plus, rate_scatt_minus, &
+            WP3_plus, WP3_minus), "RTA_driver (sampled)")
+    end if
   end subroutine RTA_driver
 
   subroutine RTA_driver_GPU_using_Ind(energy, velocity, eigenvect, Nlist, List, IJK, Ntri, Phi, R_j, R_k, &

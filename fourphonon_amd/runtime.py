@@ -40,6 +40,7 @@ EXPORTS = [
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
     "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
+    "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes",
 ]
 
 
@@ -432,6 +433,27 @@ class Context:
         out = self._rate3_outputs()
         _check(load().fp4_rta_3ph(self.h, *[_p(o) for o in out]))
         return dict(zip(self._RATE3, out))
+
+    def NP_driver_sample(self, nsample, seed):
+        """Sampled NP_driver -> (N_plus, P_plus, N_minus, P_minus) estimates."""
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int32) for _ in range(2)]
+        P = [np.zeros((self.nb, self.nlist), order="F") for _ in range(2)]
+        _check(load().fp4_np_3ph_sample(self.h, C.c_int32(nsample), C.c_uint64(seed), _p(N[0]), _p(P[0]), _p(N[1]),
+                                        _p(P[1])))
+        return N[0], P[0], N[1], P[1]
+
+    def RTA_driver_sample(self, nsample, seed, modes=None):
+        """Sampled RTA_driver -> (rates dict, sampled processes evaluated or None)."""
+        out = self._rate3_outputs()
+        if modes is None:
+            _check(load().fp4_rta_3ph_sample(self.h, C.c_int32(nsample), C.c_uint64(seed), *[_p(o) for o in out]))
+            return dict(zip(self._RATE3, out)), None
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        n = C.c_int64()
+        _check(load().fp4_rta_3ph_sample_modes(self.h, C.c_int32(nsample), C.c_uint64(seed), C.c_int32(len(modes)),
+                                               _p(modes), *[_p(o) for o in out], C.byref(n)))
+        return dict(zip(self._RATE3, out)), n.value
 
     def RTA_driver_modes(self, modes, ii_lo=1, ii_hi=None):
         """RTA_driver over a mode subset and a q' window -> (dict, nproc)."""

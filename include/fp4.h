@@ -180,6 +180,23 @@ int fp4_rta_4ph_sample_modes(fp4_ctx *ctx, int32_t nsample, uint64_t seed, int32
                              double *rate_4ph, double *rate_pp, double *rate_pm, double *rate_mm, double *WP4_pp,
                              double *WP4_pm, double *WP4_mm, int64_t *nhits);
 
+/* Three-phonon sampling estimators: NP_driver / RTA_driver with
+   num_sample_process_3ph_phase_space / num_sample_process_3ph set
+   (Src/processes.f90:828-979, 1474-1673, dispatched at 1033-1050 and
+   1736-1760).  Each draw picks (q', j, k) uniformly from the Nb*nptk*Nb
+   candidates of the full loops; counts, phase space, Gamma and WP3 are
+   scaled as the reference does (its left-to-right order, the "-" rate
+   halved).  Streams keyed by (seed, channel, mode) as above.  A count
+   estimate beyond int32 returns FP4_ERR_OVERFLOW.  The context needs the
+   third-order constants (fp4_ctx_set_ifc3) for the rates. */
+int fp4_np_3ph_sample(fp4_ctx *ctx, int32_t nsample, uint64_t seed, int32_t *N_plus, double *Pspace_plus_total,
+                      int32_t *N_minus, double *Pspace_minus_total);
+int fp4_rta_3ph_sample(fp4_ctx *ctx, int32_t nsample, uint64_t seed, double *rate_scatt, double *rate_scatt_plus,
+                       double *rate_scatt_minus, double *WP3_plus, double *WP3_minus);
+int fp4_rta_3ph_sample_modes(fp4_ctx *ctx, int32_t nsample, uint64_t seed, int32_t nmodes, const int32_t *modes,
+                             double *rate_scatt, double *rate_scatt_plus, double *rate_scatt_minus, double *WP3_plus,
+                             double *WP3_minus, int64_t *nhits);
+
 /* ---- three-phonon path (Src/processes.f90:25-1836) ----------------------
    Channel 0 = "+" (absorption), 1 = "-" (emission).  Counts and lists are
    int32, as the reference declares them; a channel whose total would not fit

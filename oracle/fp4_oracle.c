@@ -19,6 +19,7 @@
  *   RTA_*                processes.f90:3828-4129, RTA_driver_4ph 5174-5311
  *   *_sample             processes.f90:2480-2746, 4811-5171 (sampling
  *                        estimators; the random stream is this file's own)
+ *   3ph *_sample         processes.f90:828-979, 1474-1673
  *   iteration0           iterations.f90:13-30
  *   iteration_4ph        iterations.f90:95-215 (3ph and 4ph terms)
  *   TConduct             conductivity.f90:11-37
@@ -893,12 +894,13 @@ typedef struct {
 
 typedef void (*visit3_fn)(const orc_case *c, const orc3_hit *h, void *ud);
 
-static void test3(const orc_case *c, int ch, const int32_t *q, double omega, int ii, int j, visit3_fn f, void *ud) {
-  const int nb = c->nbands;
+/* branches k_lo..k_hi of the third phonon for (ii, j) */
+static void test3(const orc_case *c, int ch, const int32_t *q, double omega, int ii, int j, int k_lo, int k_hi,
+                  visit3_fn f, void *ud) {
   const int n1 = c->ngrid[0], n2 = c->ngrid[1], n3 = c->ngrid[2];
   const int32_t *qp = c->IJK + 3 * (size_t)(ii - 1);
   const double omegap = E(c, ii, j);
-  for (int k = 1; k <= nb; k++) {
+  for (int k = k_lo; k <= k_hi; k++) {
     int t0, t1, t2;
     if (ch == 0) {
       t0 = q[0] + qp[0]; t1 = q[1] + qp[1]; t2 = q[2] + qp[2];
@@ -936,10 +938,10 @@ static void enumerate3(const orc_case *c, int ch, int mm, int order, int ii_lo, 
   if (ii_hi > nptk) ii_hi = nptk;
   if (order == 0) {
     for (int ii = ii_lo; ii <= ii_hi; ii++)
-      for (int j = 1; j <= nb; j++) test3(c, ch, q, omega, ii, j, f, ud);
+      for (int j = 1; j <= nb; j++) test3(c, ch, q, omega, ii, j, 1, nb, f, ud);
   } else {
     for (int j = 1; j <= nb; j++)
-      for (int ii = ii_lo; ii <= ii_hi; ii++) test3(c, ch, q, omega, ii, j, f, ud);
+      for (int ii = ii_lo; ii <= ii_hi; ii++) test3(c, ch, q, omega, ii, j, 1, nb, f, ud);
   }
 }
 
@@ -1231,4 +1233,119 @@ int orc3_ind_driver(const orc_case *c, int nthreads, const int32_t *N_plus, cons
   for (int m = 0; m < nm; m++) rate[m] = rate_plus[m] + rate_minus[m];
   for (int ch = 0; ch < 2; ch++) free(acc[ch]);
   return bad ? -1 : 0;
+}
+
+/* ------------------------------------------ three-phonon sampling estimators
+ * processes.f90:828-979 (NP_plus/minus_sample) and 1474-1673
+ * (RTA_plus/minus_sample), dispatched at 1033-1050 and 1736-1760.  One draw
+ * per sample picks (ii, j, k) uniformly from the Nb*nptk*Nb candidates of the
+ * full loops (index rand_num-1 = ((ii-1)*Nb + j-1)*Nb + k-1, as the
+ * reference decodes it); every candidate is one the full drivers visit, so
+ * the sums scale by Nb*nptk*Nb/nsample.  Draw s of segment (ch, mm):
+ * mix64(key + G*(s+1)), key = mix64(seed + G*(2*(mm-1)+ch+1)).
+ * ------------------------------------------------------------------------ */
+void orc3_sample_mode(const orc_case *c, int ch, int mm, int32_t nsample, uint64_t seed, int want_rates,
+                      int64_t *hits, double *P, double *gamma, double *wp3) {
+  const int nb = c->nbands, nptk = c->nptk;
+  const int i = fmodulo(mm - 1, nb) + 1;
+  const int qi = c->List[(mm - 1) / nb];
+  const int32_t *q = c->IJK + 3 * (size_t)(qi - 1);
+  const double omega = E(c, qi, i);
+  np3_acc na;
+  na.n = 0;
+  na.p = 0.0;
+  na.den = (double)((int64_t)nptk * nptk);
+  rta3_acc ra;
+  memset(&ra, 0, sizeof(ra));
+  ra.i = i;
+  ra.qi = qi;
+  ra.ch = ch;
+  ra.nb = nb;
+  ra.onlyharmonic = c->onlyharmonic;
+  if (omega != 0.0) {
+    const uint64_t key = orc_mix64(seed + ORC_GOLDEN * (uint64_t)(2 * (mm - 1) + ch + 1));
+    for (int64_t s = 0; s < nsample; s++) {
+      const int r = orc_below(orc_mix64(key + ORC_GOLDEN * (uint64_t)(s + 1)), nb * nptk * nb);
+      const int k = r % nb + 1;
+      const int j = (r / nb) % nb + 1;
+      const int ii = r / (nb * nb) + 1;
+      test3(c, ch, q, omega, ii, j, k, k, visit_np3, &na);
+      if (want_rates) test3(c, ch, q, omega, ii, j, k, k, visit_rta3, &ra);
+    }
+  }
+  *hits = na.n;
+  *P = na.p;
+  if (gamma) *gamma = ra.gamma;
+  if (wp3) *wp3 = ra.wp3;
+}
+
+/* NP_driver with num_sample_process_3ph_phase_space = nsample; N and P
+   scaled in the reference's left-to-right order (processes.f90:898-899).
+   Returns -1 when a count estimate exceeds int32. */
+int orc3_np_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int32_t *N_plus,
+                          double *P_plus, int32_t *N_minus, double *P_minus) {
+  const int nm = c->nbands * c->nlist;
+  const int nb = c->nbands, nptk = c->nptk;
+  int32_t *N[2] = {N_plus, N_minus};
+  double *P[2] = {P_plus, P_minus};
+  int bad = 0;
+  set_threads(nthreads);
+  for (int m = 0; m < nm; m++)
+    for (int ch = 0; ch < 2; ch++) {
+      N[ch][m] = 0;
+      P[ch][m] = 0.0;
+    }
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int mm = 1; mm <= nm; mm++) {
+    const int i = fmodulo(mm - 1, nb) + 1;
+    const int ll = (mm - 1) / nb + 1;
+    if (!(E(c, c->List[ll - 1], i) <= c->omega_max)) continue;
+    for (int ch = 0; ch < 2; ch++) {
+      int64_t hits;
+      double p;
+      orc3_sample_mode(c, ch, mm, nsample, seed, 0, &hits, &p, NULL, NULL);
+      const double n = (double)hits / (double)nsample * nb * nptk * nb;
+      if (n > 2147483647.0) bad |= 1;
+      N[ch][mm - 1] = (int32_t)n;
+      P[ch][mm - 1] = p * nb * nptk * nb / (double)nsample;
+    }
+  }
+  return bad ? -1 : 0;
+}
+
+/* RTA_driver with num_sample_process_3ph = nsample over a mode list
+   (processes.f90:1559-1570, 1660-1671, minus channel halved at :1759). */
+void orc3_rta_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int nmodes,
+                            const int32_t *modes, double *rate, double *rate_plus, double *rate_minus,
+                            double *wp3_plus, double *wp3_minus, int64_t *hits) {
+  const int nb = c->nbands, nptk = c->nptk;
+  double *R[2] = {rate_plus, rate_minus};
+  double *W[2] = {wp3_plus, wp3_minus};
+  int64_t nh = 0;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nh)
+  for (int t = 0; t < nmodes; t++) {
+    const int mm = modes[t];
+    const int i = fmodulo(mm - 1, nb) + 1;
+    const int ll = (mm - 1) / nb + 1;
+    const double omega = E(c, c->List[ll - 1], i);
+    if (!(omega <= c->omega_max)) continue;
+    for (int ch = 0; ch < 2; ch++) {
+      int64_t h;
+      double p, g, w;
+      orc3_sample_mode(c, ch, mm, nsample, seed, 1, &h, &p, &g, &w);
+      nh += h;
+      if (omega != 0.0) w = (ch == 0) ? w / nptk : w * 5.e-1 / nptk;
+      g = g * nb * nptk * nb / (double)nsample;
+      w = w * nb * nptk * nb / (double)nsample;
+      g = g * (double)F_GAMMA3_UNITS * 1.e8 / nptk;
+      R[ch][mm - 1] = (ch == 0) ? g : g * 5.e-1;
+      W[ch][mm - 1] = w;
+    }
+  }
+  for (int t = 0; t < nmodes; t++) {
+    const int m = modes[t] - 1;
+    rate[m] = rate_plus[m] + rate_minus[m];
+  }
+  if (hits) *hits = nh;
 }

@@ -140,6 +140,13 @@ int orc3_ind_only_driver(const orc_case *c, int nthreads, const int32_t *N_plus,
                          int32_t *i2_plus, int32_t *i3_plus, int32_t *i2_minus, int32_t *i3_minus);
 void orc3_rta_driver(const orc_case *c, int nthreads, int nmodes, const int32_t *modes, int32_t ii_lo, int32_t ii_hi,
                      double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus);
+void orc3_sample_mode(const orc_case *c, int ch, int mm, int32_t nsample, uint64_t seed, int want_rates,
+                      int64_t *hits, double *P, double *gamma, double *wp3);
+int orc3_np_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int32_t *N_plus,
+                          double *P_plus, int32_t *N_minus, double *P_minus);
+void orc3_rta_sample_driver(const orc_case *c, int nthreads, int32_t nsample, uint64_t seed, int nmodes,
+                            const int32_t *modes, double *rate, double *rate_plus, double *rate_minus,
+                            double *wp3_plus, double *wp3_minus, int64_t *hits);
 int orc3_ind_driver(const orc_case *c, int nthreads, const int32_t *N_plus, const int32_t *N_minus, int32_t *i2_plus,
                     int32_t *i3_plus, double *g_plus, int32_t *i2_minus, int32_t *i3_minus, double *g_minus,
                     double *rate, double *rate_plus, double *rate_minus, double *wp3_plus, double *wp3_minus);

@@ -259,6 +259,35 @@ class Oracle:
                                *[_ptr(o) for o in out])
         return dict(zip(["rate", "rate_plus", "rate_minus", "wp3_plus", "wp3_minus"], out))
 
+    def sample3_mode(self, ch, mm, nsample, seed, want_rates=False):
+        """Raw 3ph sums of one (channel, mode) -> (hits, P, Gamma, WP3) before unit and sampling factors."""
+        if want_rates:
+            self._need3()
+        h, p, g, w = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        self.L.orc3_sample_mode(C.byref(self.c), C.c_int(ch), C.c_int(mm), C.c_int32(nsample), C.c_uint64(seed),
+                                C.c_int(int(want_rates)), C.byref(h), C.byref(p), C.byref(g), C.byref(w))
+        return h.value, p.value, g.value, w.value
+
+    def np3_sample_driver(self, nsample, seed, nthreads=0):
+        nm = self.nmodes
+        N = [np.zeros(nm, np.int32) for _ in range(2)]
+        P = [np.zeros(nm, np.float64) for _ in range(2)]
+        rc = self.L.orc3_np_sample_driver(C.byref(self.c), C.c_int(nthreads), C.c_int32(nsample), C.c_uint64(seed),
+                                          _ptr(N[0]), _ptr(P[0]), _ptr(N[1]), _ptr(P[1]))
+        if rc != 0:
+            raise OverflowError("a 3ph count estimate exceeds int32")
+        return N, P
+
+    def rta3_sample_driver(self, nsample, seed, modes=None, nthreads=0):
+        self._need3()
+        nm = self.nmodes
+        modes = np.arange(1, nm + 1, dtype=np.int32) if modes is None else np.ascontiguousarray(modes, np.int32)
+        out = [np.zeros(nm, np.float64) for _ in range(5)]
+        hits = C.c_int64()
+        self.L.orc3_rta_sample_driver(C.byref(self.c), C.c_int(nthreads), C.c_int32(nsample), C.c_uint64(seed),
+                                      C.c_int(len(modes)), _ptr(modes), *[_ptr(o) for o in out], C.byref(hits))
+        return dict(zip(["rate", "rate_plus", "rate_minus", "wp3_plus", "wp3_minus"], out)), hits.value
+
     def ind3_driver(self, N, nthreads=0):
         self._need3()
         nm = self.nmodes

@@ -95,3 +95,38 @@ def test_bad_sample_count_is_rejected(ctx):
         c.NP_driver_4ph_sample(0, 1)
     with pytest.raises(rt.FP4Error):
         c.RTA_driver_4ph_sample(-5, 1)
+
+
+# ------------------------------------------------------------- three phonons
+@pytest.mark.parametrize("nsample,seed", [(500, 3), (24001, 99)])
+def test_np3_sample_matches_oracle(ctx, nsample, seed):
+    from oracle.oracle import Oracle
+    case, c = ctx
+    banner(f"NP_driver sampled, nsample {nsample}, seed {seed}")
+    got = c.NP_driver_sample(nsample, seed)
+    Nw, Pw = Oracle(case).np3_sample_driver(nsample, seed, nthreads=8)
+    for ch in range(2):
+        assert np.array_equal(got[2 * ch], Nw[ch]), ch
+        assert rel(got[2 * ch + 1].ravel(order="F"), Pw[ch]) <= 1e-12, ch
+
+
+def test_rta3_sample_matches_oracle_and_is_unbiased(ctx):
+    from oracle.oracle import Oracle
+    case, c = ctx
+    banner("RTA_driver sampled vs oracle and vs the full FP64 rates")
+    r, _ = c.RTA_driver_sample(5000, 8)
+    w, _ = Oracle(case).rta3_sample_driver(5000, 8, nthreads=8)
+    for dk, ok in (("rate_scatt", "rate"), ("rate_scatt_plus", "rate_plus"), ("rate_scatt_minus", "rate_minus"),
+                   ("WP3_plus", "wp3_plus"), ("WP3_minus", "wp3_minus")):
+        err = rel(r[dk].ravel(order="F"), w[ok])
+        print(f"{dk}: max rel err {err:.2e}")
+        assert err <= 1e-12, dk
+    modes = np.array([5, 17, 30, 44], np.int32)
+    full, _ = c.RTA_driver_modes(modes)
+    est = [c.RTA_driver_sample(4000, seed, modes=modes)[0] for seed in range(200, 208)]
+    for key in ("rate_scatt_plus", "rate_scatt_minus", "WP3_plus", "WP3_minus"):
+        want = full[key].ravel(order="F")[modes - 1].sum()
+        got = np.array([e[key].ravel(order="F")[modes - 1].sum() for e in est])
+        se = got.std(ddof=1) / np.sqrt(len(got))
+        print(f"{key}: full {want:.6e}, mean {got.mean():.6e}, se {se:.2e}")
+        assert want > 0 and abs(got.mean() - want) <= 5 * se, key

@@ -118,3 +118,42 @@ def test_deterministic_and_keyed_by_mode():
     h1 = o.sample_mode(1, 17, 20000, 42)[0]
     h2 = o.sample_mode(1, 17, 80000, 42)[0]
     assert 3.0 < h2 / h1 < 5.0
+
+
+# ------------------------------------------------------------- three phonons
+def test_3ph_counts_and_rates_unbiased_against_full_drivers():
+    """Every one of the Nb*nptk*Nb candidates per mode is one the full 3ph
+    loops visit, so hits/nsample estimates N/(Nb*nptk*Nb) directly."""
+    case = _case()
+    o = Oracle(case)
+    Nf, Pf = o.np3_driver(nthreads=8)
+    total, n = case.modes.nbands ** 2 * case.modes.nptk, 20000
+    for seed in SEEDS[:2]:
+        N, P = o.np3_sample_driver(n, seed, nthreads=8)
+        for ch in range(2):
+            bar = _count_bar(Nf[ch].astype(np.int64), total, n, 5.0)
+            d = abs(int(N[ch].sum()) - int(Nf[ch].sum()))
+            print(f"3ph channel {ch}: full {int(Nf[ch].sum())}, sampled {int(N[ch].sum())}, bar {bar:.0f}")
+            assert d <= bar and bar < 0.05 * Nf[ch].sum(), (ch, d, bar)
+    modes = np.array([5, 17, 30, 44], np.int32)
+    full = o.rta3_driver(modes=modes, nthreads=8)
+    est = [o.rta3_sample_driver(4000, seed, modes=modes, nthreads=8)[0] for seed in SEEDS]
+    for key in ("rate_plus", "rate_minus", "wp3_plus", "wp3_minus", "rate"):
+        want = full[key][modes - 1].sum()
+        got = np.array([e[key][modes - 1].sum() for e in est])
+        se = got.std(ddof=1) / np.sqrt(len(got))
+        print(f"3ph {key}: full {want:.6e}, mean {got.mean():.6e}, se {se:.2e}")
+        assert want > 0 and abs(got.mean() - want) <= 5 * se and se < 0.1 * want, key
+
+
+def test_3ph_sampling_with_every_candidate_drawn_often():
+    """With nsample >> Nb*nptk*Nb each candidate is drawn ~nsample/total
+    times, so the estimate converges to the full value: at nsample = 200*total
+    the relative error of the channel totals is below 1%."""
+    case = _case()
+    o = Oracle(case)
+    Nf, _ = o.np3_driver(nthreads=8)
+    total = case.modes.nbands ** 2 * case.modes.nptk
+    N, _ = o.np3_sample_driver(200 * total, 3, nthreads=8)
+    for ch in range(2):
+        assert abs(int(N[ch].sum()) - int(Nf[ch].sum())) <= 0.01 * Nf[ch].sum(), ch
