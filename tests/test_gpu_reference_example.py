@@ -85,3 +85,34 @@ def test_four_phonon_weighted_phase_space(si):
         err = _rel(r[k].ravel(order="F")[modes - 1], g["WP4"][ch][modes - 1])
         print(f"{k}: max rel {err:.2e}")
         assert err <= 2e-4, k
+
+
+def test_driver_output_files_diff_against_the_example(si, tmp_path):
+    """The device results written with the reference's file formats
+    (fourphonon_amd/formats.py) next to the example's own files
+    (tests/golden/si_files): BTE.Numprocess_4ph with the ++ and -- columns
+    identical line by line, and the 3ph phase-space files and totals at the
+    printing precision."""
+    import os
+    from fourphonon_amd import formats
+    g, case, c = si
+    files = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "si_files")
+    m = case.modes
+    N4 = c.NP_driver_4ph()
+    formats.write_numprocess_4ph(os.path.join(tmp_path, "BTE.Numprocess_4ph"), N4[0], N4[2], N4[4], m.Nlist,
+                                 m.nbands)
+    got = [ln.split() for ln in open(os.path.join(tmp_path, "BTE.Numprocess_4ph"))]
+    want = [ln.split() for ln in open(os.path.join(files, "BTE.Numprocess_4ph"))]
+    assert len(got) == len(want)
+    assert all(a[0] == b[0] and a[2] == b[2] for a, b in zip(got, want))
+    assert sum(a[1] != b[1] for a, b in zip(got, want)) <= 60
+    N3 = c.NP_driver()
+    formats.write_phase_space_3ph(str(tmp_path), N3[1], N3[3], m.Nequi)
+    for name in ("BTE.P3_plus", "BTE.P3_minus", "BTE.P3"):
+        a = np.array(formats.read_table(os.path.join(tmp_path, name)))
+        b = np.array(formats.read_table(os.path.join(files, name)))
+        assert _rel(a.ravel(), b.ravel()) <= 1e-8, name
+    for name in ("BTE.P3_total", "BTE.P3_plus_total", "BTE.P3_minus_total"):
+        a = formats.read_table(os.path.join(tmp_path, name))[0][0]
+        b = formats.read_table(os.path.join(files, name))[0][0]
+        assert abs(a / b - 1) <= 1e-9, name
