@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape instead of the autotuner's")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
                     help="MFMA form of the timed steps: FP32 MFMA or BF16 hi/lo MFMA (FP32 accumulate)")
     ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16"], default="rta16",
@@ -169,6 +170,13 @@ def main():
 
     # ---- timed dense (MFMA) steps over the whole workload
     prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    tuned = None
+    if not args.no_autotune:
+        # kernel shape from the autotuner, on every 29th mode over q' = 1..64 (untimed)
+        t1 = time.perf_counter()
+        tuned = ctx.autotune_dense(prec, all_modes[::29], ii_lo=1, ii_hi=64)
+        tuned["seconds"] = time.perf_counter() - t1
+        log(f"autotune: IB {tuned['ib']}, TQ {tuned['tq']}, {tuned['slots']} blocks ({tuned['seconds']:.1f}s)")
     for w in range(args.warmup):
         ctx.RTA_driver_4ph_modes(all_modes, prec)
         log(f"warmup {w + 1}/{args.warmup} done")
@@ -209,6 +217,7 @@ def main():
                                "all 870 modes", "ngrid": [16, 16, 16], "nbands": 6, "nlist": int(m.Nlist),
                    "ntri4": int(case.ifc4.Ntri), "processes": nproc_total, "triples": ntriples},
         "prefilter_seconds": t_pref,
+        "autotune": tuned,
         "roofline": roofline,
     }
 

@@ -1866,6 +1866,51 @@ int fp4_ctx_dense_info(fp4_ctx *c, int64_t *triples, int64_t *items, int32_t *ng
   return FP4_OK;
 }
 
+int fp4_ctx_autotune_dense(fp4_ctx *c, int32_t precision, int32_t nmodes, const int32_t *modes, int32_t ii_lo,
+                           int32_t ii_hi, int32_t *ib, int32_t *tq, int32_t *slots, double *best_ms) {
+  int rc;
+  if ((rc = check_ready(c, true))) return rc;
+  if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP64_DENSE && precision != FP4_PREC_BF16X3)
+    return fail(FP4_ERR_ARG, "autotune: precision %d has no dense kernel", precision);
+  if (ii_lo < 1 || ii_hi > c->nptk || ii_hi < ii_lo) return fail(FP4_ERR_ARG, "bad ii window %d..%d", ii_lo, ii_hi);
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  // The kernels share work across the branches of one irreducible point
+  // (IB modes per block), so the sample is widened to every branch of each
+  // point it touches; a one-branch-per-point sample would favour small IB.
+  {
+    std::vector<int> all;
+    for (int mm : pl.modes) {
+      const int first = (mm - 1) / c->nb * c->nb + 1;
+      if (all.empty() || all.back() < first)
+        for (int b = 0; b < c->nb; b++) all.push_back(first + b);
+    }
+    pl.modes = all;
+  }
+  DenseRun dr;
+  dr.m = mode_view(c);
+  dr.roots = c->roots.as<double2>();
+  dr.T = c->T;
+  dr.onlyharmonic = c->onlyharmonic;
+  dr.natoms = c->natoms;
+  dr.precision = precision;
+  dr.stream = c->stream;
+  dr.tuning = nullptr;
+  DenseTuning best;
+  double ms = 0.0;
+  int ncand = 0;
+  rc = dense_autotune(c->dense, dr, pl.modes, ii_lo - 1, ii_hi - ii_lo + 1, &best, &ms, &ncand);
+  if (rc) return fail(rc, "autotune: %s", dense_error());
+  c->dense_tuning = best;
+  int a, b, d;
+  dense_shape(c->dense, &a, &b, &d);
+  if (ib) *ib = d;
+  if (tq) *tq = b;
+  if (slots) *slots = best.slots;
+  if (best_ms) *best_ms = ms;
+  return FP4_OK;
+}
+
 int fp4_ctx_dense_shape(fp4_ctx *c, int32_t *mfma, int32_t *tq, int32_t *ib) {
   if (!c || !mfma || !tq || !ib) return fail(FP4_ERR_ARG, "null argument");
   int a, b, d;

@@ -1150,6 +1150,7 @@ struct Shape {
   size_t (*bytes)(int, int, int);
   size_t (*stat)();
   void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
+  int (*per_cu)(size_t);   // resident blocks per CU at this dynamic LDS size
 };
 
 template <class T, int NB, int IB, int TQ, int MF>
@@ -1159,8 +1160,20 @@ void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
   hipLaunchKernelGGL((k_dense<T, NB, IB, TQ, MF>), dim3(slots), dim3(NT_DENSE), dyn, st, a);
 }
 
-#define FP4_SHAPE(T, P, NB, IB, TQ, MF) \
-  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>}
+template <class T, int NB, int IB, int TQ, int MF>
+int per_cu_shape(size_t dyn) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF>),
+                                                   NT_DENSE, dyn) != hipSuccess)
+    return 1;
+  return std::max(n, 1);
+}
+
+#define FP4_SHAPE(T, P, NB, IB, TQ, MF)                                                                        \
+  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>, \
+   &per_cu_shape<T, NB, IB, TQ, MF>}
 
 // In order of preference; the first shape whose LDS fits is used.  The FP32
 // shapes with 8 q'' per tile run the Psi GEMM on MFMA.
@@ -1204,7 +1217,13 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   if (s->natoms > 4) return derr(FP4_ERR_UNSUPPORTED, "dense form: more than 4 atoms per cell");
-  const Shape *shp = pick_shape(r.precision, nb, s->ng, s->nc, s->L, (size_t)ldsmax);
+  const bool tuned = r.tuning && r.tuning->shape_prec == r.precision;
+  const Shape *shp = nullptr;
+  if (tuned && r.tuning->shape >= 0 && r.tuning->shape < (int)(sizeof(kShapes) / sizeof(kShapes[0]))) {
+    const Shape &c = kShapes[r.tuning->shape];
+    if (c.prec == r.precision && c.nb == nb && c.bytes(s->ng, s->nc, s->L) <= (size_t)ldsmax) shp = &c;
+  }
+  if (!shp) shp = pick_shape(r.precision, nb, s->ng, s->nc, s->L, (size_t)ldsmax);
   if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
   const Shape &sh = *shp;
   s->last_mfma = sh.mfma;
@@ -1222,7 +1241,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   const long nitems = 3L * nls * nib * nii;
   int ncu = 0;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  int slots = (r.tuning && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu);
+  int slots = (tuned && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu);
   if (slots > nitems) slots = (int)std::max(1L, nitems);
   int rc;
   const size_t per_item = sizeof(double) * 2 + sizeof(int);
@@ -1337,6 +1356,54 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
           w[ch * nsub + it->second] = hw[sidx];
           n[ch * nsub + it->second] = hn[sidx];
         }
+  return 0;
+}
+
+int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
+                   DenseTuning *out, double *best_ms, int *ncand) {
+  if (!s || !out) return derr(FP4_ERR_ARG, "autotune: null argument");
+  int dev = 0, ldsmax = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int nshapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+  DenseTuning best;
+  double tbest = 0.0;
+  int n = 0;
+  std::vector<double> g, w;
+  std::vector<long long> cnt;
+  for (int i = 0; i < nshapes; i++) {
+    const Shape &c = kShapes[i];
+    if (c.prec != r.precision || c.nb != r.m.nb) continue;
+    const size_t dyn = c.bytes(s->ng, s->nc, s->L);
+    if (dyn > (size_t)ldsmax) continue;
+    const int per_cu = std::min(c.per_cu(dyn), 2);
+    for (int k = 1; k <= per_cu; k++) {
+      DenseTuning t;
+      t.shape = i;
+      t.shape_prec = r.precision;
+      t.slots = k * std::max(ncu, 1);
+      DenseRun rr = r;
+      rr.tuning = &t;
+      double ms = 0.0;
+      int rc;
+      // the first run of a shape pays its one-time set-up; the second is timed
+      if ((rc = dense_rates(s, rr, modes, ii_lo0, nii, g, w, cnt, &ms))) return rc;
+      if ((rc = dense_rates(s, rr, modes, ii_lo0, nii, g, w, cnt, &ms))) return rc;
+      n++;
+      if (best.shape < 0 || ms < tbest) {
+        best = t;
+        tbest = ms;
+      }
+    }
+  }
+  if (best.shape < 0) return derr(FP4_ERR_UNSUPPORTED, "autotune: no kernel shape fits this Nbands / cluster set");
+  *out = best;
+  if (best_ms) *best_ms = tbest;
+  if (ncand) *ncand = n;
+  s->last_mfma = kShapes[best.shape].mfma;
+  s->last_tq = kShapes[best.shape].TQ;
+  s->last_ib = kShapes[best.shape].IB;
   return 0;
 }
 

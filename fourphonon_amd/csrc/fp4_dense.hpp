@@ -28,7 +28,9 @@ struct DCluster {
 };
 
 struct DenseTuning {
-  int slots = 0;          // resident blocks of the persistent kernel; 0 = 4 per CU
+  int slots = 0;          // resident blocks of the persistent kernel; 0 = one per CU
+  int shape = -1;         // kernel shape (IB, TQ, MFMA path) chosen by dense_autotune; -1 = first that fits
+  int shape_prec = -1;    // the precision the shape and slots were tuned for
 };
 
 struct DenseState;        // device copies of the grouped clusters (opaque)
@@ -61,6 +63,15 @@ struct DenseRun {
 // number of processes.
 int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
                 std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms);
+// The project's autotuner: runs every built kernel shape of r.precision and
+// Nbands that fits in LDS, at one and, where occupancy allows, two resident
+// blocks per CU, on the given modes and q' window, and stores the fastest
+// (by the kernel's HIP-event time) in *out.  *ncand receives the number of
+// candidates timed, best_ms the winner's kernel time.  Rates computed with
+// different shapes agree to rounding (the per-item sums are regrouped).
+int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
+                   DenseTuning *out, double *best_ms, int *ncand);
+
 const char *dense_error();
 
 }  // namespace fp4

@@ -40,7 +40,7 @@ EXPORTS = [
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
     "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
-    "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes",
+    "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense",
 ]
 
 
@@ -401,6 +401,16 @@ class Context:
         g, f = C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_info(self.h, C.byref(t), C.byref(it), C.byref(g), C.byref(f)))
         return {"triples": t.value, "items": it.value, "ngroups": g.value, "nfine": f.value}
+
+    def autotune_dense(self, precision, modes, ii_lo=1, ii_hi=None):
+        """Runs the dense-form autotuner on a mode subset and q' window -> dict(ib, tq, slots, ms)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        ib, tq, sl = C.c_int32(), C.c_int32(), C.c_int32()
+        ms = C.c_double()
+        _check(load().fp4_ctx_autotune_dense(self.h, C.c_int32(precision), C.c_int32(len(modes)), _p(modes),
+                                             C.c_int32(ii_lo), C.c_int32(self.nptk if ii_hi is None else ii_hi),
+                                             C.byref(ib), C.byref(tq), C.byref(sl), C.byref(ms)))
+        return {"ib": ib.value, "tq": tq.value, "slots": sl.value, "ms": ms.value}
 
     def dense_shape(self):
         """-> dict(mfma, tq, ib) of the last dense-form call."""

@@ -369,6 +369,19 @@ int fp4_ctx_dense_info(fp4_ctx *ctx, int64_t *triples, int64_t *items, int32_t *
 /* Diagnostics; no counterpart in the reference.  Shape of the last dense
    kernel: MFMA path (1) or VALU path (0), q'' per tile, modes per block. */
 int fp4_ctx_dense_shape(fp4_ctx *ctx, int32_t *mfma, int32_t *tq, int32_t *ib);
+/* The dense form's autotuner; no counterpart in the reference (its CPU loops
+   have no tile shapes).  Times every built kernel shape of this precision and
+   Nbands that fits in LDS (modes per block IB, q'' per tile TQ, MFMA path),
+   at one and, where occupancy allows, two resident blocks per CU, on every
+   branch of the irreducible points of the given strictly increasing mode
+   subset (the kernels share work across the branches of a point) over the
+   q' window ii_lo..ii_hi, and
+   keeps the fastest for this context's later dense-form calls of that
+   precision.  Outputs (each may be NULL): the chosen IB, TQ, resident block
+   count and its kernel time in ms.  Rates from different shapes agree to
+   rounding. */
+int fp4_ctx_autotune_dense(fp4_ctx *ctx, int32_t precision, int32_t nmodes, const int32_t *modes, int32_t ii_lo,
+                           int32_t ii_hi, int32_t *ib, int32_t *tq, int32_t *slots, double *best_ms);
 /* Diagnostics; no counterpart in the reference.  Runs one 16x16x4 FP32 MFMA
    with known operands and returns the largest deviation from the exact
    product, checking the lane layout the dense kernel relies on. */

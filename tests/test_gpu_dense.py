@@ -155,3 +155,27 @@ def test_dense_golden_wurtzite_12x12x8_window():
         if prec != rt.PREC_FP64_DENSE:
             assert c.dense_shape()["mfma"] == (1 if prec == rt.PREC_FP32 else 2)
     c.close()
+
+
+@pytest.mark.parametrize("prec", [rt.PREC_BF16X3, rt.PREC_FP32, rt.PREC_FP64_DENSE], ids=["bf16x3", "fp32", "fp64"])
+def test_autotuner_picks_a_built_shape_and_keeps_the_rates(prec):
+    """The autotuner times the built shapes on a sample and keeps the fastest;
+    later calls run that shape and give the same rates as the default shape
+    (to rounding) and the FP64 form's within the form's bar."""
+    case = crystal.config_case(1, ngrid=(4, 4, 4), shell=2)
+    c = rt.Context(case)
+    try:
+        modes = np.arange(1, c.nmodes + 1, dtype=np.int32)
+        before, _ = c.RTA_driver_4ph_modes(modes, prec)
+        banner(f"autotune precision {prec}")
+        t = c.autotune_dense(prec, modes[::5], ii_lo=1, ii_hi=16)
+        print(f"chosen IB {t['ib']}, TQ {t['tq']}, slots {t['slots']}, {t['ms']:.3f} ms")
+        assert t["ib"] in (1, 2, 3, 6) and t["tq"] in (2, 4, 8) and t["slots"] >= 1 and t["ms"] > 0
+        after, _ = c.RTA_driver_4ph_modes(modes, prec)
+        assert c.dense_shape()["ib"] == t["ib"] and c.dense_shape()["tq"] == t["tq"]
+        assert rel_err(after["rate_scatt_4ph"], before["rate_scatt_4ph"]) <= 1e-9
+        ref, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
+        bar = 1e-9 if prec == rt.PREC_FP64_DENSE else 1e-3
+        assert rel_err(after["rate_scatt_4ph"], ref["rate_scatt_4ph"]) <= bar
+    finally:
+        c.close()
