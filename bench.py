@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix peak
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0    # dense BF16 matrix peak
+FP16_PEAK_TFLOPS = 2500.0    # dense FP16 matrix peak
 
 
 def log(*a):
@@ -129,8 +130,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
     ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape instead of the autotuner's")
-    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="bf16x3",
-                    help="MFMA form of the timed steps: FP32 MFMA or BF16 hi/lo MFMA (FP32 accumulate)")
+    ap.add_argument("--precision", choices=["fp32", "bf16x3", "fp16x3"], default="bf16x3",
+                    help="MFMA form of the timed steps: FP32 MFMA, BF16 hi/lo or FP16 hi/lo MFMA (FP32 accumulate)")
     ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16"], default="rta16",
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
                          "4ph RTA + 3ph iterative kappa at 100/300/700 K; sample16: the sampling estimator on "
@@ -169,7 +170,7 @@ def main():
     log(f"prefilter: {nproc_total} processes in {t_pref:.2f}s")
 
     # ---- timed dense (MFMA) steps over the whole workload
-    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
     tuned = None
     if not args.no_autotune:
         # kernel shape from the autotuner, on every 29th mode over q' = 1..64 (untimed)
@@ -199,8 +200,8 @@ def main():
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
     shape = ctx.dense_shape()
-    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{['valu', 'mfma', 'mfma-bf16x3'][shape['mfma']]}>"
-    peak = FP32_PEAK_TFLOPS if args.precision == "fp32" else BF16_PEAK_TFLOPS
+    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{['valu', 'mfma', 'mfma-bf16x3', 'mfma-fp16x3'][shape['mfma']]}>"
+    peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS, "fp16x3": FP16_PEAK_TFLOPS}[args.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname),
                 "kernel": kname, "kernel_ms": round(avg_ms, 3), "launches": k_n,
@@ -212,7 +213,8 @@ def main():
         "metric": "4ph processes/s on 1 MI355X, MFMA form vs FP64 form, kappa within 1e-4 of FP64",
         "value": value, "unit": "processes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32" if args.precision == "fp32" else "bf16x3 (bf16 hi/lo MFMA, fp32 accumulate)", "data": "synthetic",
+        "dtype": {"fp32": "fp32", "bf16x3": "bf16x3 (bf16 hi/lo MFMA, fp32 accumulate)",
+                  "fp16x3": "fp16x3 (fp16 hi/lo MFMA, fp32 accumulate)"}[args.precision], "data": "synthetic",
         "config": {"workload": "diamond 16x16x16, 2nd-shell 4th-order IFCs (Ntri=1522), 4ph RTA at 300 K, "
                                "all 870 modes", "ngrid": [16, 16, 16], "nbands": 6, "nlist": int(m.Nlist),
                    "ntri4": int(case.ifc4.Ntri), "processes": nproc_total, "triples": ntriples},
@@ -264,23 +266,26 @@ def main():
         out["kappa_rel_err"] = float(abs(np.trace(k32) - np.trace(k64)) / abs(np.trace(k64)))
         log(f"rate max rel err {out['rate_max_rel_err']:.2e}, kappa rel err {out['kappa_rel_err']:.2e}")
 
-        # the other MFMA form, one pass over the whole workload
-        other = "fp32" if args.precision == "bf16x3" else "bf16x3"
-        oprec = rt.PREC_FP32 if other == "fp32" else rt.PREC_BF16X3
-        ctx.reset_stats()
-        t1 = time.perf_counter()
-        ro, no = ctx.RTA_driver_4ph_modes(all_modes, oprec)
-        to = time.perf_counter() - t1
-        o_ms, _ = ctx.kernel_stats(rt.Context.KS_DENSE)
-        opeak = FP32_PEAK_TFLOPS if other == "fp32" else BF16_PEAK_TFLOPS
-        oach = dense_algorithmic_flops(case, ctx.dense_info()["triples"]) / (o_ms * 1e-3) / 1e12
-        ao = ro["rate_scatt_4ph"].ravel(order="F")
-        out["other_mfma_form"] = {
-            "precision": other, "value": no / to, "unit": "processes/s", "kernel_ms": o_ms,
-            "achieved_tflops": round(oach, 3), "peak": opeak, "frac": round(oach / opeak, 4),
-            "rate_max_rel_err": float(np.max(np.abs(ao[nz] - b[nz]) / np.abs(b[nz]))),
-            "kappa_rel_err": float(abs(np.trace(kappa_rta(case, ao)) - np.trace(k64)) / abs(np.trace(k64)))}
-        log(f"{other} form {no / to:.3e} proc/s")
+        # the other MFMA forms, one pass each over the whole workload
+        forms = {"fp32": (rt.PREC_FP32, FP32_PEAK_TFLOPS), "bf16x3": (rt.PREC_BF16X3, BF16_PEAK_TFLOPS),
+                 "fp16x3": (rt.PREC_FP16X3, FP16_PEAK_TFLOPS)}
+        out["other_mfma_forms"] = []
+        for other, (oprec, opeak) in forms.items():
+            if other == args.precision:
+                continue
+            ctx.reset_stats()
+            t1 = time.perf_counter()
+            ro, no = ctx.RTA_driver_4ph_modes(all_modes, oprec)
+            to = time.perf_counter() - t1
+            o_ms, _ = ctx.kernel_stats(rt.Context.KS_DENSE)
+            oach = dense_algorithmic_flops(case, ctx.dense_info()["triples"]) / (o_ms * 1e-3) / 1e12
+            ao = ro["rate_scatt_4ph"].ravel(order="F")
+            out["other_mfma_forms"].append({
+                "precision": other, "value": no / to, "unit": "processes/s", "kernel_ms": o_ms,
+                "achieved_tflops": round(oach, 3), "peak": opeak, "frac": round(oach / opeak, 4),
+                "rate_max_rel_err": float(np.max(np.abs(ao[nz] - b[nz]) / np.abs(b[nz]))),
+                "kappa_rel_err": float(abs(np.trace(kappa_rta(case, ao)) - np.trace(k64)) / abs(np.trace(k64)))})
+            log(f"{other} form {no / to:.3e} proc/s, rate max rel err {out['other_mfma_forms'][-1]['rate_max_rel_err']:.2e}")
 
     # ---- three-phonon companion on the same crystal (2nd-shell FC3): counts
     # and FP64 per-process rates, all modes; reported beside the headline
@@ -332,7 +337,7 @@ def run_sample16(args):
 
     case = crystal.config_case(2)
     ctx = rt.Context(case)
-    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
     t1 = time.perf_counter()
     full = ctx.RTA_driver_4ph(prec)["rate_scatt_4ph"].ravel(order="F")
     t_full = time.perf_counter() - t1
@@ -375,7 +380,7 @@ def run_iterative20(args):
     ctx = rt.Context(case)
     nm = ctx.nmodes
     all_modes = np.arange(1, nm + 1, dtype=np.int32)
-    prec = rt.PREC_BF16X3 if args.precision == "bf16x3" else rt.PREC_FP32
+    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
     N4 = [np.zeros(nm, np.int64) for _ in range(3)]
     L4 = [[np.zeros(1, np.int64)] * 3 for _ in range(3)]
     G4 = [np.zeros(1)] * 3

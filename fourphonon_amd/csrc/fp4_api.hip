@@ -793,7 +793,8 @@ int fp4_rta_4ph_modes(fp4_ctx *c, int32_t precision, int32_t nmodes, const int32
       if ((rc = emit_batch(c, pl, b, false))) return rc;
       if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
     }
-  } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3) {
+  } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3 ||
+             precision == FP4_PREC_FP16X3) {
     DenseRun dr;
     dr.m = mode_view(c);
     dr.roots = c->roots.as<double2>();
@@ -1870,7 +1871,8 @@ int fp4_ctx_autotune_dense(fp4_ctx *c, int32_t precision, int32_t nmodes, const 
                            int32_t ii_hi, int32_t *ib, int32_t *tq, int32_t *slots, double *best_ms) {
   int rc;
   if ((rc = check_ready(c, true))) return rc;
-  if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP64_DENSE && precision != FP4_PREC_BF16X3)
+  if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP64_DENSE && precision != FP4_PREC_BF16X3 &&
+      precision != FP4_PREC_FP16X3)
     return fail(FP4_ERR_ARG, "autotune: precision %d has no dense kernel", precision);
   if (ii_lo < 1 || ii_hi > c->nptk || ii_hi < ii_lo) return fail(FP4_ERR_ARG, "bad ii window %d..%d", ii_lo, ii_hi);
   Plan pl;

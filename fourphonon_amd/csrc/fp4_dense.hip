@@ -72,6 +72,7 @@ struct DenseState {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   long long last_triples = 0, last_items = 0;
   int last_mfma = 0, last_tq = 0, last_ib = 0;
+  float a_scale = 1.0f;    // FP16 form: 2^k with |A'| * a_scale <= 2^14 for every group
 };
 
 template <class T>
@@ -108,6 +109,7 @@ struct DenseArgs {
   int smem_static;       // bytes of DenseSmem at the start of dynamic LDS
   float bf[9];           // rlattvec(:,nu)/ngrid(nu) as floats
   float sbf;             // scalebroad
+  float a_scale;         // FP16 form: power of two with |A'| * a_scale <= 2^14
 };
 
 template <class T>
@@ -261,6 +263,57 @@ __global__ void k_mfma_bf16_selftest(double *err) {
   if (l == 0) *err = e;
 }
 
+// FP16 operands (the FP16 hi/lo form): round-to-nearest-even binary16 bits.
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned f16_rn(float f) {
+  return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)f);
+}
+__device__ __forceinline__ float f16_val(unsigned h) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+}
+// f = hi + lo + O(2^-22 |f|) for |f| in the normal binary16 range
+__device__ __forceinline__ void f16_split(float f, unsigned &hi, unsigned &lo) {
+  hi = f16_rn(f);
+  lo = f16_rn(f - f16_val(hi));
+}
+__device__ __forceinline__ f16x4 f16_pack(unsigned w0, unsigned w1) {
+  f16x4 v;
+  v[0] = __builtin_bit_cast(_Float16, (unsigned short)(w0 & 0xFFFFu));
+  v[1] = __builtin_bit_cast(_Float16, (unsigned short)(w0 >> 16));
+  v[2] = __builtin_bit_cast(_Float16, (unsigned short)(w1 & 0xFFFFu));
+  v[3] = __builtin_bit_cast(_Float16, (unsigned short)(w1 >> 16));
+  return v;
+}
+
+// v_mfma_f32_16x16x16f16: the same lane layout as the BF16 instruction,
+// checked the same way.
+__global__ void k_mfma_f16_selftest(double *err) {
+  const int l = threadIdx.x;
+  const int i = l & 15, kg = l >> 4;
+  f16x4 av, bv;
+  for (int q = 0; q < 4; q++) {
+    const int k = 4 * kg + q;
+    av[q] = (_Float16)(float)(1 + i + 3 * k);
+    bv[q] = (_Float16)(float)(2 + 5 * (l & 15) - k);
+  }
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc, 0, 0, 0);
+  double e = 0.0;
+  for (int r = 0; r < 4; r++) {
+    const int ii = 4 * (l >> 4) + r, jj = l & 15;
+    double want = 0.0;
+    for (int k = 0; k < 16; k++) want += (double)(1 + ii + 3 * k) * (double)(2 + 5 * jj - k);
+    e = fmax(e, fabs(want - (double)acc[r]));
+  }
+  for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+  if (l == 0) *err = e;
+}
+
+// Scale of the phase operand in the FP16 form: |root| <= 1 becomes <= 2^14,
+// well inside binary16's range, with the lo part normal for components above
+// ~2^-10.
+constexpr float kPhaseScaleF16 = 16384.0f;
+
 template <class T, int NB, int IB, int TQ>
 struct DenseSmem {
   static constexpr int NOUT2 = IB * NB * 9;   // A' entries per group = Psi entries per (K,L) and triple
@@ -297,7 +350,7 @@ struct DenseSmem {
   double red[NT_DENSE / 64][3 * IB];
 };
 
-// MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi.
+// MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi, 3 = FP16 hi/lo MFMA Psi.
 template <class T, int NB, int IB, int TQ, int MF>
 __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   using S = DenseSmem<T, NB, IB, TQ>;
@@ -308,6 +361,10 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   constexpr int NW = NT_DENSE / 64;
   constexpr int QPT = QMAX / NT_DENSE;        // survivors per thread in a contraction
   static_assert(MF == 0 || (sizeof(T) == 4 && 2 * TQ == 16), "MFMA path: float, 8 q'' per tile");
+  // FP16 form: A' is scaled by a_scale (a power of two that bounds it by 2^14)
+  // and the phases by kPhaseScaleF16 before the split; Psi is unscaled after
+  // the GEMM, exactly (powers of two)
+  const float psi_unscale = (MF == 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
   static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
   constexpr unsigned long long kJall = (1ull << NB) - 1ull;   // bits of all j of one i
   constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
@@ -466,7 +523,15 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           __syncthreads();
         }
         if (tid < NOUT2) {
-          if constexpr (MF == 2) {
+          if constexpr (MF == 3) {
+            // hi plane and lo plane as in the BF16 form, binary16 values of A' * a_scale
+            unsigned rh, rl, ih, il;
+            f16_split((float)acc.x * a.a_scale, rh, rl);
+            f16_split((float)acc.y * a.a_scale, ih, il);
+            unsigned *bh = reinterpret_cast<unsigned *>(s_A);
+            bh[(size_t)gi * NOUT2 + tid] = rh | (ih << 16);
+            bh[(size_t)a.ng * NOUT2 + (size_t)gi * NOUT2 + tid] = rl | (il << 16);
+          } else if constexpr (MF == 2) {
             unsigned rh, rl, ih, il;
             bf16_split((float)acc.x, rh, rl);
             bf16_split((float)acc.y, ih, il);
@@ -629,11 +694,17 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               const int kk = fmod_pos(k, a.L);
               rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
             }
-            if constexpr (MF == 2) {
-              // A operand rows (t, part), k = (g, qq): part 0 (re, -im), part 1 (im, re), hi and lo planes
+            if constexpr (MF == 2 || MF == 3) {
+              // A operand rows (t, part), k = (g, qq): part 0 (re, -im), part 1 (im, re), hi and lo
+              // planes; the sign flip is the top bit in both 16-bit formats
               unsigned rh, rl, ih, il;
-              bf16_split((float)rt.x, rh, rl);
-              bf16_split((float)rt.y, ih, il);
+              if constexpr (MF == 3) {
+                f16_split((float)rt.x * kPhaseScaleF16, rh, rl);
+                f16_split((float)rt.y * kPhaseScaleF16, ih, il);
+              } else {
+                bf16_split((float)rt.x, rh, rl);
+                bf16_split((float)rt.y, ih, il);
+              }
               unsigned *ph = reinterpret_cast<unsigned *>(s_phA);
               unsigned *pl = ph + (size_t)TQ * a.ng * 2;
               ph[2 * x] = rh | ((ih ^ 0x8000u) << 16);
@@ -731,10 +802,11 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                     }
                   }
                 }
-              } else if constexpr (MF == 2) {
-                // BF16 hi/lo form: Psi = A_hi B_hi + A_hi B_lo + A_lo B_hi on
-                // v_mfma_f32_16x16x16bf16_1k, FP32 accumulation; a k-step is
-                // 8 groups, lane group kg holding groups 2kg and 2kg+1
+              } else if constexpr (MF == 2 || MF == 3) {
+                // BF16 (FP16) hi/lo form: Psi = A_hi B_hi + A_hi B_lo + A_lo B_hi
+                // on v_mfma_f32_16x16x16bf16_1k (v_mfma_f32_16x16x16f16), FP32
+                // accumulation; a k-step is 8 groups, lane group kg holding
+                // groups 2kg and 2kg+1
                 constexpr int NTILE = (NOUT2 + 15) / 16;
                 constexpr int TPW = (NTILE + NW - 1) / NW;
                 const int i = lane & 15, t = i >> 1, part = i & 1, kg = lane >> 4;
@@ -758,18 +830,28 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                   const bool v0 = g < g1, v1 = g + 1 < g1;
                   const int gc0 = v0 ? g : g0, gc1 = v1 ? g + 1 : g0;
                   const unsigned pa = (t * a.ng + gc0) * 2 + part, pb = (t * a.ng + gc1) * 2 + part;
-                  const bf16x4 ah = bf16_pack(v0 ? Ph[pa] : 0u, v1 ? Ph[pb] : 0u);
-                  const bf16x4 al = bf16_pack(v0 ? Pl[pa] : 0u, v1 ? Pl[pb] : 0u);
+                  const unsigned ph0 = v0 ? Ph[pa] : 0u, ph1 = v1 ? Ph[pb] : 0u;
+                  const unsigned pl0 = v0 ? Pl[pa] : 0u, pl1 = v1 ? Pl[pb] : 0u;
 #pragma unroll
                   for (int c = 0; c < TPW; c++) {
                     if (wave + NW * c >= NTILE) continue;
                     const size_t b0 = (size_t)gc0 * NOUT2 + ncol[c], b1 = (size_t)gc1 * NOUT2 + ncol[c];
                     const bool w0 = v0 && nin[c], w1 = v1 && nin[c];
-                    const bf16x4 bh = bf16_pack(w0 ? Bh[b0] : 0u, w1 ? Bh[b1] : 0u);
-                    const bf16x4 bl = bf16_pack(w0 ? Bl[b0] : 0u, w1 ? Bl[b1] : 0u);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[c], 0, 0, 0);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[c], 0, 0, 0);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[c], 0, 0, 0);
+                    const unsigned bh0 = w0 ? Bh[b0] : 0u, bh1 = w1 ? Bh[b1] : 0u;
+                    const unsigned bl0 = w0 ? Bl[b0] : 0u, bl1 = w1 ? Bl[b1] : 0u;
+                    if constexpr (MF == 3) {
+                      const f16x4 ah = f16_pack(ph0, ph1), al = f16_pack(pl0, pl1);
+                      const f16x4 bh = f16_pack(bh0, bh1), bl = f16_pack(bl0, bl1);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[c], 0, 0, 0);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[c], 0, 0, 0);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[c], 0, 0, 0);
+                    } else {
+                      const bf16x4 ah = bf16_pack(ph0, ph1), al = bf16_pack(pl0, pl1);
+                      const bf16x4 bh = bf16_pack(bh0, bh1), bl = bf16_pack(bl0, bl1);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[c], 0, 0, 0);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[c], 0, 0, 0);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[c], 0, 0, 0);
+                    }
                   }
                 }
 #pragma unroll
@@ -780,9 +862,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                     for (int r = 0; r < 4; r++) {
                       const int row = 4 * (lane >> 4) + r;
                       if ((row & 1) == 0)
-                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)acc[c][r];
+                        s.u.psi[(row >> 1) * NOUT2 + n].re = (T)(acc[c][r] * psi_unscale);
                       else
-                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)acc[c][r];
+                        s.u.psi[(row >> 1) * NOUT2 + n].im = (T)(acc[c][r] * psi_unscale);
                     }
                   }
                 }
@@ -1068,6 +1150,25 @@ DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<do
   }
   s->ng = (int)G.size();
   s->nf = (int)F.size();
+  {
+    // |A'_g[i][j][s][r]| <= sum_{c in g} minv_c sum_{u,t} |Phi_c[u,t,s,r]|
+    // (eigenvector components and phases have modulus <= 1)
+    double bmax = 0.0;
+    for (const DGroup &g : G)
+      for (int sr = 0; sr < 9; sr++) {
+        double b = 0.0;
+        for (int fi = g.f_first; fi < g.f_first + g.f_count; fi++)
+          for (int c = F[fi].c_first; c < F[fi].c_first + F[fi].c_count; c++) {
+            double x = 0.0;
+            for (int ut = 0; ut < 9; ut++) x += std::fabs(P[(size_t)c * 81 + ut + 9 * sr]);
+            b += std::fabs(Cs[c].minv) * x;
+          }
+        bmax = std::max(bmax, b);
+      }
+    int e = 0;
+    if (bmax > 0.0) (void)std::frexp(bmax, &e);   // bmax < 2^e
+    s->a_scale = (float)std::ldexp(1.0, 14 - e);
+  }
   s->nc = n;
   bool ok = hipMalloc(&s->g, sizeof(DGroup) * G.size()) == hipSuccess &&
             hipMalloc(&s->f, sizeof(FGroup) * F.size()) == hipSuccess &&
@@ -1113,7 +1214,13 @@ int dense_selftest_mfma(double *err) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(&e16, d, sizeof(double), hipMemcpyDeviceToHost);
-  *err = e32 > e16 ? e32 : e16;
+  double ef16 = 0.0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_f16_selftest, dim3(1), dim3(64), 0, 0, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&ef16, d, sizeof(double), hipMemcpyDeviceToHost);
+  *err = std::max(std::max(e32, e16), ef16);
   (void)hipFree(d);
   return e == hipSuccess ? 0 : derr(FP4_ERR_DEVICE, "selftest: kernel failed");
 }
@@ -1146,7 +1253,7 @@ size_t lds_bytes(int ng, int ntri, int L) {
 // One kernel shape: modes per block (IB) and q'' per tile (TQ).
 struct Shape {
   int prec, nb, IB, TQ;
-  int mfma;   // 0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA
+  int mfma;   // 0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA, 3 FP16 hi/lo MFMA
   size_t (*bytes)(int, int, int);
   size_t (*stat)();
   void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
@@ -1184,6 +1291,9 @@ const Shape kShapes[] = {
     FP4_SHAPE(float, FP4_PREC_BF16X3, 3, 3, 8, 2),           FP4_SHAPE(float, FP4_PREC_BF16X3, 6, 6, 8, 2),
     FP4_SHAPE(float, FP4_PREC_BF16X3, 6, 2, 8, 2),           FP4_SHAPE(float, FP4_PREC_BF16X3, 9, 3, 8, 2),
     FP4_SHAPE(float, FP4_PREC_BF16X3, 12, 1, 8, 2),
+    FP4_SHAPE(float, FP4_PREC_FP16X3, 3, 3, 8, 3),           FP4_SHAPE(float, FP4_PREC_FP16X3, 6, 6, 8, 3),
+    FP4_SHAPE(float, FP4_PREC_FP16X3, 6, 2, 8, 3),           FP4_SHAPE(float, FP4_PREC_FP16X3, 9, 3, 8, 3),
+    FP4_SHAPE(float, FP4_PREC_FP16X3, 12, 1, 8, 3),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 3, 3, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 3, 4, 0),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 1, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 9, 1, 2, 0),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, 0),
@@ -1210,7 +1320,8 @@ int ensure(void *&p, size_t &bytes, size_t need) {
 int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
                 std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms) {
   if (!s) return derr(FP4_ERR_STATE, "dense state missing");
-  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3)
+  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3 &&
+      r.precision != FP4_PREC_FP16X3)
     return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
   const int nb = r.m.nb;
   int dev = 0, ldsmax = 0;
@@ -1307,6 +1418,8 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
     for (int nu = 0; nu < 3; nu++)
       for (int c = 0; c < 3; c++) a.bf[3 * nu + c] = (float)(r.m.b[3 * nu + c] / ng3[nu]);
     a.sbf = (float)std::fabs(r.m.scalebroad);
+    a.a_scale = s->a_scale;
+  a.a_scale = s->a_scale;
   }
   a.out_g = og;
   a.out_w = ow;

@@ -38,7 +38,7 @@ module fp4_processes
   private
 
   integer(c_int32_t), parameter, public :: FP4_PREC_FP64 = 0, FP4_PREC_FP64_DENSE = 1, &
-       FP4_PREC_FP32 = 2, FP4_PREC_BF16X3 = 3
+       FP4_PREC_FP32 = 2, FP4_PREC_BF16X3 = 3, FP4_PREC_FP16X3 = 4
 
   type, bind(C) :: fp4_config
      integer(c_int32_t) :: ngrid(3)

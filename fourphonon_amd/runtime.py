@@ -25,6 +25,7 @@ PREC_FP64 = 0
 PREC_FP64_DENSE = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
+PREC_FP16X3 = 4
 
 ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR_OVERFLOW", 5: "FP4_ERR_STATE",
           6: "FP4_ERR_UNSUPPORTED"}
@@ -416,7 +417,7 @@ class Context:
         """-> dict(mfma, tq, ib) of the last dense-form call."""
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"mfma": a.value, "tq": b.value, "ib": c.value}   # mfma: 0 VALU, 1 FP32, 2 BF16 hi/lo
+        return {"mfma": a.value, "tq": b.value, "ib": c.value}   # mfma: 0 VALU, 1 FP32, 2 BF16 hi/lo, 3 FP16 hi/lo
 
     # ------------------------------------------------------- three phonons
     _RATE3 = ["rate_scatt", "rate_scatt_plus", "rate_scatt_minus", "WP3_plus", "WP3_minus"]

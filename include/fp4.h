@@ -42,7 +42,8 @@ enum {
   FP4_PREC_FP64 = 0,       /* per-process FP64, reference operation order */
   FP4_PREC_FP64_DENSE = 1, /* dense fold + contraction, FP64 */
   FP4_PREC_FP32 = 2,       /* dense fold + contraction on FP32 MFMA */
-  FP4_PREC_BF16X3 = 3      /* dense, BF16 hi/lo split (3 products) on MFMA, FP32 accumulate */
+  FP4_PREC_BF16X3 = 3,     /* dense, BF16 hi/lo split (3 products) on MFMA, FP32 accumulate */
+  FP4_PREC_FP16X3 = 4      /* dense, FP16 hi/lo split (3 products, power-of-two operand scaling) on MFMA */
 };
 
 typedef struct fp4_ctx fp4_ctx;
@@ -367,7 +368,8 @@ int fp4_ctx_reset_stats(fp4_ctx *ctx);
    (K, L, R_k-R_l) and fine (I, J, K, L, R_k-R_l) cluster groups. */
 int fp4_ctx_dense_info(fp4_ctx *ctx, int64_t *triples, int64_t *items, int32_t *ngroups, int32_t *nfine);
 /* Diagnostics; no counterpart in the reference.  Shape of the last dense
-   kernel: MFMA path (1) or VALU path (0), q'' per tile, modes per block. */
+   kernel: its Psi GEMM path (0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA, 3 FP16
+   hi/lo MFMA), q'' per tile, modes per block. */
 int fp4_ctx_dense_shape(fp4_ctx *ctx, int32_t *mfma, int32_t *tq, int32_t *ib);
 /* The dense form's autotuner; no counterpart in the reference (its CPU loops
    have no tile shapes).  Times every built kernel shape of this precision and

@@ -105,7 +105,7 @@ def test_batch_golden(c):
             assert_close(r[key].ravel(order="F")[rm - 1], g[wk][c], RTOL, key)
         if case.structure.natoms <= 4:
             want = g["rate_4ph"][c]
-            for prec in (rt.PREC_FP32, rt.PREC_BF16X3):
+            for prec in (rt.PREC_FP32, rt.PREC_BF16X3, rt.PREC_FP16X3):
                 rd, nd = ctx.RTA_driver_4ph_modes([rm], prec, ii_lo=lo, ii_hi=hi)
                 got = rd["rate_scatt_4ph"].ravel(order="F")[rm - 1]
                 assert nd == n64

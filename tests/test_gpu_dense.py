@@ -87,13 +87,15 @@ def test_dense_fp32_tolerances(pair):
     assert kerr <= 1e-4
 
 
-def test_dense_bf16x3_tolerances(pair):
-    """BF16 hi/lo MFMA form: per-mode rates within 1e-3 and kappa within 1e-4 of FP64."""
+@pytest.mark.parametrize("prec,mf", [(rt.PREC_BF16X3, 2), (rt.PREC_FP16X3, 3)], ids=["bf16x3", "fp16x3"])
+def test_dense_split_form_tolerances(pair, prec, mf):
+    """BF16 and FP16 hi/lo MFMA forms: per-mode rates within 1e-3 and kappa
+    within 1e-4 of FP64."""
     case, c, o = pair
-    banner("dense BF16x3 vs FP64")
+    banner(f"dense hi/lo form {prec} vs FP64")
     ref = c.RTA_driver_4ph(rt.PREC_FP64)
-    got = c.RTA_driver_4ph(rt.PREC_BF16X3)
-    assert c.dense_shape()["mfma"] == 2
+    got = c.RTA_driver_4ph(prec)
+    assert c.dense_shape()["mfma"] == mf
     err = rel_err(got["rate_scatt_4ph"], ref["rate_scatt_4ph"])
     print(f"per-mode rate max rel err {err:.3e}")
     assert err <= 1e-3
@@ -112,7 +114,7 @@ def test_dense_golden_diamond_8cube():
     banner("dense vs golden diamond 8x8x8")
     c = rt.Context(crystal.config_case(1))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
         r, n = c.RTA_driver_4ph_modes(rm, prec)
         assert n == int(g["N"][:, rm - 1].sum())
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
@@ -129,7 +131,7 @@ def test_dense_golden_diamond_16cube_window():
     banner("dense vs golden diamond 16x16x16 shell 2 (q' window)")
     c = rt.Context(crystal.config_case(2))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
         r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
                         ("rate_scatt_minusminus", "rate_mm")):
@@ -147,17 +149,18 @@ def test_dense_golden_wurtzite_12x12x8_window():
     banner("dense vs golden wurtzite 12x12x8 shell 2 (q' window)")
     c = rt.Context(crystal.config_case(3))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
         r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
                         ("rate_scatt_minusminus", "rate_mm")):
             assert rel_err(r[key].ravel(order="F")[rm - 1], g[wk]) <= tol, (prec, key)
         if prec != rt.PREC_FP64_DENSE:
-            assert c.dense_shape()["mfma"] == (1 if prec == rt.PREC_FP32 else 2)
+            assert c.dense_shape()["mfma"] == {rt.PREC_FP32: 1, rt.PREC_BF16X3: 2, rt.PREC_FP16X3: 3}[prec]
     c.close()
 
 
-@pytest.mark.parametrize("prec", [rt.PREC_BF16X3, rt.PREC_FP32, rt.PREC_FP64_DENSE], ids=["bf16x3", "fp32", "fp64"])
+@pytest.mark.parametrize("prec", [rt.PREC_BF16X3, rt.PREC_FP16X3, rt.PREC_FP32, rt.PREC_FP64_DENSE],
+                         ids=["bf16x3", "fp16x3", "fp32", "fp64"])
 def test_autotuner_picks_a_built_shape_and_keeps_the_rates(prec):
     """The autotuner times the built shapes on a sample and keeps the fastest;
     later calls run that shape and give the same rates as the default shape
