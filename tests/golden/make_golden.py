@@ -71,6 +71,23 @@ def part_cfg1():
                         rate_threads=THREADS, **{k: v[rmodes - 1] for k, v in r.items()})
 
 
+def part_cfg1k():
+    """cfg1 as the workload defines it: 4ph and 3ph RTA rates of every mode
+    (the RTA kappa follows from them)."""
+    case = crystal.config_case(1)
+    o = Oracle(case)
+    allm = np.arange(1, o.nmodes + 1, dtype=np.int32)
+    log("cfg1k: 3ph rates for all", len(allm), "modes")
+    r3 = o.rta3_driver(modes=allm, nthreads=THREADS)
+    log("cfg1k: 4ph rates for all", len(allm), "modes")
+    t0 = time.time()
+    r4 = o.rta_driver(modes=allm, nthreads=THREADS)
+    log("cfg1k: 4ph rates took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "cfg1k.npz"), ngrid=np.array([8, 8, 8]), shell=1, workload=1,
+                        rate_seconds=time.time() - t0, rate_threads=THREADS,
+                        **{"r4_" + k: v for k, v in r4.items()}, **{"r3_" + k: v for k, v in r3.items()})
+
+
 def part_cfg2():
     case = crystal.config_case(2)
     o = Oracle(case)
