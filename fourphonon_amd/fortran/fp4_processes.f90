@@ -120,6 +120,15 @@ module fp4_processes
        integer(c_int64_t), intent(in) :: Npp(*), Npm(*), Nmm(*)
        integer(c_int64_t), intent(out) :: a2(*), a3(*), a4(*), b2(*), b3(*), b4(*), c2(*), c3(*), c4(*)
      end function c_ind_only
+     integer(c_int) function c_autotune(ctx, prec, nm, modes, lo, hi, ib, tq, sl, ms) &
+          bind(C, name="fp4_ctx_autotune_dense")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: prec, nm, lo, hi
+       integer(c_int32_t), intent(in) :: modes(*)
+       integer(c_int32_t), intent(out) :: ib, tq, sl
+       real(c_double), intent(out) :: ms
+     end function c_autotune
      integer(c_int) function c_rta(ctx, prec, r4, rpp, rpm, rmm, wpp, wpm, wmm) bind(C, name="fp4_rta_4ph")
        import :: c_int, c_ptr, c_double, c_int32_t
        type(c_ptr), value :: ctx
@@ -278,7 +287,7 @@ module fp4_processes
   real(c_double), allocatable, target, save :: masses_(:)
   integer(c_int32_t), allocatable, target, save :: types_(:)
 
-  public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision, fp4_set_sampling
+  public :: fp4_init, fp4_finalize, fp4_set_temperature, fp4_set_precision, fp4_set_sampling, fp4_autotune
   public :: NP_driver_4ph, Ind_only_driver_4ph, RTA_driver_4ph, RTA_driver_4ph_GPU_using_Ind, Ind_driver_4ph
   public :: Vp_pp, Vp_pm, Vp_mm
   public :: NP_driver, Ind_only_driver, RTA_driver, RTA_driver_GPU_using_Ind, Ind_driver, Vp_plus, Vp_minus
@@ -353,6 +362,25 @@ contains
     integer(kind=4), intent(in) :: p
     precision = p
   end subroutine fp4_set_precision
+
+  ! The dense form's autotuner for the current precision, on every stride-th
+  ! irreducible point (all its branches) over q' = 1..min(64, nptk); call it
+  ! after the first driver call has uploaded the modes and constants.  The
+  ! chosen shape is kept for the later RTA_driver_4ph calls.
+  subroutine fp4_autotune(Nlist, stride)
+    integer(kind=4), intent(in) :: Nlist, stride
+    integer(c_int32_t), allocatable :: modes(:)
+    integer(c_int32_t) :: ib, tq, sl
+    real(c_double) :: ms
+    integer :: ll, n
+    allocate(modes((Nlist + max(stride, 1) - 1) / max(stride, 1)))
+    n = 0
+    do ll = 1, Nlist, max(stride, 1)
+       n = n + 1
+       modes(n) = (ll - 1) * nbands_ + 1
+    end do
+    call check(c_autotune(ctx, precision, n, modes, 1, min(64, nptk_), ib, tq, sl, ms), "fp4_autotune")
+  end subroutine fp4_autotune
 
   ! The reference's four sampling switches, in the order of Src/config.f90:23
   ! (-1: the full drivers; n > 0: n sampled candidates per channel and mode),
