@@ -103,6 +103,12 @@ program fp4_driver
   allocate(wpp(nbands, nlist), wpm(nbands, nlist), wmm(nbands, nlist))
   call RTA_driver_4ph(energy, velocity, eigenvect, nlist, List, IJK, ntri, Phi, R_j, R_k, R_l, &
        Index_i, Index_j, Index_k, Index_l, r4, rpp, rpm, rmm, wpp, wpm, wmm)
+  if (trim(mode) == "fp32") then
+     ! the dense kernel's shape from the autotuner, then the pass again with it
+     call fp4_autotune(nlist, 4)
+     call RTA_driver_4ph(energy, velocity, eigenvect, nlist, List, IJK, ntri, Phi, R_j, R_k, R_l, &
+          Index_i, Index_j, Index_k, Index_l, r4, rpp, rpm, rmm, wpp, wpm, wmm)
+  end if
   call write_w(trim(outdir)//"/BTE.w_4ph", r4)
   call write_w(trim(outdir)//"/BTE.w_4ph_plusplus", rpp)
   call write_w(trim(outdir)//"/BTE.w_4ph_plusminus", rpm)
