@@ -41,7 +41,10 @@
  * closed forms, invariants, dense linear algebra; tests/test_oracle*.py),
  * and the three-phonon drivers also against the reference's own Si example
  * run (counts equal to its totals, phase space and rates to its ten printed
- * digits; tests/test_reference_example.py).
+ * digits; tests/test_reference_example.py).  The four-phonon rate assembly
+ * (prefactor, unit constant, 1/nptk^2, occupation factors, the call to Vp_*)
+ * is rebuilt in numpy from the definition, with an independent numpy V and
+ * the unit constants derived from CODATA (tests/test_oracle_rates4.py).
  * Parity unpinned: which candidates exactly at the window edge the
  * reference's compiled code keeps (this file follows its expression order and
  * rounding, but only a run of the reference could confirm ties), and the

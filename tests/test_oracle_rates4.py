@@ -1,0 +1,215 @@
+"""Pins the oracle's four-phonon rate assembly by an independent route.
+
+The reference ships no fourth-order constants with its examples, so no
+reference vector holds a 4ph rate.  Two checks stand in for one:
+
+* the unit constants.  The reference states the units of Gamma before its
+  conversion factor (processes.f90:3336-3339 for 4ph, :430-432 for 3ph):
+  (1e-34 J s)^2 (1e12/s)^-5 amu^-4 (eV/A^4)^2 = 3.37617087e9 THz and
+  (1e-34 J s) (1e12/s)^-4 amu^-3 (eV/A^3)^2 = 5.60626442e8 THz.  Both
+  literals are derived here from CODATA values, so a mistyped literal
+  fails.
+* the assembly.  Per mode and channel, the rate is rebuilt in numpy as
+  3.37617087(f32)*1e9/nptk^2 * sum over processes of
+  hbarp^2*pi/8 * WP4 * |V|^2 (processes.f90:3898-3922, 3991-4025,
+  4098-4127).  The process list comes from the vectorised enumeration of
+  ``tests/test_oracle.py`` (not the oracle's loops).  sigma, its clamp, the
+  Bose-Einstein factors, the occupation combination of each channel and the
+  low-frequency cut are all evaluated in numpy.  V is evaluated in numpy
+  too, as one tensor contraction over all clusters and processes of a mode
+  (processes.f90:170-201 restated: the mass prefactor, the phases of the
+  second, third and fourth phonons with the channel's signs, conj on the
+  channel's phonons); on a sample of processes it must also equal the
+  oracle's ``Vp_*``, which the closed form and invariants of
+  ``tests/test_oracle.py`` pin.
+
+A factor of two in the prefactor, one missing 1/nptk, the third and fourth
+phonons swapped in the call to ``Vp_*``, or the wrong occupation combination
+in one channel moves the oracle's rate far outside the 1e-10 tolerance used
+below.
+"""
+import numpy as np
+import pytest
+
+from fourphonon_amd import crystal
+from oracle.oracle import Oracle
+from tests.test_oracle import vectorised_list
+
+HBAR, KB = 1.05457172647e-22, 1.380648813e-23          # data.f90:5-8
+HBARP = HBAR * float(np.float32(1e22))                  # processes.f90:20, 1e22 is default REAL
+UNITS4 = float(np.float32(3.37617087))                  # processes.f90:3922, default REAL literal
+
+
+# ---------------------------------------------------------------------------
+# unit constants from CODATA
+# ---------------------------------------------------------------------------
+
+# Atomic mass unit in kg and electron volt in J of three CODATA editions.
+# The 3ph literal (ShengBTE's) carries the digits of the 2010 values, the
+# 4ph literal (FourPhonon's) those of the 2014 values.
+CODATA = {"2010": (1.660538921e-27, 1.602176565e-19),
+          "2014": (1.660539040e-27, 1.6021766208e-19),
+          "2018": (1.66053906660e-27, 1.602176634e-19)}
+
+
+def _gamma_units_4ph(amu, ev):
+    """(1e-34 J s)^2 (1e12/s)^-5 amu^-4 (eV/A^4)^2, expressed in THz."""
+    per_s = (1e-34) ** 2 * (1e-12) ** 5 * amu ** -4 * (ev / (1e-10) ** 4) ** 2
+    return per_s / 1e12
+
+
+def _gamma_units_3ph(amu, ev):
+    """(1e-34 J s) (1e12/s)^-4 amu^-3 (eV/A^3)^2, expressed in THz."""
+    per_s = 1e-34 * (1e-12) ** 4 * amu ** -3 * (ev / (1e-10) ** 3) ** 2
+    return per_s / 1e12
+
+
+@pytest.mark.parametrize("edition", sorted(CODATA))
+def test_unit_constants_follow_from_codata(edition):
+    amu, ev = CODATA[edition]
+    assert abs(_gamma_units_4ph(amu, ev) / 3.37617087e9 - 1) <= 1e-6
+    assert abs(_gamma_units_3ph(amu, ev) / 5.60626442e8 - 1) <= 1e-6
+    # every printed digit: nine significant digits leave at most 1.5e-9
+    if edition == "2014":
+        assert abs(_gamma_units_4ph(amu, ev) / 3.37617087e9 - 1) <= 1.5e-9
+    if edition == "2010":
+        assert abs(_gamma_units_3ph(amu, ev) / 5.60626442e8 - 1) <= 2e-8
+    # hbar in J/THz is CODATA's hbar in J s times 1e12
+    assert abs(HBAR / (1.054571817e-34 * 1e12) - 1) <= 1e-7
+    assert abs(KB / 1.380649e-23 - 1) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# matrix elements and rates rebuilt in numpy
+# ---------------------------------------------------------------------------
+
+# per channel: signs of the second and third phonons' phases (the fourth is
+# always -), and which of phonons 2, 3, 4 enter conjugated
+CHANNEL_SIGNS = {0: (1, 1, (False, False, True)),
+                 1: (1, -1, (False, True, True)),
+                 2: (-1, -1, (True, True, True))}
+
+
+def numpy_vp(case, ch, i, q0, j, k, l, ii, jj, ss, chunk=2048):
+    """V of the processes (ii,j), (jj,k), (ss,l) of mode (q0, i), all indices 1-based."""
+    m, f, st = case.modes, case.ifc4, case.structure
+    ev = m.eigenvect
+    mass = st.masses[st.types - 1]
+    I, J, K, L = (np.asarray(x, np.int64) - 1 for x in (f.Index_i, f.Index_j, f.Index_k, f.Index_l))
+    pref = 1.0 / np.sqrt(mass[I] * mass[J] * mass[K] * mass[L])
+    comp = lambda atoms: 3 * atoms[:, None] + np.arange(3)[None, :]          # (Ntri, 3)
+    e1 = ev[q0 - 1, i - 1][comp(I)]                                          # (Ntri, 3)
+    phi = np.moveaxis(np.asarray(f.Phi), -1, 0)                              # (Ntri, u, t, s, r)
+    A = np.einsum("cutsr,cu->ctsr", phi, e1) * pref[:, None, None, None]
+    s2, s3, conj = CHANNEL_SIGNS[ch]
+    realq = (st.rlattvec @ (m.IJK / m.ngrid[:, None])).T                     # (nptk, 3) nm^-1
+    out = np.empty(len(j), np.complex128)
+    for a in range(0, len(j), chunk):
+        sl = slice(a, a + chunk)
+        e2 = ev[ii[sl, None, None] - 1, j[sl, None, None] - 1, comp(J)[None]]
+        e3 = ev[jj[sl, None, None] - 1, k[sl, None, None] - 1, comp(K)[None]]
+        e4 = ev[ss[sl, None, None] - 1, l[sl, None, None] - 1, comp(L)[None]]
+        e2, e3, e4 = (np.conj(e) if c else e for e, c in zip((e2, e3, e4), conj))
+        arg = s2 * realq[ii[sl] - 1] @ f.R_j + s3 * realq[jj[sl] - 1] @ f.R_k - realq[ss[sl] - 1] @ f.R_l
+        t = np.einsum("ctsr,ncr->ncts", A, e4)
+        t = np.einsum("ncts,ncs->nct", t, e3)
+        inner = np.einsum("nct,nct->nc", t, e2)
+        out[sl] = np.sum(np.exp(1j * arg) * inner, axis=1)
+    return out
+
+
+def numpy_rate(case, o, ch, mm, n_vp_check=4):
+    """(Gamma, WP4, number of processes) of one mode and channel, from the definition."""
+    m = case.modes
+    nb = m.nbands
+    i = (mm - 1) % nb + 1
+    q0 = int(m.List[(mm - 1) // nb])
+    w = m.energy[q0 - 1, i - 1]
+    lst = vectorised_list(case, ch, mm)
+    if w == 0.0 or lst.shape[1] == 0:
+        return 0.0, 0.0, 0
+    ii, j = (lst[0] - 1) // nb + 1, (lst[0] - 1) % nb + 1
+    jj, k = (lst[1] - 1) // nb + 1, (lst[1] - 1) % nb + 1
+    ss, l = (lst[2] - 1) // nb + 1, (lst[2] - 1) % nb + 1
+    wp, wdp, wtp = m.energy[ii - 1, j - 1], m.energy[jj - 1, k - 1], m.energy[ss - 1, l - 1]
+    sig = _sigma(case, m.velocity[jj - 1, k - 1] - m.velocity[ss - 1, l - 1])
+    sig = np.where(sig <= float(np.float32(0.001)), 1.0 / np.sqrt(np.pi), sig)
+    T = case.T
+    fp, fdp, ftp = (1.0 / np.expm1(HBAR * x / KB / T) for x in (wp, wdp, wtp))
+    if ch == 0:
+        delta = w + wp + wdp - wtp
+        occ = fp * fdp * (1 + ftp) - (1 + fp) * (1 + fdp) * ftp
+    elif ch == 1:
+        delta = w + wp - wdp - wtp
+        occ = fp * (1 + fdp) * (1 + ftp) - (1 + fp) * fdp * ftp
+    else:
+        delta = w - wp - wdp - wtp
+        occ = (1 + fp) * (1 + fdp) * (1 + ftp) - fp * fdp * ftp
+    wp4 = occ * np.exp(-delta ** 2 / sig ** 2) / (sig * np.sqrt(np.pi) * w * wp * wdp * wtp)
+    wp4 = np.where((wp <= 1.25) | (wdp <= 1.25) | (wtp <= 1.25), 0.0, wp4)
+    V = numpy_vp(case, ch, i, q0, j, k, l, ii, jj, ss)
+    rng = np.random.default_rng(1000 * ch + mm)
+    vmax = np.abs(V).max()
+    for p in rng.choice(len(V), size=min(n_vp_check, len(V)), replace=False):
+        want = o.vp(ch, i, j[p], k[p], l[p], q0, ii[p], jj[p], ss[p])
+        assert abs(V[p] - want) <= 1e-12 * vmax, (ch, mm, int(p))
+    gamma = np.sum(HBARP ** 2 * np.pi / 8.0 * wp4 * np.abs(V) ** 2) * UNITS4 * 1e9 / m.nptk ** 2
+    return gamma, np.sum(wp4) / m.nptk ** 2, lst.shape[1]
+
+
+def _sigma(case, dv):
+    """scalebroad * base_sigma(dv) for an array of velocity differences (config.f90:481-499)."""
+    b = case.structure.rlattvec
+    ng = case.modes.ngrid
+    comps = [((dv @ b[:, nu]) / float(ng[nu])) ** 2 for nu in range(3)]
+    return case.scalebroad * np.sqrt(np.maximum(np.maximum(comps[0], comps[1]), comps[2]) / 2.0)
+
+
+def _check_rates(case, modes, min_processes):
+    o = Oracle(case)
+    modes = np.asarray(modes, np.int32)
+    rta = o.rta_driver(modes=modes)
+    keys = (("rate_pp", "wp4_pp"), ("rate_pm", "wp4_pm"), ("rate_mm", "wp4_mm"))
+    total = 0
+    nonzero = 0
+    for mm in modes:
+        for ch, (rk, wk) in enumerate(keys):
+            g, wp4, n = numpy_rate(case, o, ch, int(mm))
+            total += n
+            assert rta[rk][mm - 1] == pytest.approx(g, rel=1e-10, abs=1e-300), (ch, int(mm))
+            assert rta[wk][mm - 1] == pytest.approx(wp4, rel=1e-10, abs=1e-300), (ch, int(mm))
+            nonzero += g > 0
+        want = sum(rta[rk][mm - 1] for rk, _ in keys)
+        assert rta["rate_4ph"][mm - 1] == pytest.approx(want, rel=1e-14, abs=1e-300)
+    assert total >= min_processes and nonzero >= len(modes)
+
+
+def test_rta_rates_rebuilt_in_numpy_diamond_333_all_modes():
+    case = crystal.config_case(1, ngrid=(3, 3, 3), shell=1)
+    nmodes = case.modes.nbands * case.modes.Nlist
+    _check_rates(case, np.arange(1, nmodes + 1), min_processes=300000)
+
+
+def test_rta_rates_rebuilt_in_numpy_diamond_234():
+    """A mesh with three different edges and no wedge: every q is its own point."""
+    case = crystal.config_case(1, ngrid=(2, 3, 4), shell=1, symmetric=False)
+    nmodes = case.modes.nbands * case.modes.Nlist
+    _check_rates(case, np.arange(1, nmodes + 1, 11), min_processes=100000)
+
+
+def test_rta_rates_rebuilt_in_numpy_second_shell():
+    """Second-shell clusters, where R_k and R_l differ in many clusters."""
+    case = crystal.config_case(1, ngrid=(3, 3, 3), shell=2)
+    _check_rates(case, [16], min_processes=5000)
+
+
+def test_rta_rates_rebuilt_in_numpy_wurtzite():
+    case = crystal.config_case(3, ngrid=(3, 3, 2), shell=1)
+    nb = case.modes.nbands
+    _check_rates(case, [4, nb + 5, 3 * nb + 12], min_processes=1000)
+
+
+def test_rta_rates_rebuilt_in_numpy_other_temperature():
+    """The Bose-Einstein factors enter through T: 100 K, and scalebroad 0.5."""
+    case = crystal.config_case(1, ngrid=(3, 3, 3), shell=1, T=100.0, scalebroad=0.5)
+    _check_rates(case, np.arange(1, case.modes.nbands * case.modes.Nlist + 1, 3), min_processes=100)
