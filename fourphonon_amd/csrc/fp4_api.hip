@@ -1923,6 +1923,45 @@ int fp4_ctx_dense_shape(fp4_ctx *c, int32_t *mfma, int32_t *tq, int32_t *ib) {
   return FP4_OK;
 }
 
+int fp4_ctx_dense_candidates(fp4_ctx *c, int32_t cap, int32_t *kg, int32_t *ib, int32_t *tq, int32_t *slots,
+                             double *ms) {
+  if (!c || cap < 0) return -FP4_ERR_ARG;
+  std::vector<int> k(cap), i(cap), t(cap), sl(cap);
+  const int n = dense_candidates(c->dense, cap, k.data(), i.data(), t.data(), sl.data(), ms);
+  for (int x = 0; x < n && x < cap; x++) {
+    if (kg) kg[x] = k[x];
+    if (ib) ib[x] = i[x];
+    if (tq) tq[x] = t[x];
+    if (slots) slots[x] = sl[x];
+  }
+  return n;
+}
+
+int fp4_ctx_dense_phase_profile(fp4_ctx *c, int32_t cap, double *cycles) {
+  if (!c || cap < 0 || (cap > 0 && !cycles)) return -FP4_ERR_ARG;
+  const int n = dense_phase_profile(c->dense, cap, cycles);
+  if (n < 0) fail(-n, "dense phase profile: this library was built without FP4_PHASE_PROF");
+  return n;
+}
+
+int fp4_ctx_set_dense_shape(fp4_ctx *c, int32_t precision, int32_t kg, int32_t ib, int32_t slots) {
+  if (!c) return fail(FP4_ERR_ARG, "null argument");
+  if (!c->dense) return fail(FP4_ERR_STATE, "set_dense_shape: fourth-order constants not set");
+  DenseTuning t;
+  const int rc = dense_pick(c->dense, precision, c->nb, kg, ib, slots, &t);
+  if (rc) return fail(rc, "%s", dense_error());
+  c->dense_tuning = t;
+  return FP4_OK;
+}
+
+int fp4_ctx_dense_kernel(fp4_ctx *c, int32_t *kg) {
+  if (!c || !kg) return fail(FP4_ERR_ARG, "null argument");
+  int a, b, d, k = 0;
+  dense_shape(c->dense, &a, &b, &d, &k);
+  *kg = k;
+  return FP4_OK;
+}
+
 int fp4_selftest_mfma(int32_t device, double *max_err) {
   if (!max_err) return fail(FP4_ERR_ARG, "null argument");
   int ndev = 0;

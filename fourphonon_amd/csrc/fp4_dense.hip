@@ -26,7 +26,7 @@
 // Scheduling: one persistent 512-thread block per CU pulls work items from
 // an atomic counter.  A', the per-cluster coefficients, the phase table and
 // a tile of Psi live in LDS (up to ~150 KiB of gfx950's 160 KiB); the kernel
-// shape (modes per block, q'' per tile) is the first one in kShapes whose
+// shape (modes per block, q'' per tile) is the first one in the shape list whose
 // LDS fits.  Results are written per item and reduced over q' in a fixed
 // order, and each block's partial sums are reduced in a fixed tree order, so
 // the output does not depend on which block ran which item.
@@ -40,10 +40,10 @@
 
 #include "../../include/fp4.h"
 #include "fp4_dense.hpp"
+#include "fp4_dense_dev.hpp"
 
 namespace fp4 {
 
-constexpr int NT_DENSE = 512;
 
 namespace {
 thread_local std::string d_err;
@@ -71,81 +71,24 @@ struct DenseState {
   int *counter = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   long long last_triples = 0, last_items = 0;
-  int last_mfma = 0, last_tq = 0, last_ib = 0;
+  int last_mfma = 0, last_tq = 0, last_ib = 0, last_kg = 0;
+  struct Cand {
+    int shape, slots;
+    double ms;
+  };
+  std::vector<Cand> cands;   // the last autotune's timings
+  unsigned long long *prof = nullptr;   // profiling build: phase cycle sums of the last k_dense2 launch
+  unsigned long long last_prof[kDensePhases] = {};
   float a_scale = 1.0f;    // FP16 form: 2^k with |A'| * a_scale <= 2^14 for every group
 };
 
-template <class T>
-struct C2 {
-  T re, im;
-};
-
-struct DenseArgs {
-  ModeView m;
-  const double2 *roots;
-  int L;
-  const DGroup *g;
-  const FGroup *f;
-  const DCluster *c;
-  const double *phi;
-  int klf[17];           // first coarse group of each (K, L) pair, then ng
-  const double *fbose;   // [Nb][nptk] Bose-Einstein factor of every mode at T
-  int roots_lds;         // the root table is copied to LDS (L <= kRootsLds)
-  int ng;
-  const int *lls;        // selected irreducible points (0-based ll), ascending
-  int nls;
-  int nib;               // mode blocks per q
-  int ii_lo0, nii;
-  const unsigned char *sel;  // [Nb*Nlist] selected modes
-  double T;
-  int onlyharmonic;
-  long nitems;
-  int *counter;
-  double *out_g;
-  double *out_w;
-  int *out_n;
-  int *out_t;            // [nitems] (q, q', q'') triples with >= 1 surviving process
-  int ntri;
-  int smem_static;       // bytes of DenseSmem at the start of dynamic LDS
-  float bf[9];           // rlattvec(:,nu)/ngrid(nu) as floats
-  float sbf;             // scalebroad
-  float a_scale;         // FP16 form: power of two with |A'| * a_scale <= 2^14
-};
-
-template <class T>
-__device__ __forceinline__ C2<T> cm(C2<T> a, C2<T> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ double2 zm(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// block-wide exclusive scan of per-thread counts in thread order (NW waves)
-template <int NW>
-__device__ __forceinline__ int block_excl_scan(int v, int *s_wsum, int &tot) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int incl_w = wave_excl_scan(v, lane) + v;
-  if (lane == 63) s_wsum[wv] = incl_w;
-  __syncthreads();
-  int base = 0;
-  tot = 0;
-#pragma unroll
-  for (int w = 0; w < NW; w++) {
-    if (w < wv) base += s_wsum[w];
-    tot += s_wsum[w];
-  }
-  __syncthreads();
-  return base + incl_w - v;
-}
 
 // Dynamic LDS behind DenseSmem: A' [ng][NOUT2], the cluster coefficients
 // [ntri], the tile's phase operands, the root table (when L <= kRootsLds) and
 // the coarse groups' d vectors; each region starts on a 16-byte boundary.
-constexpr int kRootsLds = 256;
 struct DynLayout {
   size_t A, c, ph, rt, gd, end;
 };
-__host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 template <class T, int NB, int IB, int TQ, int MF>
 __host__ __device__ DynLayout dyn_layout(size_t stat, int ng, int ntri, int L_lds) {
   DynLayout d{};
@@ -159,40 +102,6 @@ __host__ __device__ DynLayout dyn_layout(size_t stat, int ng, int ntri, int L_ld
   return d;
 }
 
-// Bit mask with bit i*nb set for every i < ib.
-constexpr unsigned long long rep_mask(int ib, int nb) {
-  unsigned long long r = 0ull;
-  for (int i = 0; i < ib; i++) r |= 1ull << (i * nb);
-  return r;
-}
-
-// Velocities of the tile's third and fourth phonons live in LDS, so sigma is
-// recomputed here with the same operations as pair_sigma().
-__device__ __forceinline__ double tile_sigma(const ModeView &m, int ch, const double *v3, const double *v4) {
-  double d[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) d[a] = (ch == 0) ? (-v3[a] + v4[a]) : (v3[a] - v4[a]);
-  return m.scalebroad * base_sigma(m, d[0], d[1], d[2]);
-}
-
-// Float estimate of sigma for the cull: b_nu / N_nu and scalebroad as floats,
-// velocity differences formed in FP64 and rounded once.  Its relative error
-// is below 1e-6, far inside the 1e-4 band that is sent to the exact test.
-__device__ __forceinline__ float sigma_estimate(const float *bf, int ch, const double *v3, const double *v4,
-                                                float sbf) {
-  float d[3];
-#pragma unroll
-  for (int a = 0; a < 3; a++) d[a] = (float)((ch == 0) ? (-v3[a] + v4[a]) : (v3[a] - v4[a]));
-  float mx = 0.f;
-#pragma unroll
-  for (int nu = 0; nu < 3; nu++) {
-    const float x = bf[3 * nu] * d[0] + bf[3 * nu + 1] * d[1] + bf[3 * nu + 2] * d[2];
-    mx = fmaxf(mx, x * x);
-  }
-  return sbf * sqrtf(0.5f * mx);
-}
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Lane layout of v_mfma_f32_16x16x4f32 (one 16x16 block, K = 4):
 //   A[i][k] is held by lane i + 16 k, B[k][j] by lane j + 16 k,
@@ -216,27 +125,6 @@ __global__ void k_mfma_selftest(double *err) {
   if (l == 0) *err = e;
 }
 
-// BF16 operands: round-to-nearest-even to the upper 16 bits of a float.
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ unsigned bf16_rn(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-__device__ __forceinline__ float bf16_val(unsigned h) { return __uint_as_float(h << 16); }
-// f = hi + lo + O(2^-16 |f|): the hi/lo split of one float
-__device__ __forceinline__ void bf16_split(float f, unsigned &hi, unsigned &lo) {
-  hi = bf16_rn(f);
-  lo = bf16_rn(f - bf16_val(hi));
-}
-// two (lo16 = k, hi16 = k+1) words -> the 4 consecutive-k operand of one lane
-__device__ __forceinline__ bf16x4 bf16_pack(unsigned w0, unsigned w1) {
-  bf16x4 v;
-  v[0] = (short)(w0 & 0xFFFFu);
-  v[1] = (short)(w0 >> 16);
-  v[2] = (short)(w1 & 0xFFFFu);
-  v[3] = (short)(w1 >> 16);
-  return v;
-}
 
 // v_mfma_f32_16x16x16bf16_1k lane layout used by the BF16 form: A[i][k] in
 // lane i + 16*(k/4), element k%4; B[k][j] in lane j + 16*(k/4), element k%4;
@@ -263,28 +151,6 @@ __global__ void k_mfma_bf16_selftest(double *err) {
   if (l == 0) *err = e;
 }
 
-// FP16 operands (the FP16 hi/lo form): round-to-nearest-even binary16 bits.
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ unsigned f16_rn(float f) {
-  return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)f);
-}
-__device__ __forceinline__ float f16_val(unsigned h) {
-  return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
-}
-// f = hi + lo + O(2^-22 |f|) for |f| in the normal binary16 range
-__device__ __forceinline__ void f16_split(float f, unsigned &hi, unsigned &lo) {
-  hi = f16_rn(f);
-  lo = f16_rn(f - f16_val(hi));
-}
-__device__ __forceinline__ f16x4 f16_pack(unsigned w0, unsigned w1) {
-  f16x4 v;
-  v[0] = __builtin_bit_cast(_Float16, (unsigned short)(w0 & 0xFFFFu));
-  v[1] = __builtin_bit_cast(_Float16, (unsigned short)(w0 >> 16));
-  v[2] = __builtin_bit_cast(_Float16, (unsigned short)(w1 & 0xFFFFu));
-  v[3] = __builtin_bit_cast(_Float16, (unsigned short)(w1 >> 16));
-  return v;
-}
-
 // v_mfma_f32_16x16x16f16: the same lane layout as the BF16 instruction,
 // checked the same way.
 __global__ void k_mfma_f16_selftest(double *err) {
@@ -309,10 +175,6 @@ __global__ void k_mfma_f16_selftest(double *err) {
   if (l == 0) *err = e;
 }
 
-// Scale of the phase operand in the FP16 form: |root| <= 1 becomes <= 2^14,
-// well inside binary16's range, with the lo part normal for components above
-// ~2^-10.
-constexpr float kPhaseScaleF16 = 16384.0f;
 
 template <class T, int NB, int IB, int TQ>
 struct DenseSmem {
@@ -1192,7 +1054,7 @@ DenseState *dense_create(const std::vector<ClusterRec> &cl, const std::vector<do
 
 void dense_destroy(DenseState *s) {
   if (!s) return;
-  void *ps[] = {s->g, s->f, s->c, s->phi, s->scratch, s->outb, s->fbose, s->counter};
+  void *ps[] = {s->g, s->f, s->c, s->phi, s->scratch, s->outb, s->fbose, s->counter, s->prof};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (s->e0) (void)hipEventDestroy(s->e0);
@@ -1220,12 +1082,15 @@ int dense_selftest_mfma(double *err) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(&ef16, d, sizeof(double), hipMemcpyDeviceToHost);
-  *err = std::max(std::max(e32, e16), ef16);
+  double ex32 = 0.0;
+  if (e == hipSuccess && dense2_selftest_mfma(&ex32) != 0) e = hipErrorUnknown;
+  *err = std::max(std::max(std::max(e32, e16), ef16), ex32);
   (void)hipFree(d);
   return e == hipSuccess ? 0 : derr(FP4_ERR_DEVICE, "selftest: kernel failed");
 }
 
-void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib) {
+void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib, int *kg) {
+  if (kg) *kg = s ? s->last_kg : 0;
   *mfma = s ? s->last_mfma : 0;
   *tq = s ? s->last_tq : 0;
   *ib = s ? s->last_ib : 0;
@@ -1250,15 +1115,10 @@ size_t lds_bytes(int ng, int ntri, int L) {
   return dyn_layout<T, NB, IB, TQ, MF>(static_bytes<T, NB, IB, TQ>(), ng, ntri, L <= kRootsLds ? L : 0).end;
 }
 
-// One kernel shape: modes per block (IB) and q'' per tile (TQ).
-struct Shape {
-  int prec, nb, IB, TQ;
-  int mfma;   // 0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA, 3 FP16 hi/lo MFMA
-  size_t (*bytes)(int, int, int);
-  size_t (*stat)();
-  void (*launch)(const DenseArgs &, int, size_t, hipStream_t);
-  int (*per_cu)(size_t);   // resident blocks per CU at this dynamic LDS size
-};
+template <class T, int NB, int IB, int TQ, int MF>
+size_t lds_bytes_geom(const DenseGeom &g) {
+  return lds_bytes<T, NB, IB, TQ, MF>(g.ng, g.ntri, g.L);
+}
 
 template <class T, int NB, int IB, int TQ, int MF>
 void launch_shape(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
@@ -1278,13 +1138,13 @@ int per_cu_shape(size_t dyn) {
   return std::max(n, 1);
 }
 
-#define FP4_SHAPE(T, P, NB, IB, TQ, MF)                                                                        \
-  {P, NB, IB, TQ, MF, &lds_bytes<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>, &launch_shape<T, NB, IB, TQ, MF>, \
-   &per_cu_shape<T, NB, IB, TQ, MF>}
+#define FP4_SHAPE(T, P, NB, IB, TQ, MF)                                                                          \
+  {P, NB, IB, TQ, MF, 0, &lds_bytes_geom<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>,                           \
+   &launch_shape<T, NB, IB, TQ, MF>, &per_cu_shape<T, NB, IB, TQ, MF>}
 
-// In order of preference; the first shape whose LDS fits is used.  The FP32
-// shapes with 8 q'' per tile run the Psi GEMM on MFMA.
-const Shape kShapes[] = {
+// k_dense shapes.  The FP32 shapes with 8 q'' per tile run the Psi GEMM on
+// MFMA.
+const DenseShape kShapes1[] = {
     FP4_SHAPE(float, FP4_PREC_FP32, 3, 3, 8, 1),             FP4_SHAPE(float, FP4_PREC_FP32, 6, 6, 8, 1),
     FP4_SHAPE(float, FP4_PREC_FP32, 6, 2, 8, 1),             FP4_SHAPE(float, FP4_PREC_FP32, 9, 3, 8, 1),
     FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 8, 1),            FP4_SHAPE(float, FP4_PREC_FP32, 12, 1, 4, 0),
@@ -1299,9 +1159,33 @@ const Shape kShapes[] = {
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, 0),
 };
 
-const Shape *pick_shape(int prec, int nb, int ng, int ntri, int L, size_t lds_max) {
-  for (const Shape &s : kShapes)
-    if (s.prec == prec && s.nb == nb && s.bytes(ng, ntri, L) <= lds_max) return &s;
+// All shapes in order of preference (k_dense2 first); the first whose LDS
+// fits is used when no tuning is given.
+const std::vector<DenseShape> &all_shapes() {
+  static const std::vector<DenseShape> v = [] {
+    std::vector<DenseShape> r;
+    int n2 = 0;
+    const DenseShape *s2 = dense2_shapes(&n2);
+    r.insert(r.end(), s2, s2 + n2);
+    r.insert(r.end(), std::begin(kShapes1), std::end(kShapes1));
+    return r;
+  }();
+  return v;
+}
+
+DenseGeom geom_of(const DenseState *s) {
+  DenseGeom g{};
+  g.ng = s->ng;
+  g.ntri = s->nc;
+  g.L = s->L;
+  g.npairs = s->natoms * s->natoms;
+  for (int q = 0; q < 17; q++) g.klf[q] = s->klf[q];
+  return g;
+}
+
+const DenseShape *pick_shape(int prec, int nb, const DenseGeom &g, size_t lds_max) {
+  for (const DenseShape &s : all_shapes())
+    if (s.prec == prec && s.nb == nb && s.bytes(g) <= lds_max) return &s;
   return nullptr;
 }
 
@@ -1329,17 +1213,20 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   if (s->natoms > 4) return derr(FP4_ERR_UNSUPPORTED, "dense form: more than 4 atoms per cell");
   const bool tuned = r.tuning && r.tuning->shape_prec == r.precision;
-  const Shape *shp = nullptr;
-  if (tuned && r.tuning->shape >= 0 && r.tuning->shape < (int)(sizeof(kShapes) / sizeof(kShapes[0]))) {
-    const Shape &c = kShapes[r.tuning->shape];
-    if (c.prec == r.precision && c.nb == nb && c.bytes(s->ng, s->nc, s->L) <= (size_t)ldsmax) shp = &c;
+  const DenseShape *shp = nullptr;
+  const DenseGeom geo = geom_of(s);
+  const std::vector<DenseShape> &shapes = all_shapes();
+  if (tuned && r.tuning->shape >= 0 && r.tuning->shape < (int)shapes.size()) {
+    const DenseShape &c = shapes[r.tuning->shape];
+    if (c.prec == r.precision && c.nb == nb && c.bytes(geo) <= (size_t)ldsmax) shp = &c;
   }
-  if (!shp) shp = pick_shape(r.precision, nb, s->ng, s->nc, s->L, (size_t)ldsmax);
+  if (!shp) shp = pick_shape(r.precision, nb, geo, (size_t)ldsmax);
   if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
-  const Shape &sh = *shp;
+  const DenseShape &sh = *shp;
   s->last_mfma = sh.mfma;
   s->last_tq = sh.TQ;
   s->last_ib = sh.IB;
+  s->last_kg = sh.kg;
   const int nmodes_all = nb * r.m.nlist;
   std::vector<unsigned char> sel(nmodes_all, 0);
   std::vector<int> lls;
@@ -1359,7 +1246,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   const long nseg = 3L * nls * nib * sh.IB;
   // outputs: og, ow, on per item*IB, then sg, sw, sn per segment, then lls and sel
   const size_t need = per_item * nitems * sh.IB + sizeof(int) * nitems + (sizeof(double) * 2 + sizeof(long long)) * nseg +
-                      sizeof(int) * nls + nmodes_all + 1024;
+                      sizeof(int) * (nls + s->ng) + nmodes_all + 1024;
   if ((rc = ensure(s->outb, s->out_bytes, need))) return rc;
   char *p = reinterpret_cast<char *>(s->outb);
   double *og = reinterpret_cast<double *>(p);
@@ -1378,9 +1265,16 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   p += sizeof(int) * nitems;
   int *dlls = reinterpret_cast<int *>(p);
   p += sizeof(int) * nls;
+  int *dgslot = reinterpret_cast<int *>(p);
+  p += sizeof(int) * s->ng;
   unsigned char *dsel = reinterpret_cast<unsigned char *>(p);
+  // k_dense2: phase-table slots of the coarse groups for this k-step shape
+  int pair_off[17] = {};
+  std::vector<int> gslot(std::max(s->ng, 1), 0);
+  if (sh.kg > 0) dense2_slots(geo, sh.kg, pair_off, gslot.data());
   hipStream_t st = r.stream;
   if (hipMemcpyAsync(dlls, lls.data(), sizeof(int) * nls, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dgslot, gslot.data(), sizeof(int) * s->ng, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dsel, sel.data(), nmodes_all, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemsetAsync(s->counter, 0, sizeof(int), st) != hipSuccess)
     return derr(FP4_ERR_DEVICE, "dense: upload failed");
@@ -1419,14 +1313,24 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
       for (int c = 0; c < 3; c++) a.bf[3 * nu + c] = (float)(r.m.b[3 * nu + c] / ng3[nu]);
     a.sbf = (float)std::fabs(r.m.scalebroad);
     a.a_scale = s->a_scale;
-  a.a_scale = s->a_scale;
   }
+  for (int q = 0; q < 17; q++) a.pair_off[q] = pair_off[q];
+  a.gpt = sh.kg > 0 ? dense2_gpt(geo, sh.kg) : 0;
+  a.gslot = dgslot;
+  a.lmask = (s->L > 0 && (s->L & (s->L - 1)) == 0) ? s->L - 1 : -1;
+#ifdef FP4_PHASE_PROF
+  if (!s->prof && hipMalloc(&s->prof, sizeof(unsigned long long) * kDensePhases) != hipSuccess)
+    return derr(FP4_ERR_NOMEM, "dense: hipMalloc failed");
+  if (hipMemsetAsync(s->prof, 0, sizeof(unsigned long long) * kDensePhases, st) != hipSuccess)
+    return derr(FP4_ERR_DEVICE, "dense: memset failed");
+#endif
+  a.prof = s->prof;
   a.out_g = og;
   a.out_w = ow;
   a.out_n = on;
   a.out_t = ot;
   if (hipEventRecord(s->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  sh.launch(a, slots, sh.bytes(s->ng, s->nc, s->L), st);
+  sh.launch(a, slots, sh.bytes(geo), st);
   if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense kernel launch failed");
   if (hipEventRecord(s->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
   hipLaunchKernelGGL(k_dense_reduce, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, st, og, ow, on, nseg, sh.IB,
@@ -1446,6 +1350,11 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
     for (int x : ht) tt += x;
     s->last_triples = tt;
     s->last_items = nitems;
+#ifdef FP4_PHASE_PROF
+    if (hipMemcpy(s->last_prof, s->prof, sizeof(unsigned long long) * kDensePhases, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return derr(FP4_ERR_DEVICE, "dense: copy failed");
+#endif
   }
   float kms = 0.f;
   (void)hipEventElapsedTime(&kms, s->e0, s->e1);
@@ -1479,16 +1388,19 @@ int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &mod
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int nshapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+  const std::vector<DenseShape> &shapes = all_shapes();
+  const DenseGeom geo = geom_of(s);
+  const int nshapes = (int)shapes.size();
+  s->cands.clear();
   DenseTuning best;
   double tbest = 0.0;
   int n = 0;
   std::vector<double> g, w;
   std::vector<long long> cnt;
   for (int i = 0; i < nshapes; i++) {
-    const Shape &c = kShapes[i];
+    const DenseShape &c = shapes[i];
     if (c.prec != r.precision || c.nb != r.m.nb) continue;
-    const size_t dyn = c.bytes(s->ng, s->nc, s->L);
+    const size_t dyn = c.bytes(geo);
     if (dyn > (size_t)ldsmax) continue;
     const int per_cu = std::min(c.per_cu(dyn), 2);
     for (int k = 1; k <= per_cu; k++) {
@@ -1504,6 +1416,7 @@ int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &mod
       if ((rc = dense_rates(s, rr, modes, ii_lo0, nii, g, w, cnt, &ms))) return rc;
       if ((rc = dense_rates(s, rr, modes, ii_lo0, nii, g, w, cnt, &ms))) return rc;
       n++;
+      s->cands.push_back({i, t.slots, ms});
       if (best.shape < 0 || ms < tbest) {
         best = t;
         tbest = ms;
@@ -1514,10 +1427,58 @@ int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &mod
   *out = best;
   if (best_ms) *best_ms = tbest;
   if (ncand) *ncand = n;
-  s->last_mfma = kShapes[best.shape].mfma;
-  s->last_tq = kShapes[best.shape].TQ;
-  s->last_ib = kShapes[best.shape].IB;
+  s->last_mfma = shapes[best.shape].mfma;
+  s->last_tq = shapes[best.shape].TQ;
+  s->last_ib = shapes[best.shape].IB;
+  s->last_kg = shapes[best.shape].kg;
   return 0;
+}
+
+int dense_phase_profile(const DenseState *s, int cap, double *cycles) {
+#ifdef FP4_PHASE_PROF
+  if (!s) return 0;
+  for (int p = 0; p < kDensePhases && p < cap; p++) cycles[p] = (double)s->last_prof[p];
+  return kDensePhases;
+#else
+  (void)s;
+  (void)cap;
+  (void)cycles;
+  return -FP4_ERR_UNSUPPORTED;
+#endif
+}
+
+int dense_candidates(const DenseState *s, int cap, int *kg, int *ib, int *tq, int *slots, double *ms) {
+  if (!s) return 0;
+  const std::vector<DenseShape> &shapes = all_shapes();
+  const int n = (int)s->cands.size();
+  for (int i = 0; i < n && i < cap; i++) {
+    const DenseShape &c = shapes[s->cands[i].shape];
+    if (kg) kg[i] = c.kg;
+    if (ib) ib[i] = c.IB;
+    if (tq) tq[i] = c.TQ;
+    if (slots) slots[i] = s->cands[i].slots;
+    if (ms) ms[i] = s->cands[i].ms;
+  }
+  return n;
+}
+
+int dense_pick(const DenseState *s, int precision, int nb, int kg, int ib, int slots, DenseTuning *out) {
+  if (!s || !out) return derr(FP4_ERR_ARG, "dense_pick: null argument");
+  int dev = 0, ldsmax = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const std::vector<DenseShape> &shapes = all_shapes();
+  const DenseGeom geo = geom_of(s);
+  for (int i = 0; i < (int)shapes.size(); i++) {
+    const DenseShape &c = shapes[i];
+    if (c.prec == precision && c.nb == nb && c.kg == kg && c.IB == ib && c.bytes(geo) <= (size_t)ldsmax) {
+      out->shape = i;
+      out->shape_prec = precision;
+      out->slots = slots;
+      return 0;
+    }
+  }
+  return derr(FP4_ERR_UNSUPPORTED, "dense_pick: no built shape with this precision, Nbands, IB and k-step fits");
 }
 
 }  // namespace fp4

@@ -42,8 +42,10 @@ int dense_ngroups(const DenseState *s);
 // Work of the last dense_rates call: visited triples and work items, and the
 // coarse / fine group counts of the clusters.
 void dense_info(const DenseState *s, long long *triples, long long *items, int *ngroups, int *nfine);
-// Kernel shape of the last dense_rates call: MFMA path, q'' per tile, modes per block.
-void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib);
+// Kernel shape of the last dense_rates call: MFMA path, q'' per tile, modes
+// per block, and the coarse groups per MFMA k-step (0 = k_dense, > 0 =
+// k_dense2).
+void dense_shape(const DenseState *s, int *mfma, int *tq, int *ib, int *kg = nullptr);
 // Checks the MFMA lane layout the dense kernel assumes; *err = max abs error.
 int dense_selftest_mfma(double *err);
 
@@ -71,6 +73,18 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
 // different shapes agree to rounding (the per-item sums are regrouped).
 int dense_autotune(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
                    DenseTuning *out, double *best_ms, int *ncand);
+
+// Profiling build only: shader cycles per k_dense2 phase (DensePhase order,
+// summed over waves) of the last dense call; returns the number of phases,
+// or -FP4_ERR_UNSUPPORTED in the product build.
+int dense_phase_profile(const DenseState *s, int cap, double *cycles);
+// Candidates timed by the last dense_autotune call: kernel k-step groups
+// (0 = k_dense), IB, TQ, resident blocks and kernel time; returns the count.
+int dense_candidates(const DenseState *s, int cap, int *kg, int *ib, int *tq, int *slots, double *ms);
+// A tuning that runs the first shape of this precision and Nbands with the
+// given IB and k-step groups (kg = 0: k_dense) at `slots` blocks (0 = one
+// per CU); FP4_ERR_UNSUPPORTED when no such shape is built or fits in LDS.
+int dense_pick(const DenseState *s, int precision, int nb, int kg, int ib, int slots, DenseTuning *out);
 
 const char *dense_error();
 

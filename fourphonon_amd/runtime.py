@@ -19,7 +19,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfp4.so")
+# FP4_PROFILE_BUILD=1 loads the profiling build (same kernels, plus the
+# per-phase clock counters of the dense kernel; see build.py)
+LIB_PATH = os.path.join(HERE, "libfp4prof.so" if os.environ.get("FP4_PROFILE_BUILD") == "1" else "libfp4.so")
 
 PREC_FP64 = 0
 PREC_FP64_DENSE = 1
@@ -36,7 +38,7 @@ EXPORTS = [
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
     "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_ind_only_4ph_modes",
     "fp4_ctx_kernel_stats",
-    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape",
+    "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape", "fp4_ctx_dense_kernel", "fp4_ctx_dense_candidates", "fp4_ctx_set_dense_shape", "fp4_ctx_dense_phase_profile",
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
@@ -418,6 +420,38 @@ class Context:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load().fp4_ctx_dense_shape(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"mfma": a.value, "tq": b.value, "ib": c.value}   # mfma: 0 VALU, 1 FP32, 2 BF16 hi/lo, 3 FP16 hi/lo
+
+    def dense_candidates(self, cap=64):
+        """-> list of dict(kg, ib, tq, slots, ms): the candidates the last autotune timed."""
+        kg, ib, tq, sl = (np.zeros(cap, np.int32) for _ in range(4))
+        ms = np.zeros(cap)
+        n = load().fp4_ctx_dense_candidates(self.h, C.c_int32(cap), _p(kg), _p(ib), _p(tq), _p(sl), _p(ms))
+        if n < 0:
+            _check(-n)
+        return [{"kg": int(kg[i]), "ib": int(ib[i]), "tq": int(tq[i]), "slots": int(sl[i]), "ms": float(ms[i])}
+                for i in range(min(n, cap))]
+
+    DENSE_PHASES = ["item", "setup", "scan", "tile", "bar_tile", "filter", "bar_filter", "queue", "gemm",
+                    "contract", "rate", "bar_end"]
+
+    def dense_phase_profile(self):
+        """Profiling build only: {phase: shader cycles summed over waves} of the last dense call."""
+        cyc = np.zeros(len(self.DENSE_PHASES))
+        n = load().fp4_ctx_dense_phase_profile(self.h, C.c_int32(len(cyc)), _p(cyc))
+        if n < 0:
+            _check(-n)
+        return dict(zip(self.DENSE_PHASES, cyc.tolist()))
+
+    def set_dense_shape(self, precision, kg, ib, slots=0):
+        """Makes later dense calls of this precision run the given kernel (kg = 0: block-cooperative) and IB."""
+        _check(load().fp4_ctx_set_dense_shape(self.h, C.c_int32(precision), C.c_int32(kg), C.c_int32(ib),
+                                              C.c_int32(slots)))
+
+    def dense_kernel(self):
+        """-> coarse groups per MFMA k-step of the last dense call's kernel (0: block-cooperative kernel)."""
+        k = C.c_int32()
+        _check(load().fp4_ctx_dense_kernel(self.h, C.byref(k)))
+        return k.value
 
     # ------------------------------------------------------- three phonons
     _RATE3 = ["rate_scatt", "rate_scatt_plus", "rate_scatt_minus", "WP3_plus", "WP3_minus"]

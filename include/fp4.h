@@ -371,6 +371,31 @@ int fp4_ctx_dense_info(fp4_ctx *ctx, int64_t *triples, int64_t *items, int32_t *
    kernel: its Psi GEMM path (0 VALU, 1 FP32 MFMA, 2 BF16 hi/lo MFMA, 3 FP16
    hi/lo MFMA), q'' per tile, modes per block. */
 int fp4_ctx_dense_shape(fp4_ctx *ctx, int32_t *mfma, int32_t *tq, int32_t *ib);
+/* Diagnostics; no counterpart in the reference.  Kernel of the last dense
+   call: 0 = the block-cooperative kernel, > 0 = the kernel whose waves own
+   slices of the (i, j) branch pairs, with that many coarse groups per MFMA
+   k-step. */
+int fp4_ctx_dense_kernel(fp4_ctx *ctx, int32_t *kg);
+/* Diagnostics; no counterpart in the reference.  The candidates the last
+   fp4_ctx_autotune_dense call timed (up to cap of them): kernel k-step
+   groups as in fp4_ctx_dense_kernel, IB, TQ, resident blocks and kernel
+   time in ms.  Returns the number of candidates (negative on error); any
+   output may be NULL. */
+int fp4_ctx_dense_candidates(fp4_ctx *ctx, int32_t cap, int32_t *kg, int32_t *ib, int32_t *tq, int32_t *slots,
+                             double *ms);
+/* Diagnostics; no counterpart in the reference.  In the profiling build
+   (libfp4prof.so, compiled with FP4_PHASE_PROF) the shader clock cycles of
+   the last dense call's k_dense2 launch per phase, summed over waves, in the
+   order: work-item fetch and reduction, A' set-up, q'' scan, tile data, tile
+   barrier, filter, filter barrier, survivor queue, Psi GEMM, contraction,
+   rates, end-of-tile barrier.  Returns the number of phases written (12);
+   -FP4_ERR_UNSUPPORTED in the product build. */
+int fp4_ctx_dense_phase_profile(fp4_ctx *ctx, int32_t cap, double *cycles);
+/* Tuning control; no counterpart in the reference.  Makes later dense calls
+   of this precision run the built shape with the given k-step groups (as in
+   fp4_ctx_dense_kernel) and modes per block at `slots` resident blocks
+   (0 = one per CU), in place of the autotuner's choice. */
+int fp4_ctx_set_dense_shape(fp4_ctx *ctx, int32_t precision, int32_t kg, int32_t ib, int32_t slots);
 /* The dense form's autotuner; no counterpart in the reference (its CPU loops
    have no tile shapes).  Times every built kernel shape of this precision and
    Nbands that fits in LDS (modes per block IB, q'' per tile TQ, MFMA path),
@@ -384,9 +409,10 @@ int fp4_ctx_dense_shape(fp4_ctx *ctx, int32_t *mfma, int32_t *tq, int32_t *ib);
    rounding. */
 int fp4_ctx_autotune_dense(fp4_ctx *ctx, int32_t precision, int32_t nmodes, const int32_t *modes, int32_t ii_lo,
                            int32_t ii_hi, int32_t *ib, int32_t *tq, int32_t *slots, double *best_ms);
-/* Diagnostics; no counterpart in the reference.  Runs one 16x16x4 FP32 MFMA
+/* Diagnostics; no counterpart in the reference.  Runs the MFMA instructions
+   the dense kernels issue (16x16x4 FP32, 16x16x16 and 16x16x32 BF16 and FP16)
    with known operands and returns the largest deviation from the exact
-   product, checking the lane layout the dense kernel relies on. */
+   products, checking the lane layouts the kernels rely on. */
 int fp4_selftest_mfma(int32_t device, double *max_err);
 /* Upper bound on the processes materialised per batch (default 2^26); no
    counterpart in the reference, which materialises every list at once. */

@@ -67,8 +67,13 @@ def test_dense_visits_exactly_the_prefilter_processes(pair):
     modes = np.arange(1, c.nmodes + 1)
     _, n64 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
     _, n32 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP32)
+    assert c.dense_kernel() > 0          # the wave-owned-slice kernel with its sorted-x1 screen
+    _, nb16 = c.RTA_driver_4ph_modes(modes, rt.PREC_BF16X3)
+    assert c.dense_kernel() > 0
+    _, nf16 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP16X3)
     _, nd = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64_DENSE)
-    assert n64 == n32 == nd > 0
+    assert c.dense_kernel() == 0
+    assert n64 == n32 == nb16 == nf16 == nd > 0
 
 
 def test_dense_fp32_tolerances(pair):
@@ -164,7 +169,10 @@ def test_dense_golden_wurtzite_12x12x8_window():
 def test_autotuner_picks_a_built_shape_and_keeps_the_rates(prec):
     """The autotuner times the built shapes on a sample and keeps the fastest;
     later calls run that shape and give the same rates as the default shape
-    (to rounding) and the FP64 form's within the form's bar."""
+    (to rounding) and the FP64 form's within the form's bar.  Shapes of the
+    MFMA forms may differ in their k-step grouping, so their FP32 sums
+    round differently: the shape-to-shape bar there is 1e-5, far below the
+    forms' 1e-3."""
     case = crystal.config_case(1, ngrid=(4, 4, 4), shell=2)
     c = rt.Context(case)
     try:
@@ -176,7 +184,8 @@ def test_autotuner_picks_a_built_shape_and_keeps_the_rates(prec):
         assert t["ib"] in (1, 2, 3, 6) and t["tq"] in (2, 4, 8) and t["slots"] >= 1 and t["ms"] > 0
         after, _ = c.RTA_driver_4ph_modes(modes, prec)
         assert c.dense_shape()["ib"] == t["ib"] and c.dense_shape()["tq"] == t["tq"]
-        assert rel_err(after["rate_scatt_4ph"], before["rate_scatt_4ph"]) <= 1e-9
+        same_bar = 1e-9 if prec == rt.PREC_FP64_DENSE else 1e-5
+        assert rel_err(after["rate_scatt_4ph"], before["rate_scatt_4ph"]) <= same_bar
         ref, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
         bar = 1e-9 if prec == rt.PREC_FP64_DENSE else 1e-3
         assert rel_err(after["rate_scatt_4ph"], ref["rate_scatt_4ph"]) <= bar
