@@ -794,7 +794,7 @@ int fp4_rta_4ph_modes(fp4_ctx *c, int32_t precision, int32_t nmodes, const int32
       if ((rc = rate_batch(c, pl, b, false, false, g, w))) return rc;
     }
   } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3 ||
-             precision == FP4_PREC_FP16X3) {
+             precision == FP4_PREC_FP16X3 || precision == FP4_PREC_FP16X2) {
     DenseRun dr;
     dr.m = mode_view(c);
     dr.roots = c->roots.as<double2>();
@@ -1872,7 +1872,7 @@ int fp4_ctx_autotune_dense(fp4_ctx *c, int32_t precision, int32_t nmodes, const 
   int rc;
   if ((rc = check_ready(c, true))) return rc;
   if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP64_DENSE && precision != FP4_PREC_BF16X3 &&
-      precision != FP4_PREC_FP16X3)
+      precision != FP4_PREC_FP16X3 && precision != FP4_PREC_FP16X2)
     return fail(FP4_ERR_ARG, "autotune: precision %d has no dense kernel", precision);
   if (ii_lo < 1 || ii_hi > c->nptk || ii_hi < ii_lo) return fail(FP4_ERR_ARG, "bad ii window %d..%d", ii_lo, ii_hi);
   Plan pl;

@@ -1205,7 +1205,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
                 std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms) {
   if (!s) return derr(FP4_ERR_STATE, "dense state missing");
   if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3 &&
-      r.precision != FP4_PREC_FP16X3)
+      r.precision != FP4_PREC_FP16X3 && r.precision != FP4_PREC_FP16X2)
     return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
   const int nb = r.m.nb;
   int dev = 0, ldsmax = 0;

@@ -137,31 +137,34 @@ struct Own {
   }
 };
 
+// One tile's data; the kernel keeps two, the current tile's and the next's.
+template <int NB>
+struct TileBuf {
+  C2<float> e3[8][NB][NB];   // [t][comp][k], conj applied
+  C2<float> e4[8][NB][NB];   // [t][comp][l], conj applied
+  double Ejj[8][NB], Ess[8][NB], fb3[8][NB], fb4[8][NB];
+  double inv3[8][NB], inv4[8][NB];      // 1 / w'' and 1 / w''', 0 at or below the 1.25 rad/ps cut
+  double isg[8][NB][NB];                // 1 / clamped exact sigma where a candidate survived
+  unsigned long long mask[8][NB][NB];   // surviving (i, j) of each (t, k, l)
+  unsigned hitword;                     // tile triples with a survivor (bit t)
+};
+
+// The filter's inputs of one tile: frequencies and velocities of phonons 3
+// and 4, staged two tiles ahead of the filter.
+template <int NB>
+struct FiltIn {
+  double Ej[8][NB], Es[8][NB];
+  double vj[8][NB][3], vs[8][NB][3];
+};
+
 template <int NB, int IB, int CTW>
 struct Smem2 {
   static constexpr int NIJ = IB * NB;
   static constexpr int TQ = 8;
   static constexpr int NW = NT_DENSE / 64;
   static constexpr int QW = 128;    // survivor codes per wave and chunk
-  static constexpr int FMAX = 4;    // fine groups per setup chunk
-  union {
-    struct {
-      double2 af[FMAX][81];
-      double2 b1[FMAX][IB][27];
-    } setup;
-    struct {
-      C2<float> psi[NW][TQ][16 * CTW];   // each wave's slice of Psi for one (K, L) pair
-      unsigned q[NW][QW];                // each wave's survivor codes: task | ij << 16
-    } run;
-  } u;
-  C2<float> e3[TQ][NB][NB];   // [t][comp][k], conj applied
-  C2<float> e4[TQ][NB][NB];   // [t][comp][l], conj applied
-  double Ejj[TQ][NB], Ess[TQ][NB], fb3[TQ][NB], fb4[TQ][NB];
-  double vjj[TQ][NB][3], vss[TQ][NB][3];
-  double isg[TQ][NB][NB];               // 1 / clamped exact sigma where a candidate was tested
-  double inv3[TQ][NB], inv4[TQ][NB];    // 1 / w'' and 1 / w''', 0 at or below the 1.25 rad/ps cut
-  unsigned long long mask[TQ][NB][NB];  // surviving (i, j) of each (t, k, l)
-  int jj[TQ], ss[TQ], hit[TQ], qjj[TQ][3];
+  TileBuf<NB> tb[2];
+  FiltIn<NB> fi[2];
   double xs[NIJ];                       // x1 of the item, ascending
   int perm[NIJ];                        // (i, j) index of each sorted entry
   double x1[NIJ];
@@ -172,19 +175,43 @@ struct Smem2 {
   double2 e2[NB][NB];
   int vj[NT_DENSE], vs[NT_DENSE];
   int wsum[NW];
+  int coop_ctr[2];   // chunks of the next tile's filter and phase work taken so far, by buffer
   long item;
   double red[NW][2][3];                 // per wave: sums of its (at most) two modes
 };
 
-struct Dyn2 {
-  size_t A, c, ph, rt, gd, gs, end;
+// LDS used only in the q'' loop: each wave's Psi slice for one (K, L) pair
+// and its survivor codes (task | ij << 16)
+template <int CTW>
+struct Run2 {
+  C2<float> psi[NT_DENSE / 64][8][16 * CTW];
+  unsigned q[NT_DENSE / 64][128];
 };
-__host__ __device__ inline Dyn2 dyn2_layout(size_t stat, int npad, int ng, int ntri, int gpt, int L_lds) {
+// LDS used only while folding A': one wave's sums over the clusters of a
+// fine group and their contraction with the first eigenvectors
+template <int IB>
+struct SetupW {
+  float2 F[81];
+  double2 b1[IB][27];
+};
+
+// Dynamic LDS: U = the cluster coefficients and the per-wave set-up scratch,
+// or Run2 in the q'' loop (never both); A' in the B layout; the phase operand;
+// roots, group vectors and phase-table slots.
+struct Dyn2 {
+  size_t U, W, A, ph, rt, gd, gs, end;
+};
+template <int IB, int CTW>
+__host__ __device__ inline Dyn2 dyn2_layout(size_t stat, int npad, int ng, int ntri, int gpt, int L_lds,
+                                            int abytes) {
   Dyn2 d{};
-  d.A = align16(stat);
-  d.c = align16(d.A + (size_t)8 * ng * npad);              // A': two word planes or float2 [ng][npad]
-  d.ph = align16(d.c + sizeof(C2<float>) * (size_t)ntri);
-  d.rt = align16(d.ph + (size_t)128 * gpt);                // phases: 2 planes x 16 rows x gpt words
+  d.U = align16(stat);
+  d.W = align16(d.U + sizeof(C2<float>) * (size_t)ntri);
+  const size_t setup_end = d.W + sizeof(SetupW<IB>) * (NT_DENSE / 64);
+  const size_t run_end = d.U + sizeof(Run2<CTW>);
+  d.A = align16(setup_end > run_end ? setup_end : run_end);
+  d.ph = align16(d.A + (size_t)abytes * ng * npad);         // A': word planes or float2 [ng][npad]
+  d.rt = align16(d.ph + (size_t)256 * gpt);                 // phases: 2 tiles x 2 planes x 16 rows x gpt words
   d.gd = align16(d.rt + sizeof(double2) * (size_t)L_lds);
   d.gs = align16(d.gd + sizeof(int) * 3 * (size_t)ng);
   d.end = align16(d.gs + sizeof(int) * (size_t)ng);
@@ -267,8 +294,92 @@ __device__ __forceinline__ void lds_wave_sync() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 
+// A' of the coarse groups gi = wave, wave + NW, ...: each wave folds whole
+// groups (fine groups in order) with wave-local synchronisation only, into
+// the B layout.
+template <int NB, int IB, int MF, int NPAD, class S>
+__device__ __forceinline__ void fold_groups(const DGroup *g, const FGroup *fg, const double *phi, int ng, float a_scale,
+                                         const S &s, SetupW<IB> &sw, const C2<float> *s_c, unsigned *Bh,
+                                         unsigned *Bl, float *Bf, int wave, int lane) {
+  constexpr int NOUT2 = IB * NB * 9, NW = NT_DENSE / 64;
+  constexpr int NPL = (NOUT2 + 63) / 64;   // A' entries of a group per lane
+  for (int gi = wave; gi < ng; gi += NW) {
+    const DGroup G = g[gi];
+    // FP32 partial sums (A' is rounded to FP32 or to BF16/FP16 pairs afterwards)
+    float2 acc[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; k++) acc[k] = make_float2(0.f, 0.f);
+    for (int fi = G.f_first; fi < G.f_first + G.f_count; fi++) {
+      const FGroup F = fg[fi];
+      for (int e = lane; e < 81; e += 64) {
+        double sr = 0.0, si = 0.0;
+        for (int ci = F.c_first; ci < F.c_first + F.c_count; ci++) {
+          const double p0 = phi[(size_t)ci * 81 + e];
+          const C2<float> c0 = s_c[ci];
+          sr += p0 * (double)c0.re;
+          si += p0 * (double)c0.im;
+        }
+        sw.F[e] = make_float2((float)sr, (float)si);
+      }
+      lds_wave_sync();
+      for (int x = lane; x < IB * 27; x += 64) {
+        const int il = x / 27, tsr = x % 27;
+        double2 sacc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int uu = 0; uu < 3; uu++) {
+          const float2 f = sw.F[uu + 3 * tsr];
+          const double2 v = zm(make_double2((double)f.x, (double)f.y), s.e1[3 * F.I + uu][il]);
+          sacc.x += v.x;
+          sacc.y += v.y;
+        }
+        sw.b1[il][tsr] = sacc;
+      }
+      lds_wave_sync();
+#pragma unroll
+      for (int k = 0; k < NPL; k++) {
+        const int n = lane + 64 * k;
+        if (n < NOUT2) {
+          const int il = n / (NB * 9), rem = n % (NB * 9), j = rem / 9, sr = rem % 9;
+          double2 v3 = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int t = 0; t < 3; t++) {
+            const double2 v = zm(sw.b1[il][t + 3 * sr], s.e2[3 * F.J + t][j]);
+            v3.x += v.x;
+            v3.y += v.y;
+          }
+          acc[k].x += (float)v3.x;
+          acc[k].y += (float)v3.y;
+        }
+      }
+      lds_wave_sync();
+    }
+#pragma unroll
+    for (int k = 0; k < NPL; k++) {
+      const int n = lane + 64 * k;
+      if (n < NOUT2) {
+        if constexpr (MF == 1) {
+          Bf[((size_t)gi * NPAD + n) * 2] = (float)acc[k].x;
+          Bf[((size_t)gi * NPAD + n) * 2 + 1] = (float)acc[k].y;
+        } else {
+          unsigned rh, rl, ih, il;
+          if constexpr (MF >= 3) {
+            f16_split((float)acc[k].x * a_scale, rh, rl);
+            f16_split((float)acc[k].y * a_scale, ih, il);
+          } else {
+            bf16_split((float)acc[k].x, rh, rl);
+            bf16_split((float)acc[k].y, ih, il);
+          }
+          Bh[(size_t)gi * NPAD + n] = rh | (ih << 16);
+          if constexpr (MF != 4) Bl[(size_t)gi * NPAD + n] = rl | (il << 16);
+        }
+      }
+    }
+  }
+}
+
 // MF: 1 = FP32 MFMA (KG = 2 groups per k-step), 2 = BF16 hi/lo, 3 = FP16
-// hi/lo (KG = 16 on the K=32 instructions, 8 on the K=16 ones).
+// hi/lo, 4 = FP16 hi/lo phases against one FP16 A' plane (KG = 16 on the
+// K=32 instructions, 8 on the K=16 ones).
 template <int NB, int IB, int MF, int KG>
 __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
   constexpr int NIJ = IB * NB;
@@ -277,7 +388,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
   constexpr int GPL = (MF == 1) ? 1 : KG / 4;        // groups per lane group and k-step (16-bit forms)
   constexpr int NPAD = (MF == 1) ? O::npad(1, 0) : (GPL == 4 ? O::npad(8, 4) : O::npad(16, 8));
   using S = Smem2<NB, IB, CTW>;
-  constexpr int TQ = S::TQ, NW = S::NW, QW = S::QW, QPT = QW / 64, FMAX = S::FMAX;
+  constexpr int TQ = S::TQ, NW = S::NW, QW = S::QW, QPT = QW / 64;
+  static_assert(QW == 128 && TQ == 8, "Run2 sizes");
   constexpr int NOUT2 = NIJ * 9;
   constexpr int NA = NB / 3;
   constexpr int NKLT = TQ * NB * NB;   // (t, k, l) tasks of a tile
@@ -287,18 +399,22 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
   static_assert(two_modes_per_wave<NIJ, NB>(), "each wave's (i, j) range spans at most two modes");
   constexpr unsigned long long kJall = (1ull << NB) - 1ull;
   constexpr unsigned long long kRep = rep_mask(IB, NB);
-  const float psi_unscale = (MF == 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
+  const float psi_unscale = (MF >= 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
+  constexpr int ABYTES = (MF == 4) ? 4 : 8;   // A' bytes per entry
 
   extern __shared__ __align__(16) unsigned char dyn[];
   S &s = *reinterpret_cast<S *>(dyn);
-  const Dyn2 dl = dyn2_layout(a.smem_static, NPAD, a.ng, a.ntri, a.gpt, a.roots_lds ? a.L : 0);
+  const Dyn2 dl = dyn2_layout<IB, CTW>(a.smem_static, NPAD, a.ng, a.ntri, a.gpt, a.roots_lds ? a.L : 0, ABYTES);
   unsigned *Bh = reinterpret_cast<unsigned *>(dyn + dl.A);   // 16-bit forms: hi plane [ng][NPAD]
-  unsigned *Bl = Bh + (size_t)a.ng * NPAD;                    // lo plane
+  unsigned *Bl = Bh + (size_t)a.ng * NPAD;                    // lo plane (not in the FP16X2 form)
   float *Bf = reinterpret_cast<float *>(dyn + dl.A);          // FP32: [ng][NPAD][re, im]
-  C2<float> *s_c = reinterpret_cast<C2<float> *>(dyn + dl.c);
-  unsigned *Ph = reinterpret_cast<unsigned *>(dyn + dl.ph);  // 16-bit forms: [16 rows][gpt] hi, then lo
-  unsigned *Pl = Ph + (size_t)16 * a.gpt;
-  float *Pf = reinterpret_cast<float *>(dyn + dl.ph);        // FP32: [16 rows][gpt][qq]
+  C2<float> *s_c = reinterpret_cast<C2<float> *>(dyn + dl.U);
+  Run2<CTW> &run = *reinterpret_cast<Run2<CTW> *>(dyn + dl.U);
+  // phase operand of tile buffer b: 16-bit forms [16 rows][gpt] hi plane at
+  // Ph + 32 b gpt, lo plane 16 gpt words later; FP32 [16 rows][gpt][qq] at
+  // Pf + 32 b gpt
+  unsigned *Ph = reinterpret_cast<unsigned *>(dyn + dl.ph);
+  float *Pf = reinterpret_cast<float *>(dyn + dl.ph);
   double2 *s_rt = reinterpret_cast<double2 *>(dyn + dl.rt);
   int *s_gd = reinterpret_cast<int *>(dyn + dl.gd);
   int *s_gs = reinterpret_cast<int *>(dyn + dl.gs);
@@ -306,6 +422,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
   const ModeView &m = a.m;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  SetupW<IB> &sw = reinterpret_cast<SetupW<IB> *>(dyn + dl.W)[wave];
   const double hbarp = kHbar * (double)k1e22;
   const double gpre = hbarp * hbarp * kPi / 8.0;
   const int ijl = O::lo(wave), ijh = O::hi(wave);
@@ -322,7 +439,11 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
   for (int x = tid; x < a.ng; x += NT_DENSE) s_gs[x] = a.gslot[x];
   if (a.roots_lds)
     for (int x = tid; x < a.L; x += NT_DENSE) s_rt[x] = a.roots[x];
-  for (int x = tid; x < 32 * a.gpt; x += NT_DENSE) Ph[x] = 0u;
+  for (int x = tid; x < 64 * a.gpt; x += NT_DENSE) Ph[x] = 0u;
+  if (tid < 2) {
+    s.coop_ctr[tid] = 0;
+    s.tb[tid].hitword = 0u;
+  }
   for (int x = tid; x < a.ng * (NPAD - NOUT2); x += NT_DENSE) {
     const int r = x / (NPAD - NOUT2), c = NOUT2 + x % (NPAD - NOUT2);
     if constexpr (MF == 1) {
@@ -330,7 +451,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
       Bf[((size_t)r * NPAD + c) * 2 + 1] = 0.f;
     } else {
       Bh[(size_t)r * NPAD + c] = 0u;
-      Bl[(size_t)r * NPAD + c] = 0u;
+      if constexpr (MF != 4) Bl[(size_t)r * NPAD + c] = 0u;
     }
   }
   __syncthreads();
@@ -422,152 +543,265 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
       __syncthreads();
       const double xsmax = fmax(fabs(s.xs[0]), fabs(s.xs[NIJ - 1]));
       // ------------- setup 2: A' of every coarse group, into the B layout
-      for (int gi = 0; gi < a.ng; gi++) {
-        const DGroup G = a.g[gi];
-        double2 acc = make_double2(0.0, 0.0);
-        for (int f0 = 0; f0 < G.f_count; f0 += FMAX) {
-          const int nf = min(FMAX, G.f_count - f0);
-          for (int x = tid; x < nf * 81; x += NT_DENSE) {
-            const int fl = x / 81, e = x % 81;
-            const FGroup F = a.f[G.f_first + f0 + fl];
-            double sr = 0.0, si = 0.0;
-            int ci = F.c_first;
-            const int ce = F.c_first + F.c_count;
-            for (; ci + 1 < ce; ci += 2) {
-              const double p0 = a.phi[(size_t)ci * 81 + e], p1 = a.phi[(size_t)(ci + 1) * 81 + e];
-              const C2<float> c0 = s_c[ci], c1 = s_c[ci + 1];
-              sr += p0 * (double)c0.re;
-              si += p0 * (double)c0.im;
-              sr += p1 * (double)c1.re;
-              si += p1 * (double)c1.im;
-            }
-            if (ci < ce) {
-              const double p0 = a.phi[(size_t)ci * 81 + e];
-              const C2<float> c0 = s_c[ci];
-              sr += p0 * (double)c0.re;
-              si += p0 * (double)c0.im;
-            }
-            s.u.setup.af[fl][e] = make_double2(sr, si);
-          }
-          __syncthreads();
-          for (int x = tid; x < nf * IB * 27; x += NT_DENSE) {
-            const int fl = x / (IB * 27), il = (x / 27) % IB, tsr = x % 27;
-            const int I = a.f[G.f_first + f0 + fl].I;
-            double2 sacc = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int uu = 0; uu < 3; uu++) {
-              const double2 v = zm(s.u.setup.af[fl][uu + 3 * tsr], s.e1[3 * I + uu][il]);
-              sacc.x += v.x;
-              sacc.y += v.y;
-            }
-            s.u.setup.b1[fl][il][tsr] = sacc;
-          }
-          __syncthreads();
-          if (tid < NOUT2) {
-            const int il = tid / (NB * 9), rem = tid % (NB * 9), j = rem / 9, sr = rem % 9;
-            for (int fl = 0; fl < nf; fl++) {
-              const int J = a.f[G.f_first + f0 + fl].J;
-#pragma unroll
-              for (int t = 0; t < 3; t++) {
-                const double2 v = zm(s.u.setup.b1[fl][il][t + 3 * sr], s.e2[3 * J + t][j]);
-                acc.x += v.x;
-                acc.y += v.y;
-              }
-            }
-          }
-          __syncthreads();
-        }
-        if (tid < NOUT2) {
-          if constexpr (MF == 1) {
-            Bf[((size_t)gi * NPAD + tid) * 2] = (float)acc.x;
-            Bf[((size_t)gi * NPAD + tid) * 2 + 1] = (float)acc.y;
-          } else {
-            unsigned rh, rl, ih, il;
-            if constexpr (MF == 3) {
-              f16_split((float)acc.x * a.a_scale, rh, rl);
-              f16_split((float)acc.y * a.a_scale, ih, il);
-            } else {
-              bf16_split((float)acc.x, rh, rl);
-              bf16_split((float)acc.y, ih, il);
-            }
-            Bh[(size_t)gi * NPAD + tid] = rh | (ih << 16);
-            Bl[(size_t)gi * NPAD + tid] = rl | (il << 16);
-          }
-        }
-      }
+      fold_groups<NB, IB, MF, NPAD>(a.g, a.f, a.phi, a.ng, a.a_scale, s, sw, s_c, Bh, Bl, Bf, wave, lane);
       __syncthreads();
       PROF(kPhSetup);
 
-      // prefetch of the next tile's global data into registers (as k_dense)
-      constexpr bool PF = (TQ * NB * NB <= NT_DENSE) && (10 * TQ * NB <= NT_DENSE);
+      // Registers carrying a later tile's global data, loaded ahead and
+      // written to LDS when that tile is staged.
+      constexpr bool PF = (TQ * NB * NB <= NT_DENSE) && (4 * TQ * NB <= NT_DENSE);
       double2 pf_e3 = make_double2(0.0, 0.0), pf_e4 = make_double2(0.0, 0.0);
       double pf_s = 0.0;
-      int pf_q = 0;
       bool have_pf = false;
       auto fetch = [&](int tb, int tqb) {
-        if (tid < TQ * NB * NB) {
+        if (tid < TQ * NB * NB) {   // eigenvector components of phonons 3 and 4
           const int t = tid / (NB * NB), comp = (tid / NB) % NB, b = tid % NB;
           if (t < tqb) {
             pf_e3 = m.eig[((size_t)s.vj[tb + t] * NB + b) * NB + comp];
             pf_e4 = m.eig[((size_t)s.vs[tb + t] * NB + b) * NB + comp];
           }
         }
-        if (tid < 10 * TQ * NB) {
+        if (tid < 4 * TQ * NB) {    // one of w'', w''', f'', f'''
           const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
           if (t < tqb) {
-            const int jv = s.vj[tb + t], sv = s.vs[tb + t];
-            if (f == 0)
-              pf_s = E(m, jv, b);
-            else if (f == 1)
-              pf_s = E(m, sv, b);
-            else if (f == 2)
-              pf_s = a.fbose[(size_t)b * m.nptk + jv];
-            else if (f == 3)
-              pf_s = a.fbose[(size_t)b * m.nptk + sv];
-            else if (f < 7)
-              pf_s = Vel(m, jv, b, f - 4);
-            else
-              pf_s = Vel(m, sv, b, f - 7);
+            const int q = (f == 0 || f == 2) ? s.vj[tb + t] : s.vs[tb + t];
+            pf_s = (f < 2) ? E(m, q, b) : a.fbose[(size_t)b * m.nptk + q];
           }
-        }
-        if (tid < 3 * TQ) {
-          const int t = tid / 3, c = tid % 3;
-          pf_q = m.ijk[3 * ((t < tqb) ? s.vj[tb + t] : 0) + c];
         }
       };
-      auto stash = [&](int tqb) {
-        if (tid < TQ * NB * NB) {
-          const int t = tid / (NB * NB), comp = (tid / NB) % NB, b = tid % NB;
-          C2<float> v3 = {0.f, 0.f}, v4 = {0.f, 0.f};
-          if (t < tqb) {
-            v3 = C2<float>{(float)pf_e3.x, (float)((ch != 0) ? -pf_e3.y : pf_e3.y)};
-            v4 = C2<float>{(float)pf_e4.x, (float)(-pf_e4.y)};
-          }
-          s.e3[t][comp][b] = v3;
-          s.e4[t][comp][b] = v4;
-        }
-        if (tid < 10 * TQ * NB) {
+      // filter inputs: one of w'', w''', v''(3), v'''(3) per thread
+      constexpr bool PFV = (8 * TQ * NB <= NT_DENSE);
+      double pf_v = 0.0;
+      auto fetchv = [&](int tb, int tqb) {
+        if (tid < 8 * TQ * NB) {
           const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
           if (t < tqb) {
-            if (f == 0)
-              s.Ejj[t][b] = pf_s;
-            else if (f == 1)
-              s.Ess[t][b] = pf_s;
-            else if (f == 2)
-              s.fb3[t][b] = pf_s;
-            else if (f == 3)
-              s.fb4[t][b] = pf_s;
-            else if (f < 7)
-              s.vjj[t][b][f - 4] = pf_s;
-            else
-              s.vss[t][b][f - 7] = pf_s;
+            const int q = (f == 0 || (f >= 2 && f < 5)) ? s.vj[tb + t] : s.vs[tb + t];
+            pf_v = (f < 2) ? E(m, q, b) : Vel(m, q, b, (f < 5) ? f - 2 : f - 5);
           }
         }
-        if (tid < 3 * TQ) s.qjj[tid / 3][tid % 3] = pf_q;
+      };
+      auto put_v = [&](FiltIn<NB> &F, int f, int t, int b, double v) {
+        if (f == 0)
+          F.Ej[t][b] = v;
+        else if (f == 1)
+          F.Es[t][b] = v;
+        else if (f < 5)
+          F.vj[t][b][f - 2] = v;
+        else
+          F.vs[t][b][f - 5] = v;
+      };
+      auto stagev = [&](int tqb, int fb) {   // from pf_v
+        if (tid < 8 * TQ * NB) {
+          const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
+          if (t < tqb) put_v(s.fi[fb], f, t, b, pf_v);
+        }
+      };
+      auto stagev_direct = [&](int tb, int tqb, int fb) {
+        for (int x = tid; x < 8 * TQ * NB; x += NT_DENSE) {
+          const int f = x / (TQ * NB), y = x % (TQ * NB), t = y / NB, b = y % NB;
+          if (t < tqb) {
+            const int q = (f == 0 || (f >= 2 && f < 5)) ? s.vj[tb + t] : s.vs[tb + t];
+            put_v(s.fi[fb], f, t, b, (f < 2) ? E(m, q, b) : Vel(m, q, b, (f < 5) ? f - 2 : f - 5));
+          }
+        }
+      };
+      // tile data of (tb, tqb) into buffer bf: eigenvectors of phonons 3 and
+      // 4, their frequencies, Bose-Einstein factors and reciprocals
+      auto stage = [&](int tb, int tqb, int bf) {
+        TileBuf<NB> &B = s.tb[bf];
+        if constexpr (PF) {
+          if (tid < TQ * NB * NB) {
+            const int t = tid / (NB * NB), comp = (tid / NB) % NB, b = tid % NB;
+            C2<float> v3 = {0.f, 0.f}, v4 = {0.f, 0.f};
+            if (t < tqb) {
+              v3 = C2<float>{(float)pf_e3.x, (float)((ch != 0) ? -pf_e3.y : pf_e3.y)};
+              v4 = C2<float>{(float)pf_e4.x, (float)(-pf_e4.y)};
+            }
+            B.e3[t][comp][b] = v3;
+            B.e4[t][comp][b] = v4;
+          }
+          if (tid < 4 * TQ * NB) {
+            const int f = tid / (TQ * NB), y = tid % (TQ * NB), t = y / NB, b = y % NB;
+            if (t < tqb) {
+              const double iv = (pf_s > 1.25) ? 1.0 / pf_s : 0.0;
+              if (f == 0) {
+                B.Ejj[t][b] = pf_s;
+                B.inv3[t][b] = iv;
+              } else if (f == 1) {
+                B.Ess[t][b] = pf_s;
+                B.inv4[t][b] = iv;
+              } else if (f == 2) {
+                B.fb3[t][b] = pf_s;
+              } else {
+                B.fb4[t][b] = pf_s;
+              }
+            }
+          }
+        } else {
+          for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
+            const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
+            C2<float> v3 = {0.f, 0.f}, v4 = {0.f, 0.f};
+            if (t < tqb) {
+              const double2 e3 = m.eig[((size_t)s.vj[tb + t] * NB + b) * NB + comp];
+              const double2 e4 = m.eig[((size_t)s.vs[tb + t] * NB + b) * NB + comp];
+              v3 = C2<float>{(float)e3.x, (float)((ch != 0) ? -e3.y : e3.y)};
+              v4 = C2<float>{(float)e4.x, (float)(-e4.y)};
+            }
+            B.e3[t][comp][b] = v3;
+            B.e4[t][comp][b] = v4;
+          }
+          for (int x = tid; x < TQ * NB; x += NT_DENSE) {
+            const int t = x / NB, b = x % NB;
+            if (t < tqb) {
+              const int jv = s.vj[tb + t], sv = s.vs[tb + t];
+              const double wj = E(m, jv, b), ws = E(m, sv, b);
+              B.Ejj[t][b] = wj;
+              B.Ess[t][b] = ws;
+              B.inv3[t][b] = (wj > 1.25) ? 1.0 / wj : 0.0;
+              B.inv4[t][b] = (ws > 1.25) ? 1.0 / ws : 0.0;
+              B.fb3[t][b] = a.fbose[(size_t)b * m.nptk + jv];
+              B.fb4[t][b] = a.fbose[(size_t)b * m.nptk + sv];
+            }
+          }
+        }
+      };
+      // phase operand of tile (tb, tqb) into buffer bf, tasks x0 .. x0 + 63:
+      // row (t, part), slot of group gi
+      auto phases = [&](int tb, int tqb, int bf, int x0) {
+        {
+          const int x = x0 + lane;
+          if (x >= TQ * a.ng) return;
+          const int t = x / a.ng, gi = x % a.ng;
+          double2 rt = make_double2(0.0, 0.0);
+          if (t < tqb) {
+            const int jv = s.vj[tb + t];
+            const int k = s3 * (m.ijk[3 * jv] * s_gd[3 * gi] + m.ijk[3 * jv + 1] * s_gd[3 * gi + 1] +
+                                m.ijk[3 * jv + 2] * s_gd[3 * gi + 2]);
+            const int kk = root_index(k, a.L, a.lmask);
+            rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
+          }
+          const int slot = s_gs[gi];
+          if constexpr (MF == 1) {
+            // part 0: (re, -im), part 1: (im, re) over (qq = 0, 1)
+            float *pb = Pf + (size_t)bf * 32 * a.gpt;
+            float *r0 = pb + ((size_t)(2 * t) * a.gpt + slot) * 2;
+            float *r1 = pb + ((size_t)(2 * t + 1) * a.gpt + slot) * 2;
+            r0[0] = (float)rt.x;
+            r0[1] = -(float)rt.y;
+            r1[0] = (float)rt.y;
+            r1[1] = (float)rt.x;
+          } else {
+            unsigned rh, rl, ih, il;
+            if constexpr (MF >= 3) {
+              f16_split((float)rt.x * kPhaseScaleF16, rh, rl);
+              f16_split((float)rt.y * kPhaseScaleF16, ih, il);
+            } else {
+              bf16_split((float)rt.x, rh, rl);
+              bf16_split((float)rt.y, ih, il);
+            }
+            // the sign flip is the top bit in both 16-bit formats
+            unsigned *ph = Ph + (size_t)bf * 32 * a.gpt, *pl = ph + (size_t)16 * a.gpt;
+            ph[(size_t)(2 * t) * a.gpt + slot] = rh | ((ih ^ 0x8000u) << 16);
+            ph[(size_t)(2 * t + 1) * a.gpt + slot] = ih | (rh << 16);
+            pl[(size_t)(2 * t) * a.gpt + slot] = rl | ((il ^ 0x8000u) << 16);
+            pl[(size_t)(2 * t + 1) * a.gpt + slot] = il | (rl << 16);
+          }
+        }
+      };
+      // filter of tile (tb, tqb) into buffer bf from the staged inputs fi[fb]:
+      // one (t, k, l) task per thread; screen by the sorted x1, exact sigma
+      // and exact test only inside the bracket
+      auto filter = [&](int tb, int tqb, int bf, int fb, int x0) {
+        TileBuf<NB> &B = s.tb[bf];
+        const FiltIn<NB> &F = s.fi[fb];
+        unsigned hitbits = 0u;
+        {
+          const int x = x0 + lane;
+          const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
+          unsigned long long mask = 0ull;
+          if (x < NKLT && t < tqb) {
+            const int jj0 = s.vj[tb + t], ss0 = s.vs[tb + t];
+            const double ej = F.Ej[t][k], es = F.Es[t][l];
+            if (ej != 0.0 && es != 0.0 && !(ch != 0 && ss0 == jj0 && l < k)) {
+              double v3[3], v4[3];
+#pragma unroll
+              for (int d = 0; d < 3; d++) {
+                v3[d] = F.vj[t][k][d];
+                v4[d] = F.vs[t][l][d];
+              }
+              const double odp = (ch == 0) ? ej : -ej;
+              const double otp = es;
+              const double c = otp - odp;
+              const float sf = sigma_estimate(a.bf, ch, v3, v4, a.sbf);
+              // sigma <= 1.0001 * estimate; 2^-40 (|x1| + |w''| + |w'''|) bounds the
+              // rounding of the two FP64 additions of the mismatch and of c -+ W
+              const double W = (double)sf * 1.0001 + 0x1p-40 * (xsmax + fabs(odp) + fabs(otp));
+              const double lo_v = c - W, hi_v = c + W;
+              // first p with xs[p] >= lo_v: count the fences xs[4q + 3] below
+              // lo_v (independent loads), then two binary steps in the block
+              constexpr int NFEN = NIJ / 4;
+              int p = 0;
+#pragma unroll
+              for (int q = 0; q < NFEN; q++) p += (s.xs[4 * q + 3] < lo_v) ? 4 : 0;
+              if (p + 2 <= NIJ && s.xs[p + 1] < lo_v) p += 2;
+              if (p + 1 <= NIJ && s.xs[p] < lo_v) p += 1;
+              if (p < NIJ && s.xs[p] <= hi_v) {
+                const double sg = tile_sigma(m, ch, v3, v4);
+                unsigned long long hitm = 0ull;
+                for (; p < NIJ; p++) {
+                  const double xv = s.xs[p];
+                  if (xv > hi_v) break;
+                  const double delta = (xv + odp) - otp;
+                  if (fabs(delta) <= sg) hitm |= 1ull << s.perm[p];
+                }
+                // exclusions of the reference's loops as bit masks over (i, j)
+                unsigned long long excl = 0ull;
+                if (ch != 1 && jj0 == ii0) excl |= kRep * (kJall & ~((2ull << k) - 1ull));   // j > k
+                if (ch == 1) {
+                  if (ii0 == jj0) excl |= kRep << k;                                         // j == k
+                  if (ii0 == ss0) excl |= kRep << l;                                         // j == l
+                  const int ilk = k - ib * IB, ill = l - ib * IB;
+                  if (qi0 == jj0 && ilk >= 0 && ilk < IB) excl |= kJall << (ilk * NB);      // i == k
+                  if (qi0 == ss0 && ill >= 0 && ill < IB) excl |= kJall << (ill * NB);      // i == l
+                }
+                mask = hitm & base_ok & ~excl;
+                if (mask) B.isg[t][k][l] = 1.0 / ((sg <= (double)kClamp) ? 1.0 / sqrt(kPi) : sg);
+              }
+            }
+          }
+          if (x < NKLT) {
+            B.mask[t][k][l] = mask;
+            if (mask) hitbits |= 1u << t;
+          }
+        }
+        unsigned hw = 0u;
+#pragma unroll
+        for (int t = 0; t < TQ; t++)
+          if (__ballot((hitbits >> t) & 1u) != 0ull) hw |= 1u << t;
+        if (lane == 0 && hw) atomicOr(&B.hitword, hw);
+      };
+      // the filter and phase work of one tile in chunks of 64 tasks, taken by
+      // whichever wave is free first (each task writes only its own results)
+      auto coop = [&](int tb, int tqb, int bf, int fb) {
+        const int nfc = (NKLT + 63) / 64, npc = (TQ * a.ng + 63) / 64;
+        for (;;) {
+          int c = 0;
+          if (lane == 0) c = atomicAdd(&s.coop_ctr[bf], 1);
+          c = __shfl(c, 0, 64);
+          if (c >= nfc + npc) break;
+          if (c < nfc)
+            filter(tb, tqb, bf, fb, 64 * c);
+          else
+            phases(tb, tqb, bf, 64 * (c - nfc));
+        }
       };
 
-      // ------------- q'' loop
+      // ------------- q'' loop: windows of up to NT_DENSE valid q'', tiles of
+      // TQ; tile T's per-wave work and the staging and filter of tile T + 1
+      // run between the same two barriers (double-buffered tile data)
       const int jj_first = (ch == 1) ? 0 : ii0;
+      int cur = 0;
       for (int base = jj_first; base < m.nptk; base += NT_DENSE) {
         const int jj = base + tid;
         int ok = 0, ssv = 0;
@@ -586,155 +820,36 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
         }
         __syncthreads();
         PROF(kPhScan);
-        for (int t0 = 0; t0 < tot; t0 += TQ) {
-          const int tq = min(TQ, tot - t0);
-          // ---- tile data
-          if (tid < TQ) {
-            s.jj[tid] = (tid < tq) ? s.vj[t0 + tid] : 0;
-            s.ss[tid] = (tid < tq) ? s.vs[t0 + tid] : 0;
-            s.hit[tid] = 0;
-          }
+        if (tot == 0) continue;
+        {
+          const int tq0 = min(TQ, tot);
+          // filter inputs of the window's first two tiles, then the third's into registers
+          stagev_direct(0, tq0, 0);
+          if (TQ < tot) stagev_direct(TQ, min(TQ, tot - TQ), 1);
+          if constexpr (PFV)
+            if (2 * TQ < tot) fetchv(2 * TQ, min(TQ, tot - 2 * TQ));
+          if constexpr (PF) fetch(0, tq0);
+          stage(0, tq0, cur);
           if constexpr (PF) {
-            if (!have_pf) fetch(t0, tq);
-            stash(tq);
-            const int tn = t0 + TQ;
-            have_pf = tn < tot;
-            if (have_pf) fetch(tn, min(TQ, tot - tn));
-          } else {
-            if (tid < TQ) {
-              const int jv = (tid < tq) ? s.vj[t0 + tid] : 0;
-#pragma unroll
-              for (int c = 0; c < 3; c++) s.qjj[tid][c] = m.ijk[3 * jv + c];
-            }
-            for (int x = tid; x < TQ * NB * NB; x += NT_DENSE) {
-              const int t = x / (NB * NB), comp = (x / NB) % NB, b = x % NB;
-              C2<float> v3 = {0.f, 0.f}, v4 = {0.f, 0.f};
-              if (t < tq) {
-                const double2 e3 = m.eig[((size_t)s.vj[t0 + t] * NB + b) * NB + comp];
-                const double2 e4 = m.eig[((size_t)s.vs[t0 + t] * NB + b) * NB + comp];
-                v3 = C2<float>{(float)e3.x, (float)((ch != 0) ? -e3.y : e3.y)};
-                v4 = C2<float>{(float)e4.x, (float)(-e4.y)};
-              }
-              s.e3[t][comp][b] = v3;
-              s.e4[t][comp][b] = v4;
-            }
-            for (int x = tid; x < TQ * NB; x += NT_DENSE) {
-              const int t = x / NB, b = x % NB;
-              if (t < tq) {
-                const int jv = s.vj[t0 + t], sv = s.vs[t0 + t];
-                s.Ejj[t][b] = E(m, jv, b);
-                s.Ess[t][b] = E(m, sv, b);
-                s.fb3[t][b] = a.fbose[(size_t)b * m.nptk + jv];
-                s.fb4[t][b] = a.fbose[(size_t)b * m.nptk + sv];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                  s.vjj[t][b][d] = Vel(m, jv, b, d);
-                  s.vss[t][b][d] = Vel(m, sv, b, d);
-                }
-              }
-            }
+            have_pf = TQ < tot;
+            if (have_pf) fetch(TQ, min(TQ, tot - TQ));
           }
           PROF(kPhTile);
           __syncthreads();
           PROF(kPhBar1);
-          if (tid < 2 * TQ * NB) {
-            const int t = (tid / NB) % TQ, b = tid % NB;
-            const double w = (tid < TQ * NB) ? s.Ejj[t][b] : s.Ess[t][b];
-            const double iv = (t < tq && w > 1.25) ? 1.0 / w : 0.0;
-            if (tid < TQ * NB)
-              s.inv3[t][b] = iv;
-            else
-              s.inv4[t][b] = iv;
-          }
-          // ---- phase operand of the tile: row (t, part), slot of group gi
-          for (int x = tid; x < TQ * a.ng; x += NT_DENSE) {
-            const int t = x / a.ng, gi = x % a.ng;
-            double2 rt = make_double2(0.0, 0.0);
-            if (t < tq) {
-              const int k = s3 * (s.qjj[t][0] * s_gd[3 * gi] + s.qjj[t][1] * s_gd[3 * gi + 1] +
-                                  s.qjj[t][2] * s_gd[3 * gi + 2]);
-              const int kk = root_index(k, a.L, a.lmask);
-              rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
-            }
-            const int slot = s_gs[gi];
-            if constexpr (MF == 1) {
-              // part 0: (re, -im), part 1: (im, re) over (qq = 0, 1)
-              float *r0 = Pf + ((size_t)(2 * t) * a.gpt + slot) * 2;
-              float *r1 = Pf + ((size_t)(2 * t + 1) * a.gpt + slot) * 2;
-              r0[0] = (float)rt.x;
-              r0[1] = -(float)rt.y;
-              r1[0] = (float)rt.y;
-              r1[1] = (float)rt.x;
-            } else {
-              unsigned rh, rl, ih, il;
-              if constexpr (MF == 3) {
-                f16_split((float)rt.x * kPhaseScaleF16, rh, rl);
-                f16_split((float)rt.y * kPhaseScaleF16, ih, il);
-              } else {
-                bf16_split((float)rt.x, rh, rl);
-                bf16_split((float)rt.y, ih, il);
-              }
-              // the sign flip is the top bit in both 16-bit formats
-              Ph[(size_t)(2 * t) * a.gpt + slot] = rh | ((ih ^ 0x8000u) << 16);
-              Ph[(size_t)(2 * t + 1) * a.gpt + slot] = ih | (rh << 16);
-              Pl[(size_t)(2 * t) * a.gpt + slot] = rl | ((il ^ 0x8000u) << 16);
-              Pl[(size_t)(2 * t + 1) * a.gpt + slot] = il | (rl << 16);
-            }
-          }
-          // ---- filter: one (t, k, l) task per thread; screen by the sorted
-          // x1, exact sigma and exact test only inside the bracket
-          for (int x = tid; x < NKLT; x += NT_DENSE) {
-            const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
-            unsigned long long mask = 0ull;
-            double sg = -1.0;
-            if (t < tq && s.Ejj[t][k] != 0.0 && s.Ess[t][l] != 0.0 && !(ch != 0 && s.ss[t] == s.jj[t] && l < k)) {
-              const double odp = (ch == 0) ? s.Ejj[t][k] : -s.Ejj[t][k];
-              const double otp = s.Ess[t][l];
-              const double c = otp - odp;
-              const float sf = sigma_estimate(a.bf, ch, s.vjj[t][k], s.vss[t][l], a.sbf);
-              // sigma <= 1.0001 * estimate; 2^-40 (|x1| + |w''| + |w'''|) bounds the
-              // rounding of the two FP64 additions of the mismatch and of c -+ W
-              const double W = (double)sf * 1.0001 + 0x1p-40 * (xsmax + fabs(odp) + fabs(otp));
-              const double lo_v = c - W, hi_v = c + W;
-              // first p with xs[p] >= lo_v
-              int p = 0;
-#pragma unroll
-              for (int stp = 64; stp > 0; stp >>= 1)
-                if (stp < 2 * NIJ && p + stp <= NIJ && s.xs[p + stp - 1] < lo_v) p += stp;
-              if (p < NIJ && s.xs[p] <= hi_v) {
-                sg = tile_sigma(m, ch, s.vjj[t][k], s.vss[t][l]);
-                unsigned long long hitm = 0ull;
-                for (; p < NIJ; p++) {
-                  const double xv = s.xs[p];
-                  if (xv > hi_v) break;
-                  const double delta = (xv + odp) - otp;
-                  if (fabs(delta) <= sg) hitm |= 1ull << s.perm[p];
-                }
-                // exclusions of the reference's loops as bit masks over (i, j)
-                const int jj0 = s.jj[t], ss0 = s.ss[t];
-                unsigned long long excl = 0ull;
-                if (ch != 1 && jj0 == ii0) excl |= kRep * (kJall & ~((2ull << k) - 1ull));   // j > k
-                if (ch == 1) {
-                  if (ii0 == jj0) excl |= kRep << k;                                         // j == k
-                  if (ii0 == ss0) excl |= kRep << l;                                         // j == l
-                  const int ilk = k - ib * IB, ill = l - ib * IB;
-                  if (qi0 == jj0 && ilk >= 0 && ilk < IB) excl |= kJall << (ilk * NB);      // i == k
-                  if (qi0 == ss0 && ill >= 0 && ill < IB) excl |= kJall << (ill * NB);      // i == l
-                }
-                mask = hitm & base_ok & ~excl;
-                if (mask) s.hit[t] = 1;
-              }
-            }
-            s.mask[t][k][l] = mask;
-            if (mask) s.isg[t][k][l] = 1.0 / ((sg <= (double)kClamp) ? 1.0 / sqrt(kPi) : sg);
-          }
+          coop(0, tq0, cur, 0);
           PROF(kPhFilter);
           __syncthreads();
-          PROF(kPhBar2);
-
+          PROF(kPhBar1);
+        }
+        for (int t0 = 0; t0 < tot; t0 += TQ) {
+          const TileBuf<NB> &B = s.tb[cur];
+          const unsigned *Phc = Ph + (size_t)cur * 32 * a.gpt;
+          const unsigned *Plc = Phc + (size_t)16 * a.gpt;
+          const float *Pfc = Pf + (size_t)cur * 32 * a.gpt;
           // ---- per wave: survivors of the owned (i, j), Psi slice, contraction, rates
           if (ijh > ijl) {
-            const unsigned long long *mflat = &s.mask[0][0][0];
+            const unsigned long long *mflat = &B.mask[0][0][0];
             int total = 0;
             for (int cbase = 0; cbase == 0 || cbase < total; cbase += QW) {
               // queue the survivors with positions [cbase, cbase + QW) in
@@ -752,7 +867,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                   while (mk) {
                     const int b = __builtin_ctzll(mk);
                     mk &= mk - 1ull;
-                    if (pos >= cbase && pos < cbase + QW) s.u.run.q[wave][pos - cbase] = (unsigned)x | ((unsigned)b << 16);
+                    if (pos >= cbase && pos < cbase + QW) run.q[wave][pos - cbase] = (unsigned)x | ((unsigned)b << 16);
                     pos++;
                   }
                 }
@@ -767,7 +882,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
 #pragma unroll
               for (int u = 0; u < QPT; u++) {
                 const int qi = lane + 64 * u;
-                info[u] = (qi < qn) ? (int)s.u.run.q[wave][qi] : -1;
+                info[u] = (qi < qn) ? (int)run.q[wave][qi] : -1;
                 V[u] = C2<float>{0.f, 0.f};
                 const int x = info[u] & 0xFFFF, ij = (info[u] >> 16) & 63;
                 const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
@@ -775,9 +890,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                 o_e3[u] = t * NB * NB + k;
                 o_e4[u] = t * NB * NB + l;
               }
-              const C2<float> *psiw = &s.u.run.psi[wave][0][0];
-              const C2<float> *e3f = &s.e3[0][0][0];
-              const C2<float> *e4f = &s.e4[0][0][0];
+              const C2<float> *psiw = &run.psi[wave][0][0];
+              const C2<float> *e3f = &B.e3[0][0][0];
+              const C2<float> *e4f = &B.e4[0][0][0];
               if (!a.onlyharmonic) {
                 for (int kl = 0; kl < NA * NA; kl++) {
                   const int g0 = a.klf[kl], g1 = a.klf[kl + 1];
@@ -790,7 +905,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                   for (int ks = 0; ks < nks; ks++) {
                     if constexpr (MF == 1) {
                       const int g = g0 + 2 * ks + (kg >> 1), qq = kg & 1;
-                      const float av = Pf[((size_t)row * a.gpt + a.pair_off[kl] + 2 * ks + (kg >> 1)) * 2 + qq];
+                      const float av = Pfc[((size_t)row * a.gpt + a.pair_off[kl] + 2 * ks + (kg >> 1)) * 2 + qq];
                       const int gr = (g < g1) ? g : g1 - 1;
                       const float *bp = Bf + ((size_t)gr * NPAD + colbase + n) * 2 + qq;
                       float bv[CTW];
@@ -807,17 +922,27 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                         roff[e] = ((g < g1) ? g : g1 - 1) * NPAD + colbase + n;
                       }
                       if constexpr (GPL == 4) {
-                        const uint4 ah = *reinterpret_cast<const uint4 *>(Ph + (size_t)row * a.gpt + slot);
-                        const uint4 al = *reinterpret_cast<const uint4 *>(Pl + (size_t)row * a.gpt + slot);
+                        const uint4 ah = *reinterpret_cast<const uint4 *>(Phc + (size_t)row * a.gpt + slot);
+                        const uint4 al = *reinterpret_cast<const uint4 *>(Plc + (size_t)row * a.gpt + slot);
                         uint4 bh[CTW], bl[CTW];
 #pragma unroll
                         for (int c = 0; c < CTW; c++) {
                           bh[c] = make_uint4(Bh[roff[0] + 16 * c], Bh[roff[1] + 16 * c], Bh[roff[2] + 16 * c],
                                              Bh[roff[3] + 16 * c]);
-                          bl[c] = make_uint4(Bl[roff[0] + 16 * c], Bl[roff[1] + 16 * c], Bl[roff[2] + 16 * c],
-                                             Bl[roff[3] + 16 * c]);
+                          if constexpr (MF != 4)
+                            bl[c] = make_uint4(Bl[roff[0] + 16 * c], Bl[roff[1] + 16 * c], Bl[roff[2] + 16 * c],
+                                               Bl[roff[3] + 16 * c]);
                         }
-                        if constexpr (MF == 3) {
+                        if constexpr (MF == 4) {
+                          // phases hi + lo against the single A' plane
+                          const f16x8 xh = __builtin_bit_cast(f16x8, ah), xl = __builtin_bit_cast(f16x8, al);
+#pragma unroll
+                          for (int c = 0; c < CTW; c++)
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, __builtin_bit_cast(f16x8, bh[c]), acc[c], 0, 0, 0);
+#pragma unroll
+                          for (int c = 0; c < CTW; c++)
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, __builtin_bit_cast(f16x8, bh[c]), acc[c], 0, 0, 0);
+                        } else if constexpr (MF == 3) {
                           const f16x8 xh = __builtin_bit_cast(f16x8, ah), xl = __builtin_bit_cast(f16x8, al);
 #pragma unroll
                           for (int c = 0; c < CTW; c++)
@@ -841,15 +966,23 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                             acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, __builtin_bit_cast(bf16x8, bh[c]), acc[c], 0, 0, 0);
                         }
                       } else {
-                        const uint2 ah = *reinterpret_cast<const uint2 *>(Ph + (size_t)row * a.gpt + slot);
-                        const uint2 al = *reinterpret_cast<const uint2 *>(Pl + (size_t)row * a.gpt + slot);
+                        const uint2 ah = *reinterpret_cast<const uint2 *>(Phc + (size_t)row * a.gpt + slot);
+                        const uint2 al = *reinterpret_cast<const uint2 *>(Plc + (size_t)row * a.gpt + slot);
                         uint2 bh[CTW], bl[CTW];
 #pragma unroll
                         for (int c = 0; c < CTW; c++) {
                           bh[c] = make_uint2(Bh[roff[0] + 16 * c], Bh[roff[1] + 16 * c]);
-                          bl[c] = make_uint2(Bl[roff[0] + 16 * c], Bl[roff[1] + 16 * c]);
+                          if constexpr (MF != 4) bl[c] = make_uint2(Bl[roff[0] + 16 * c], Bl[roff[1] + 16 * c]);
                         }
-                        if constexpr (MF == 3) {
+                        if constexpr (MF == 4) {
+                          const f16x4 xh = __builtin_bit_cast(f16x4, ah), xl = __builtin_bit_cast(f16x4, al);
+#pragma unroll
+                          for (int c = 0; c < CTW; c++)
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(xh, __builtin_bit_cast(f16x4, bh[c]), acc[c], 0, 0, 0);
+#pragma unroll
+                          for (int c = 0; c < CTW; c++)
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(xl, __builtin_bit_cast(f16x4, bh[c]), acc[c], 0, 0, 0);
+                        } else if constexpr (MF == 3) {
                           const f16x4 xh = __builtin_bit_cast(f16x4, ah), xl = __builtin_bit_cast(f16x4, al);
 #pragma unroll
                           for (int c = 0; c < CTW; c++)
@@ -881,8 +1014,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
 #pragma unroll
                     for (int c = 0; c < CTW; c++) {
                       const int col = 16 * c + n;
-                      s.u.run.psi[wave][t0r][col] = C2<float>{acc[c][0] * psi_unscale, acc[c][1] * psi_unscale};
-                      s.u.run.psi[wave][t0r + 1][col] = C2<float>{acc[c][2] * psi_unscale, acc[c][3] * psi_unscale};
+                      run.psi[wave][t0r][col] = C2<float>{acc[c][0] * psi_unscale, acc[c][1] * psi_unscale};
+                      run.psi[wave][t0r + 1][col] = C2<float>{acc[c][2] * psi_unscale, acc[c][3] * psi_unscale};
                     }
                   }
                   lds_wave_sync();
@@ -915,12 +1048,12 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                   const int x = info[u] & 0xFFFF, ij = info[u] >> 16;
                   const int l = x % NB, k = (x / NB) % NB, t = x / (NB * NB);
                   const int il = ij / NB, j = ij % NB;
-                  const double w = s.om[il], op = s.Eii[j], odp = s.Ejj[t][k], otp = s.Ess[t][l];
+                  const double w = s.om[il], op = s.Eii[j], odp = B.Ejj[t][k], otp = B.Ess[t][l];
                   const double delta = mismatch(ch, w, op, odp, otp);
                   // WP4 of processes.f90:3895-3901 with the divisions as
                   // reciprocals (1 / sigma of the clamped sigma, 1 / (w w'),
                   // 1 / w'', 1 / w'''; the last three are 0 below the cut)
-                  const double fp = s.fb1[j], fdp = s.fb3[t][k], ftp = s.fb4[t][l];
+                  const double fp = s.fb1[j], fdp = B.fb3[t][k], ftp = B.fb4[t][l];
                   double occ;
                   if (ch == 0)
                     occ = fp * fdp * (1 + ftp) - (1 + fp) * (1 + fdp) * ftp;
@@ -928,9 +1061,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                     occ = fp * (1 + fdp) * (1 + ftp) - (1 + fp) * fdp * ftp;
                   else
                     occ = (1 + fp) * (1 + fdp) * (1 + ftp) - fp * fdp * ftp;
-                  const double is = s.isg[t][k][l];
+                  const double is = B.isg[t][k][l];
                   const double wpv = occ * exp(-(delta * delta) * (is * is)) * (is * kInvSqrtPi) *
-                                     (s.inv12[ij] * s.inv3[t][k] * s.inv4[t][l]);
+                                     (s.inv12[ij] * B.inv3[t][k] * B.inv4[t][l]);
                   double g = 0.0;
                   if (!a.onlyharmonic) {
                     const double vr = (double)V[u].re, vi = (double)V[u].im;
@@ -949,9 +1082,39 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
               PROF(kPhRate);
             }
           }
-          if (tid < tq && s.hit[tid]) nhit++;
+          if (tid == 0) {
+            // this tile's hits; its buffer's word and chunk counter are next
+            // used two tiles on, after the coming barrier
+            nhit += __popc(s.tb[cur].hitword);
+            s.tb[cur].hitword = 0u;
+            s.coop_ctr[cur] = 0;
+          }
+          // staging, phase operand and filter of the next tile, into the other buffer
+          const int tn = t0 + TQ;
+          if (tn < tot) {
+            const int tqn = min(TQ, tot - tn);
+            const int tn2 = tn + TQ, nt = tn / TQ;   // nt: window tile index of tn
+            if (tn2 < tot) {
+              // filter inputs of tile tn2, then of the tile after it into registers
+              if constexpr (PFV) {
+                stagev(min(TQ, tot - tn2), (nt + 1) & 1);
+                if (tn2 + TQ < tot) fetchv(tn2 + TQ, min(TQ, tot - tn2 - TQ));
+              } else {
+                stagev_direct(tn2, min(TQ, tot - tn2), (nt + 1) & 1);
+              }
+            }
+            stage(tn, tqn, cur ^ 1);
+            if constexpr (PF) {
+              have_pf = tn2 < tot;
+              if (have_pf) fetch(tn2, min(TQ, tot - tn2));
+            }
+            PROF(kPhTile);
+            coop(tn, tqn, cur ^ 1, nt & 1);
+            PROF(kPhFilter);
+          }
           __syncthreads();
           PROF(kPhBar3);
+          cur ^= 1;
         }
       }
     }
@@ -1009,8 +1172,8 @@ size_t stat2() {
 template <int NB, int IB, int MF, int KG>
 size_t bytes2(const DenseGeom &g) {
   const int gpt = dense2_gpt(g, KG);
-  return dyn2_layout(stat2<NB, IB, MF, KG>(), npad_of<NB, IB, MF, KG>(), g.ng, g.ntri, gpt,
-                     g.L <= kRootsLds ? g.L : 0)
+  return dyn2_layout<IB, Own<IB * NB>::CTW>(stat2<NB, IB, MF, KG>(), npad_of<NB, IB, MF, KG>(), g.ng, g.ntri, gpt,
+                                             g.L <= kRootsLds ? g.L : 0, MF == 4 ? 4 : 8)
       .end;
 }
 
@@ -1037,11 +1200,16 @@ int per_cu2(size_t dyn) {
 
 const DenseShape kShapes2[] = {
     FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 16), FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 8),
+    FP4_SHAPE2(FP4_PREC_BF16X3, 6, 3, 2, 8),  FP4_SHAPE2(FP4_PREC_FP16X3, 6, 3, 3, 8),
+    FP4_SHAPE2(FP4_PREC_FP32, 6, 3, 1, 2),    FP4_SHAPE2(FP4_PREC_FP16X2, 6, 3, 4, 8),
     FP4_SHAPE2(FP4_PREC_BF16X3, 3, 3, 2, 8),  FP4_SHAPE2(FP4_PREC_BF16X3, 9, 3, 2, 8),
     FP4_SHAPE2(FP4_PREC_BF16X3, 12, 3, 2, 8), FP4_SHAPE2(FP4_PREC_BF16X3, 12, 2, 2, 8),
     FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 16), FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 8),
     FP4_SHAPE2(FP4_PREC_FP16X3, 3, 3, 3, 8),  FP4_SHAPE2(FP4_PREC_FP16X3, 9, 3, 3, 8),
     FP4_SHAPE2(FP4_PREC_FP16X3, 12, 3, 3, 8), FP4_SHAPE2(FP4_PREC_FP16X3, 12, 2, 3, 8),
+    FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 16), FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 8),
+    FP4_SHAPE2(FP4_PREC_FP16X2, 3, 3, 4, 8),  FP4_SHAPE2(FP4_PREC_FP16X2, 9, 3, 4, 8),
+    FP4_SHAPE2(FP4_PREC_FP16X2, 12, 3, 4, 8), FP4_SHAPE2(FP4_PREC_FP16X2, 12, 2, 4, 8),
     FP4_SHAPE2(FP4_PREC_FP32, 6, 6, 1, 2),    FP4_SHAPE2(FP4_PREC_FP32, 3, 3, 1, 2),
     FP4_SHAPE2(FP4_PREC_FP32, 9, 3, 1, 2),    FP4_SHAPE2(FP4_PREC_FP32, 12, 3, 1, 2),
     FP4_SHAPE2(FP4_PREC_FP32, 12, 2, 1, 2),

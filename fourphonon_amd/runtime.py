@@ -28,6 +28,7 @@ PREC_FP64_DENSE = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
 PREC_FP16X3 = 4
+PREC_FP16X2 = 5
 
 ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR_OVERFLOW", 5: "FP4_ERR_STATE",
           6: "FP4_ERR_UNSUPPORTED"}

@@ -43,7 +43,9 @@ enum {
   FP4_PREC_FP64_DENSE = 1, /* dense fold + contraction, FP64 */
   FP4_PREC_FP32 = 2,       /* dense fold + contraction on FP32 MFMA */
   FP4_PREC_BF16X3 = 3,     /* dense, BF16 hi/lo split (3 products) on MFMA, FP32 accumulate */
-  FP4_PREC_FP16X3 = 4      /* dense, FP16 hi/lo split (3 products, power-of-two operand scaling) on MFMA */
+  FP4_PREC_FP16X3 = 4,     /* dense, FP16 hi/lo split (3 products, power-of-two operand scaling) on MFMA */
+  FP4_PREC_FP16X2 = 5      /* dense, FP16 hi/lo split of the phase factors only (2 products, power-of-two
+                              scaling) on MFMA, FP32 accumulate */
 };
 
 typedef struct fp4_ctx fp4_ctx;
