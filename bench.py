@@ -4,17 +4,19 @@ Workload: the synthetic diamond crystal (a = 5.43 A, m = 28.085 amu, spring
 FC2, seeded second-shell fourth-order IFCs) on a 16x16x16 Gamma-centred mesh
 (4096 q, 6 branches, 145 irreducible q, 870 modes), RTA at 300 K,
 scalebroad 1.  A step is one RTA_driver_4ph pass over all 870 modes
-(all three channels) in the dense MFMA form (BF16 hi/lo by default, FP32 with
---precision fp32); value = processes/s of that pass, with every input
-resident in HBM before the timed region.
+(all three channels) in the dense MFMA form (FP16 with hi/lo phase factors
+against one FP16 A' operand by default; BF16 hi/lo, FP16 hi/lo or FP32 with
+--precision); value = processes/s of that pass, with every input resident in
+HBM before the timed region.
 
 Also reported: the FP64 per-process form (the reference-faithful kernel) and
 the dense form on the same stratified sample (every 29th mode plus the modes
 with the fewest and most processes, second-phonon window q' = 1..64), the
 per-mode rate and RTA kappa errors of the FP32 form against the dense FP64
 form on the full workload, the dominant kernel's roofline numbers (HIP events
-on the library's stream), and the CPU oracle timed on the host cores on a
-bounded sample.
+on the library's stream), and the CPU oracle timed on all host cores on a
+bounded piece of the same stratified sample, with the GPU forms timed on that
+same piece beside it.
 
 Library calls are synchronous (they return after their stream has drained),
 so a wall clock around the steps brackets exactly the device work.
@@ -39,6 +41,40 @@ FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 matrix peak
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0    # dense BF16 matrix peak
 FP16_PEAK_TFLOPS = 2500.0    # dense FP16 matrix peak
+CPU_QP = 4                   # q' window of the CPU baseline piece (about 20 s on 16 cores)
+
+
+# MFMA forms: (library precision name, matrix peak in TFLOP/s of the issued datatype, dtype label)
+FORMS = {
+    "fp16x2": ("PREC_FP16X2", FP16_PEAK_TFLOPS, "fp16x2 (fp16 MFMA, hi/lo phase factors against one fp16 A' operand, "
+                                                "fp32 accumulate)"),
+    "bf16x3": ("PREC_BF16X3", BF16_PEAK_TFLOPS, "bf16x3 (bf16 hi/lo MFMA, fp32 accumulate)"),
+    "fp16x3": ("PREC_FP16X3", FP16_PEAK_TFLOPS, "fp16x3 (fp16 hi/lo MFMA, fp32 accumulate)"),
+    "fp32": ("PREC_FP32", FP32_PEAK_TFLOPS, "fp32"),
+}
+
+
+def prec_of(rt, name):
+    return getattr(rt, FORMS[name][0])
+
+
+def kernel_name(ctx, nb):
+    """Name of the last dense launch's kernel, as in the kernel trace."""
+    shape, kg = ctx.dense_shape(), ctx.dense_kernel()
+    form = ["valu", "fp32", "bf16x3", "fp16x3", "fp16x2"][shape["mfma"]]
+    if kg > 0:
+        return f"k_dense2<{nb},{shape['ib']},{form},kg{kg}>"
+    return f"k_dense<{nb},{shape['ib']},{shape['tq']},{form}>"
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(*a):
@@ -130,8 +166,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-form and accuracy legs")
     ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape instead of the autotuner's")
-    ap.add_argument("--precision", choices=["fp32", "bf16x3", "fp16x3"], default="bf16x3",
-                    help="MFMA form of the timed steps: FP32 MFMA, BF16 hi/lo or FP16 hi/lo MFMA (FP32 accumulate)")
+    ap.add_argument("--precision", choices=list(FORMS), default="fp16x2",
+                    help="MFMA form of the timed steps: FP16 with hi/lo phases (fp16x2), BF16 or FP16 hi/lo "
+                         "(3 products), or FP32 MFMA; FP32 accumulation throughout")
     ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16"], default="rta16",
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
                          "4ph RTA + 3ph iterative kappa at 100/300/700 K; sample16: the sampling estimator on "
@@ -170,7 +207,7 @@ def main():
     log(f"prefilter: {nproc_total} processes in {t_pref:.2f}s")
 
     # ---- timed dense (MFMA) steps over the whole workload
-    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
+    prec = prec_of(rt, args.precision)
     tuned = None
     if not args.no_autotune:
         # kernel shape from the autotuner, on every 29th mode over q' = 1..64 (untimed)
@@ -199,9 +236,8 @@ def main():
     executed = dense_flops(case, info["ngroups"], info["nfine"], ntriples, nproc_total, info["items"])
     avg_ms = k_ms / max(k_n, 1)
     achieved = flops / (avg_ms * 1e-3) / 1e12
-    shape = ctx.dense_shape()
-    kname = f"k_dense<float,{m.nbands},{shape['ib']},{shape['tq']},{['valu', 'mfma', 'mfma-bf16x3', 'mfma-fp16x3'][shape['mfma']]}>"
-    peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS, "fp16x3": FP16_PEAK_TFLOPS}[args.precision]
+    kname = kernel_name(ctx, m.nbands)
+    peak = FORMS[args.precision][1]
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname),
                 "kernel": kname, "kernel_ms": round(avg_ms, 3), "launches": k_n,
@@ -213,8 +249,7 @@ def main():
         "metric": "4ph processes/s on 1 MI355X, MFMA form vs FP64 form, kappa within 1e-4 of FP64",
         "value": value, "unit": "processes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "fp32", "bf16x3": "bf16x3 (bf16 hi/lo MFMA, fp32 accumulate)",
-                  "fp16x3": "fp16x3 (fp16 hi/lo MFMA, fp32 accumulate)"}[args.precision], "data": "synthetic",
+        "dtype": FORMS[args.precision][2], "data": "synthetic",
         "config": {"workload": "diamond 16x16x16, 2nd-shell 4th-order IFCs (Ntri=1522), 4ph RTA at 300 K, "
                                "all 870 modes", "ngrid": [16, 16, 16], "nbands": 6, "nlist": int(m.Nlist),
                    "ntri4": int(case.ifc4.Ntri), "processes": nproc_total, "triples": ntriples},
@@ -261,18 +296,18 @@ def main():
         b = r64["rate_scatt_4ph"].ravel(order="F")
         nz = b != 0
         out["rate_max_rel_err"] = float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])))
+        out["rate_p99_rel_err"] = float(np.percentile(np.abs(a[nz] - b[nz]) / np.abs(b[nz]), 99))
         k32, k64 = kappa_rta(case, a), kappa_rta(case, b)
         out["kappa_rta_trace_fp64"] = float(np.trace(k64) / 3)
         out["kappa_rel_err"] = float(abs(np.trace(k32) - np.trace(k64)) / abs(np.trace(k64)))
         log(f"rate max rel err {out['rate_max_rel_err']:.2e}, kappa rel err {out['kappa_rel_err']:.2e}")
 
         # the other MFMA forms, one pass each over the whole workload
-        forms = {"fp32": (rt.PREC_FP32, FP32_PEAK_TFLOPS), "bf16x3": (rt.PREC_BF16X3, BF16_PEAK_TFLOPS),
-                 "fp16x3": (rt.PREC_FP16X3, FP16_PEAK_TFLOPS)}
         out["other_mfma_forms"] = []
-        for other, (oprec, opeak) in forms.items():
+        for other in FORMS:
             if other == args.precision:
                 continue
+            oprec, opeak = prec_of(rt, other), FORMS[other][1]
             ctx.reset_stats()
             t1 = time.perf_counter()
             ro, no = ctx.RTA_driver_4ph_modes(all_modes, oprec)
@@ -281,7 +316,8 @@ def main():
             oach = dense_algorithmic_flops(case, ctx.dense_info()["triples"]) / (o_ms * 1e-3) / 1e12
             ao = ro["rate_scatt_4ph"].ravel(order="F")
             out["other_mfma_forms"].append({
-                "precision": other, "value": no / to, "unit": "processes/s", "kernel_ms": o_ms,
+                "precision": other, "kernel": kernel_name(ctx, m.nbands), "value": no / to, "unit": "processes/s",
+                "kernel_ms": o_ms,
                 "achieved_tflops": round(oach, 3), "peak": opeak, "frac": round(oach / opeak, 4),
                 "rate_max_rel_err": float(np.max(np.abs(ao[nz] - b[nz]) / np.abs(b[nz]))),
                 "kappa_rel_err": float(abs(np.trace(kappa_rta(case, ao)) - np.trace(k64)) / abs(np.trace(k64)))})
@@ -306,20 +342,52 @@ def main():
                                "rate_kernel_frac_fp64_peak": f3 / (ms_r3 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
         log(f"3ph: {n3} processes, counts {t_np3:.2f}s, FP64 rates {t_r3:.2f}s")
 
-    # ---- CPU baseline: the oracle on the host cores, bounded sample
+    # ---- CPU baseline: the oracle on all host cores, on a bounded piece of
+    # the stratified sample the FP64 form runs (its modes, q' = 1..CPU_QP),
+    # split into (mode, channel, q') items over a thread pool (the oracle's C
+    # calls release the GIL); the GPU forms are timed on the same piece
     if not args.no_cpu:
         try:
+            from concurrent.futures import ThreadPoolExecutor
             from oracle.oracle import Oracle
-            cores = min(16, len(os.sched_getaffinity(0)))
+            tot = Npp + Npm + Nmm
+            act = np.nonzero(tot > 0)[0] + 1
+            sample = np.unique(np.concatenate([act[::29], [act[np.argmin(tot[act - 1])],
+                                                            act[np.argmax(tot[act - 1])]]]))
+            cores = len(os.sched_getaffinity(0))
+            qp = CPU_QP
             o = Oracle(case)
-            cs = np.array([5, 118, 333, 500, 611, 777, 850, nm], np.int32)
+            items = [(ch, int(mm), q) for mm in sample for ch in range(3) for q in range(1, qp + 1)]
             t1 = time.perf_counter()
-            o.rta_driver(modes=cs, ii_lo=1, ii_hi=2, nthreads=cores)
+            with ThreadPoolExecutor(max_workers=cores) as pool:
+                cpu_g = list(pool.map(lambda it: o.rta_mode(it[0], it[1], it[2], it[2])[0], items))
             tc = time.perf_counter() - t1
-            Ncs = ctx.RTA_driver_4ph_modes(cs, rt.PREC_FP64, ii_lo=1, ii_hi=2)[1]
-            out["cpu_baseline"] = {"value": Ncs / tc, "unit": "processes/s", "cores": cores, "kind": "port",
-                                   "sample": f"FP64 C oracle (OpenMP dynamic,1), {len(cs)} modes, q'=1..2, "
-                                             f"{Ncs} processes in {tc:.1f}s"}
+            ctx.reset_stats()
+            t1 = time.perf_counter()
+            g64, n_cs = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP64, ii_lo=1, ii_hi=qp)
+            tg64 = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            gd, _ = ctx.RTA_driver_4ph_modes(sample, prec, ii_lo=1, ii_hi=qp)
+            tgd = time.perf_counter() - t1
+            # the oracle's per-item sums, against the GPU FP64 form on the same piece
+            cpu_rate = np.zeros(3 * len(sample))
+            for (ch, mm, q), g in zip(items, cpu_g):
+                cpu_rate[ch * len(sample) + int(np.searchsorted(sample, mm))] += g
+            keys = ("rate_scatt_plusplus", "rate_scatt_plusminus", "rate_scatt_minusminus")
+            gpu_rate = np.concatenate([g64[k].ravel(order="F")[sample - 1] for k in keys])
+            nzc = gpu_rate != 0
+            err = float(np.max(np.abs(cpu_rate[nzc] - gpu_rate[nzc]) / np.abs(gpu_rate[nzc])))
+            out["cpu_baseline"] = {
+                "value": n_cs / tc, "unit": "processes/s", "cores": cores, "kind": "port",
+                "cpu_model": cpu_model(),
+                "sample": f"FP64 C oracle, {len(sample)} modes (every 29th active mode + min/max count) x 3 "
+                          f"channels, q'=1..{qp}: {len(items)} (mode, channel, q') items over {cores} threads, "
+                          f"{n_cs} processes in {tc:.1f}s",
+                "gpu_fp64_form_same_piece": {"value": n_cs / tg64, "seconds": tg64},
+                "gpu_mfma_form_same_piece": {"value": n_cs / tgd, "seconds": tgd},
+                "gpu_fp64_vs_cpu_rate_max_rel_err": err}
+            log(f"CPU oracle {n_cs / tc:.3e} proc/s on {cores} cores; GPU FP64 form {n_cs / tg64:.3e}, "
+                f"MFMA form {n_cs / tgd:.3e} on the same piece")
         except Exception as e:  # the baseline must not hide the GPU result
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     ctx.close()
@@ -337,7 +405,7 @@ def run_sample16(args):
 
     case = crystal.config_case(2)
     ctx = rt.Context(case)
-    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
+    prec = prec_of(rt, args.precision)
     t1 = time.perf_counter()
     full = ctx.RTA_driver_4ph(prec)["rate_scatt_4ph"].ravel(order="F")
     t_full = time.perf_counter() - t1
@@ -380,7 +448,7 @@ def run_iterative20(args):
     ctx = rt.Context(case)
     nm = ctx.nmodes
     all_modes = np.arange(1, nm + 1, dtype=np.int32)
-    prec = {"bf16x3": rt.PREC_BF16X3, "fp16x3": rt.PREC_FP16X3}.get(args.precision, rt.PREC_FP32)
+    prec = prec_of(rt, args.precision)
     N4 = [np.zeros(nm, np.int64) for _ in range(3)]
     L4 = [[np.zeros(1, np.int64)] * 3 for _ in range(3)]
     G4 = [np.zeros(1)] * 3

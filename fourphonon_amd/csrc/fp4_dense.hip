@@ -212,7 +212,8 @@ struct DenseSmem {
   double red[NT_DENSE / 64][3 * IB];
 };
 
-// MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi, 3 = FP16 hi/lo MFMA Psi.
+// MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi, 3 = FP16 hi/lo MFMA Psi,
+// 4 = FP16 Psi with hi/lo phases against one A' plane.
 template <class T, int NB, int IB, int TQ, int MF>
 __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   using S = DenseSmem<T, NB, IB, TQ>;
@@ -226,7 +227,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   // FP16 form: A' is scaled by a_scale (a power of two that bounds it by 2^14)
   // and the phases by kPhaseScaleF16 before the split; Psi is unscaled after
   // the GEMM, exactly (powers of two)
-  const float psi_unscale = (MF == 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
+  const float psi_unscale = (MF >= 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
   static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
   constexpr unsigned long long kJall = (1ull << NB) - 1ull;   // bits of all j of one i
   constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
@@ -385,8 +386,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
           __syncthreads();
         }
         if (tid < NOUT2) {
-          if constexpr (MF == 3) {
+          if constexpr (MF == 3 || MF == 4) {
             // hi plane and lo plane as in the BF16 form, binary16 values of A' * a_scale
+            // (the FP16X2 form reads the hi plane only)
             unsigned rh, rl, ih, il;
             f16_split((float)acc.x * a.a_scale, rh, rl);
             f16_split((float)acc.y * a.a_scale, ih, il);
@@ -556,11 +558,11 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
               const int kk = fmod_pos(k, a.L);
               rt = a.roots_lds ? s_rt[kk] : a.roots[kk];
             }
-            if constexpr (MF == 2 || MF == 3) {
+            if constexpr (MF == 2 || MF == 3 || MF == 4) {
               // A operand rows (t, part), k = (g, qq): part 0 (re, -im), part 1 (im, re), hi and lo
               // planes; the sign flip is the top bit in both 16-bit formats
               unsigned rh, rl, ih, il;
-              if constexpr (MF == 3) {
+              if constexpr (MF >= 3) {
                 f16_split((float)rt.x * kPhaseScaleF16, rh, rl);
                 f16_split((float)rt.y * kPhaseScaleF16, ih, il);
               } else {
@@ -664,7 +666,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                     }
                   }
                 }
-              } else if constexpr (MF == 2 || MF == 3) {
+              } else if constexpr (MF == 2 || MF == 3 || MF == 4) {
                 // BF16 (FP16) hi/lo form: Psi = A_hi B_hi + A_hi B_lo + A_lo B_hi
                 // on v_mfma_f32_16x16x16bf16_1k (v_mfma_f32_16x16x16f16), FP32
                 // accumulation; a k-step is 8 groups, lane group kg holding
@@ -701,7 +703,15 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                     const bool w0 = v0 && nin[c], w1 = v1 && nin[c];
                     const unsigned bh0 = w0 ? Bh[b0] : 0u, bh1 = w1 ? Bh[b1] : 0u;
                     const unsigned bl0 = w0 ? Bl[b0] : 0u, bl1 = w1 ? Bl[b1] : 0u;
-                    if constexpr (MF == 3) {
+                    if constexpr (MF == 4) {
+                      // hi/lo phases against the single FP16 A' plane
+                      const f16x4 ah = f16_pack(ph0, ph1), al = f16_pack(pl0, pl1);
+                      const f16x4 bh = f16_pack(bh0, bh1);
+                      (void)bl0;
+                      (void)bl1;
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[c], 0, 0, 0);
+                      acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[c], 0, 0, 0);
+                    } else if constexpr (MF == 3) {
                       const f16x4 ah = f16_pack(ph0, ph1), al = f16_pack(pl0, pl1);
                       const f16x4 bh = f16_pack(bh0, bh1), bl = f16_pack(bl0, bl1);
                       acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[c], 0, 0, 0);
@@ -1154,20 +1164,27 @@ const DenseShape kShapes1[] = {
     FP4_SHAPE(float, FP4_PREC_FP16X3, 3, 3, 8, 3),           FP4_SHAPE(float, FP4_PREC_FP16X3, 6, 6, 8, 3),
     FP4_SHAPE(float, FP4_PREC_FP16X3, 6, 2, 8, 3),           FP4_SHAPE(float, FP4_PREC_FP16X3, 9, 3, 8, 3),
     FP4_SHAPE(float, FP4_PREC_FP16X3, 12, 1, 8, 3),
+    FP4_SHAPE(float, FP4_PREC_FP16X2, 3, 3, 8, 4),           FP4_SHAPE(float, FP4_PREC_FP16X2, 6, 6, 8, 4),
+    FP4_SHAPE(float, FP4_PREC_FP16X2, 9, 3, 8, 4),           FP4_SHAPE(float, FP4_PREC_FP16X2, 12, 1, 8, 4),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 3, 3, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 3, 4, 0),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 6, 1, 4, 0),      FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 9, 1, 2, 0),
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, 0),
 };
 
-// All shapes in order of preference (k_dense2 first); the first whose LDS
-// fits is used when no tuning is given.
+// All shapes in order of preference; the first whose LDS fits is used when
+// no tuning is given: k_dense2 with every branch of a point in one block
+// (IB = Nb) first, then k_dense, then the remaining k_dense2 shapes (fewer
+// modes per block share less work; the autotuner still times them).
 const std::vector<DenseShape> &all_shapes() {
   static const std::vector<DenseShape> v = [] {
     std::vector<DenseShape> r;
     int n2 = 0;
     const DenseShape *s2 = dense2_shapes(&n2);
-    r.insert(r.end(), s2, s2 + n2);
+    for (int i = 0; i < n2; i++)
+      if (s2[i].IB == s2[i].nb) r.push_back(s2[i]);
     r.insert(r.end(), std::begin(kShapes1), std::end(kShapes1));
+    for (int i = 0; i < n2; i++)
+      if (s2[i].IB != s2[i].nb) r.push_back(s2[i]);
     return r;
   }();
   return v;

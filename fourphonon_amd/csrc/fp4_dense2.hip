@@ -178,6 +178,7 @@ struct Smem2 {
   int coop_ctr[2];   // chunks of the next tile's filter and phase work taken so far, by buffer
   long item;
   double red[NW][2][3];                 // per wave: sums of its (at most) two modes
+  unsigned long long base_ok;           // (i, j) pairs allowed for the whole item
 };
 
 // LDS used only in the q'' loop: each wave's Psi slice for one (K, L) pair
@@ -421,7 +422,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
 
   const ModeView &m = a.m;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform within a wave; readfirstlane lets the compiler keep
+  // everything derived from it (ownership range, slices) in scalar registers
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   SetupW<IB> &sw = reinterpret_cast<SetupW<IB> *>(dyn + dl.W)[wave];
   const double hbarp = kHbar * (double)k1e22;
   const double gpre = hbarp * hbarp * kPi / 8.0;
@@ -499,13 +502,13 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
 #pragma unroll
     for (int il = 0; il < IB; il++) any |= s.act[il];
     PROF(kPhItem);
-    unsigned long long base_ok = 0ull;
-#pragma unroll
-    for (int il = 0; il < IB; il++)
-#pragma unroll
-      for (int j = 0; j < NB; j++)
-        if (s.act[il] && s.Eii[j] != 0.0) base_ok |= 1ull << (il * NB + j);
-
+    if (tid == 0) {
+      unsigned long long bo = 0ull;
+      for (int il = 0; il < IB; il++)
+        for (int j = 0; j < NB; j++)
+          if (s.act[il] && s.Eii[j] != 0.0) bo |= 1ull << (il * NB + j);
+      s.base_ok = bo;
+    }
     double acc_g[2] = {0.0, 0.0}, acc_w[2] = {0.0, 0.0};
     int acc_n[2] = {0, 0};
     int nhit = 0;
@@ -541,7 +544,6 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
         s.perm[rk] = tid;
       }
       __syncthreads();
-      const double xsmax = fmax(fabs(s.xs[0]), fabs(s.xs[NIJ - 1]));
       // ------------- setup 2: A' of every coarse group, into the B layout
       fold_groups<NB, IB, MF, NPAD>(a.g, a.f, a.phi, a.ng, a.a_scale, s, sw, s_c, Bh, Bl, Bf, wave, lane);
       __syncthreads();
@@ -736,6 +738,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
               const float sf = sigma_estimate(a.bf, ch, v3, v4, a.sbf);
               // sigma <= 1.0001 * estimate; 2^-40 (|x1| + |w''| + |w'''|) bounds the
               // rounding of the two FP64 additions of the mismatch and of c -+ W
+              const double xsmax = fmax(fabs(s.xs[0]), fabs(s.xs[NIJ - 1]));
               const double W = (double)sf * 1.0001 + 0x1p-40 * (xsmax + fabs(odp) + fabs(otp));
               const double lo_v = c - W, hi_v = c + W;
               // first p with xs[p] >= lo_v: count the fences xs[4q + 3] below
@@ -765,7 +768,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                   if (qi0 == jj0 && ilk >= 0 && ilk < IB) excl |= kJall << (ilk * NB);      // i == k
                   if (qi0 == ss0 && ill >= 0 && ill < IB) excl |= kJall << (ill * NB);      // i == l
                 }
-                mask = hitm & base_ok & ~excl;
+                mask = hitm & s.base_ok & ~excl;
                 if (mask) B.isg[t][k][l] = 1.0 / ((sg <= (double)kClamp) ? 1.0 / sqrt(kPi) : sg);
               }
             }

@@ -67,13 +67,13 @@ def test_dense_visits_exactly_the_prefilter_processes(pair):
     modes = np.arange(1, c.nmodes + 1)
     _, n64 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64)
     _, n32 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP32)
-    assert c.dense_kernel() > 0          # the wave-owned-slice kernel with its sorted-x1 screen
     _, nb16 = c.RTA_driver_4ph_modes(modes, rt.PREC_BF16X3)
-    assert c.dense_kernel() > 0
     _, nf16 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP16X3)
+    _, nh16 = c.RTA_driver_4ph_modes(modes, rt.PREC_FP16X2)
+    assert c.dense_kernel() > 0          # the wave-owned-slice kernel with its sorted-x1 screen
     _, nd = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64_DENSE)
     assert c.dense_kernel() == 0
-    assert n64 == n32 == nb16 == nf16 == nd > 0
+    assert n64 == n32 == nb16 == nf16 == nh16 == nd > 0
 
 
 def test_dense_fp32_tolerances(pair):
@@ -92,10 +92,12 @@ def test_dense_fp32_tolerances(pair):
     assert kerr <= 1e-4
 
 
-@pytest.mark.parametrize("prec,mf", [(rt.PREC_BF16X3, 2), (rt.PREC_FP16X3, 3)], ids=["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("prec,mf", [(rt.PREC_BF16X3, 2), (rt.PREC_FP16X3, 3), (rt.PREC_FP16X2, 4)],
+                         ids=["bf16x3", "fp16x3", "fp16x2"])
 def test_dense_split_form_tolerances(pair, prec, mf):
-    """BF16 and FP16 hi/lo MFMA forms: per-mode rates within 1e-3 and kappa
-    within 1e-4 of FP64."""
+    """BF16 and FP16 hi/lo MFMA forms (three products) and the FP16 form with
+    hi/lo phase factors only (two products): per-mode rates within 1e-3 and
+    kappa within 1e-4 of FP64."""
     case, c, o = pair
     banner(f"dense hi/lo form {prec} vs FP64")
     ref = c.RTA_driver_4ph(rt.PREC_FP64)
@@ -119,7 +121,8 @@ def test_dense_golden_diamond_8cube():
     banner("dense vs golden diamond 8x8x8")
     c = rt.Context(crystal.config_case(1))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3),
+                      (rt.PREC_FP16X2, 1e-3)):
         r, n = c.RTA_driver_4ph_modes(rm, prec)
         assert n == int(g["N"][:, rm - 1].sum())
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
@@ -136,7 +139,8 @@ def test_dense_golden_diamond_16cube_window():
     banner("dense vs golden diamond 16x16x16 shell 2 (q' window)")
     c = rt.Context(crystal.config_case(2))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3),
+                      (rt.PREC_FP16X2, 1e-3)):
         r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
                         ("rate_scatt_minusminus", "rate_mm")):
@@ -154,18 +158,20 @@ def test_dense_golden_wurtzite_12x12x8_window():
     banner("dense vs golden wurtzite 12x12x8 shell 2 (q' window)")
     c = rt.Context(crystal.config_case(3))
     rm = g["rate_modes"]
-    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3)):
+    for prec, tol in ((rt.PREC_FP64_DENSE, 1e-9), (rt.PREC_FP32, 1e-3), (rt.PREC_BF16X3, 1e-3), (rt.PREC_FP16X3, 1e-3),
+                      (rt.PREC_FP16X2, 1e-3)):
         r, _ = c.RTA_driver_4ph_modes(rm, prec, ii_lo=int(g["ii_lo"]), ii_hi=int(g["ii_hi"]))
         for key, wk in (("rate_scatt_plusplus", "rate_pp"), ("rate_scatt_plusminus", "rate_pm"),
                         ("rate_scatt_minusminus", "rate_mm")):
             assert rel_err(r[key].ravel(order="F")[rm - 1], g[wk]) <= tol, (prec, key)
         if prec != rt.PREC_FP64_DENSE:
-            assert c.dense_shape()["mfma"] == {rt.PREC_FP32: 1, rt.PREC_BF16X3: 2, rt.PREC_FP16X3: 3}[prec]
+            assert c.dense_shape()["mfma"] == {rt.PREC_FP32: 1, rt.PREC_BF16X3: 2, rt.PREC_FP16X3: 3,
+                                               rt.PREC_FP16X2: 4}[prec]
     c.close()
 
 
-@pytest.mark.parametrize("prec", [rt.PREC_BF16X3, rt.PREC_FP16X3, rt.PREC_FP32, rt.PREC_FP64_DENSE],
-                         ids=["bf16x3", "fp16x3", "fp32", "fp64"])
+@pytest.mark.parametrize("prec", [rt.PREC_FP16X2, rt.PREC_BF16X3, rt.PREC_FP16X3, rt.PREC_FP32, rt.PREC_FP64_DENSE],
+                         ids=["fp16x2", "bf16x3", "fp16x3", "fp32", "fp64"])
 def test_autotuner_picks_a_built_shape_and_keeps_the_rates(prec):
     """The autotuner times the built shapes on a sample and keeps the fastest;
     later calls run that shape and give the same rates as the default shape
