@@ -127,16 +127,15 @@ def dense_flops(case, ngroups, nfine, ntriples, nproc, nitems):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
-    None when no summary covers this kernel."""
+    FETCH_SIZE / WRITE_SIZE summaries (profiles/*traffic*.md, calibrated
+    against a copy kernel), or None when no summary covers this kernel."""
+    import glob
     import re
-    path = os.path.join(ROOT, "profiles", "dense_bf16x3_diamond16_shell2_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        mt = re.match(r"traffic_bytes_per_launch:\s*(\d+)\s+kernel\s+(\S+)", line.strip())
-        if mt and mt.group(2) == kernel:
-            return int(mt.group(1))
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        for line in open(path):
+            mt = re.match(r"traffic_bytes_per_launch:\s*(\d+)\s+kernel\s+(\S+)", line.strip())
+            if mt and mt.group(2) == kernel:
+                return int(mt.group(1))
     return None
 
 
@@ -173,12 +172,17 @@ def main():
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
                          "4ph RTA + 3ph iterative kappa at 100/300/700 K; sample16: the sampling estimator on "
                          "diamond 16^3 against the full rates")
+    ap.add_argument("--phase-profile", action="store_true",
+                    help="load the profiling build (libfp4prof.so) and print the dense kernel's per-phase cycle "
+                         "shares of the timed steps")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.phase_profile:
+        os.environ["FP4_PROFILE_BUILD"] = "1"   # read by fourphonon_amd.runtime at import
     if args.copy_bytes > 0:
         copy_calibration(args.copy_bytes)
     if args.workload == "iterative20":
@@ -225,6 +229,11 @@ def main():
         log(f"step {s + 1}/{args.steps} done")
     wall = time.perf_counter() - t1
     assert nproc == nproc_total, (nproc, nproc_total)
+    phases = None
+    if args.phase_profile:
+        ph = ctx.dense_phase_profile()
+        tot_c = sum(ph.values())
+        phases = {k: round(v / tot_c, 4) for k, v in ph.items()}
     k_ms, k_n = ctx.kernel_stats(rt.Context.KS_DENSE)
     ms_step = 1e3 * wall / args.steps
     value = nproc_total * args.steps / wall
@@ -257,6 +266,8 @@ def main():
         "autotune": tuned,
         "roofline": roofline,
     }
+    if phases is not None:
+        out["phase_shares_of_wave_cycles"] = phases
 
     # ---- FP64 form and dense form on the same stratified sample
     if not args.no_fp64:

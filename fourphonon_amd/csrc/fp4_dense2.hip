@@ -1202,15 +1202,15 @@ int per_cu2(size_t dyn) {
   {P, NB, IB, 8, MF, KG, &bytes2<NB, IB, MF, KG>, &stat2<NB, IB, MF, KG>, &launch2<NB, IB, MF, KG>, &per_cu2<NB, IB, MF, KG>}
 
 const DenseShape kShapes2[] = {
-    FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 16), FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 8),
+    FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 8),  FP4_SHAPE2(FP4_PREC_BF16X3, 6, 6, 2, 16),
     FP4_SHAPE2(FP4_PREC_BF16X3, 6, 3, 2, 8),  FP4_SHAPE2(FP4_PREC_FP16X3, 6, 3, 3, 8),
     FP4_SHAPE2(FP4_PREC_FP32, 6, 3, 1, 2),    FP4_SHAPE2(FP4_PREC_FP16X2, 6, 3, 4, 8),
     FP4_SHAPE2(FP4_PREC_BF16X3, 3, 3, 2, 8),  FP4_SHAPE2(FP4_PREC_BF16X3, 9, 3, 2, 8),
     FP4_SHAPE2(FP4_PREC_BF16X3, 12, 3, 2, 8), FP4_SHAPE2(FP4_PREC_BF16X3, 12, 2, 2, 8),
-    FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 16), FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 8),
+    FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 8),  FP4_SHAPE2(FP4_PREC_FP16X3, 6, 6, 3, 16),
     FP4_SHAPE2(FP4_PREC_FP16X3, 3, 3, 3, 8),  FP4_SHAPE2(FP4_PREC_FP16X3, 9, 3, 3, 8),
     FP4_SHAPE2(FP4_PREC_FP16X3, 12, 3, 3, 8), FP4_SHAPE2(FP4_PREC_FP16X3, 12, 2, 3, 8),
-    FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 16), FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 8),
+    FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 8),  FP4_SHAPE2(FP4_PREC_FP16X2, 6, 6, 4, 16),
     FP4_SHAPE2(FP4_PREC_FP16X2, 3, 3, 4, 8),  FP4_SHAPE2(FP4_PREC_FP16X2, 9, 3, 4, 8),
     FP4_SHAPE2(FP4_PREC_FP16X2, 12, 3, 4, 8), FP4_SHAPE2(FP4_PREC_FP16X2, 12, 2, 4, 8),
     FP4_SHAPE2(FP4_PREC_FP32, 6, 6, 1, 2),    FP4_SHAPE2(FP4_PREC_FP32, 3, 3, 1, 2),
