@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -25,6 +26,14 @@
 #include "fp4_kernels.hpp"
 
 using namespace fp4;
+
+// Device scratch of the allocation-free entry point fp4_np_4ph_modes_noalloc
+// (per-item counts and phase-space partials, per-segment totals, the mode
+// subset).  Every GPU the library runs on holds its own copy for the life of
+// the process.  In host code the name stands for no device address: the
+// address on each device comes from hipGetSymbolAddress, taken once per
+// device at context creation (scratch_setup) and kept in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[FP4_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -45,6 +54,29 @@ int fail(int code, const char *fmt, ...) {
     hipError_t e_ = (x);                                                                   \
     if (e_ != hipSuccess) return fail(FP4_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
   } while (0)
+
+// Device address of `scratch` per device ordinal, set by scratch_setup.
+constexpr int kMaxScratchDevices = 64;
+void *g_scratch_dev[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// With `device` current: looks up the device's copy of `scratch` once.
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return 0;  // the no-alloc entry point then reports it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void *p = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(FP4_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+  g_scratch_dev[device] = p;
+  return 0;
+}
+
+void *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
 
 struct DevBuf {
   void *p = nullptr;
@@ -563,6 +595,10 @@ int fp4_ctx_create(const fp4_config *cfg, fp4_ctx **out) {
     delete c;
     return fail(FP4_ERR_DEVICE, "stream/event creation failed");
   }
+  if (int rc = scratch_setup(c->device)) {
+    fp4_ctx_destroy(c);
+    return rc;
+  }
   for (int t = 0; t < 3; t++) c->ngrid[t] = cfg->ngrid[t];
   c->nptk = cfg->ngrid[0] * cfg->ngrid[1] * cfg->ngrid[2];
   c->nb = cfg->nbands;
@@ -1002,6 +1038,74 @@ int fp4_np_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t *
       N[ch][m] = pl.segN[ch * nmodes + m];
       P[ch][m] = pl.segP[ch * nmodes + m];
     }
+  return FP4_OK;
+}
+
+// fp4_np_4ph_modes on the device's `scratch` array instead of context
+// buffers: the same count-pass and reduction kernels, no device allocation.
+// Scratch layout (256-byte aligned parts): modes[nmodes] int32,
+// segN[3*nmodes] int64, segP[3*nmodes] double, cnt[items] int32,
+// psum[items] double, items = 3*nmodes*nptk.  The count pass writes every
+// cnt/psum entry and the reduction every segN/segP entry, so nothing left
+// by an earlier call is read.
+int fp4_np_4ph_modes_noalloc(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp,
+                             int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!N_pp || !P_pp || !N_pm || !P_pm || !N_mm || !P_mm) return fail(FP4_ERR_ARG, "null output");
+  if (nmodes < 1 || !modes) return fail(FP4_ERR_ARG, "bad mode subset");
+  for (int t = 0; t < nmodes; t++) {
+    if (modes[t] < 1 || modes[t] > c->nb * c->nlist) return fail(FP4_ERR_ARG, "mode %d out of range", modes[t]);
+    if (t && modes[t] <= modes[t - 1]) return fail(FP4_ERR_ARG, "modes must be strictly increasing");
+  }
+  unsigned char *base = static_cast<unsigned char *>(scratch_ptr(c->device));
+  if (!base) return fail(FP4_ERR_STATE, "no scratch address for device %d", c->device);
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t nseg = 3 * (size_t)nmodes, items = nseg * (size_t)c->nptk;
+  const size_t o_segN = up(sizeof(int) * (size_t)nmodes);
+  const size_t o_segP = o_segN + up(sizeof(long long) * nseg);
+  const size_t o_cnt = o_segP + up(sizeof(double) * nseg);
+  const size_t o_psum = o_cnt + up(sizeof(int) * items);
+  const size_t need = o_psum + sizeof(double) * items;
+  if (need > (size_t)FP4_NOALLOC_SCRATCH_BYTES) {
+    const size_t per_mode = 3 * (size_t)c->nptk * (sizeof(int) + sizeof(double)) + 3 * 16 + 4;
+    return fail(FP4_ERR_ARG,
+                "%d modes need %zu bytes of scratch, the limit is %zu (FP4_NOALLOC_SCRATCH_BYTES): pass at most "
+                "about %zu modes per call",
+                nmodes, need, (size_t)FP4_NOALLOC_SCRATCH_BYTES,
+                (size_t)(FP4_NOALLOC_SCRATCH_BYTES - 4 * 256) / per_mode);
+  }
+  int *d_modes = reinterpret_cast<int *>(base);
+  long long *d_segN = reinterpret_cast<long long *>(base + o_segN);
+  double *d_segP = reinterpret_cast<double *>(base + o_segP);
+  int *d_cnt = reinterpret_cast<int *>(base + o_cnt);
+  double *d_psum = reinterpret_cast<double *>(base + o_psum);
+  HIPCHECK(hipMemcpyAsync(d_modes, modes, sizeof(int) * nmodes, hipMemcpyHostToDevice, c->stream));
+  PrefilterArgs a{};
+  a.m = mode_view(c);
+  a.modes = d_modes;
+  a.nsub = nmodes;
+  a.nii = c->nptk;
+  a.ii_lo0 = 0;
+  a.item_lo = 0;
+  a.item_hi = (long)items;
+  a.cnt = d_cnt;
+  a.psum = d_psum;
+  if ((rc = timed(c, KS_PREFILTER, [&] {
+         return launch_prefilter(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+       })))
+    return rc;
+  launch_reduce_counts(d_cnt, d_psum, (int)nseg, c->nptk, d_segN, d_segP, c->stream);
+  HIPCHECK(hipGetLastError());
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  double *P[3] = {P_pp, P_pm, P_mm};
+  for (int ch = 0; ch < 3; ch++) {
+    HIPCHECK(hipMemcpyAsync(N[ch], d_segN + (size_t)ch * nmodes, sizeof(long long) * nmodes, hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHECK(hipMemcpyAsync(P[ch], d_segP + (size_t)ch * nmodes, sizeof(double) * nmodes, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
   return FP4_OK;
 }
 

@@ -37,7 +37,7 @@ ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR
 EXPORTS = [
     "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
-    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_ind_only_4ph_modes",
+    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_np_4ph_modes_noalloc", "fp4_ind_only_4ph_modes",
     "fp4_ctx_kernel_stats",
     "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape", "fp4_ctx_dense_kernel", "fp4_ctx_dense_candidates", "fp4_ctx_set_dense_shape", "fp4_ctx_dense_phase_profile",
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
@@ -198,6 +198,15 @@ class Context:
         P = [np.zeros(len(modes)) for _ in range(3)]
         _check(load().fp4_np_4ph_modes(self.h, C.c_int32(len(modes)), _p(modes), _p(N[0]), _p(P[0]), _p(N[1]),
                                        _p(P[1]), _p(N[2]), _p(P[2])))
+        return N[0], P[0], N[1], P[1], N[2], P[2]
+
+    def NP_driver_4ph_modes_noalloc(self, modes):
+        """NP_driver_4ph_modes through the allocation-free entry point (fixed per-GPU scratch array)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        N = [np.zeros(len(modes), np.int64) for _ in range(3)]
+        P = [np.zeros(len(modes)) for _ in range(3)]
+        _check(load().fp4_np_4ph_modes_noalloc(self.h, C.c_int32(len(modes)), _p(modes), _p(N[0]), _p(P[0]),
+                                               _p(N[1]), _p(P[1]), _p(N[2]), _p(P[2])))
         return N[0], P[0], N[1], P[1], N[2], P[2]
 
     def Ind_only_driver_4ph(self, N_pp, N_pm, N_mm):

@@ -144,6 +144,22 @@ int fp4_vp_4ph(fp4_ctx *ctx, int32_t channel, int64_t n, const int32_t *idx, dou
 int fp4_np_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp,
                      int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm);
 
+/* Allocation-free form of fp4_np_4ph_modes.  It has no counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments and results as fp4_np_4ph_modes (bit for
+   bit); its working memory is a fixed array of FP4_NOALLOC_SCRATCH_BYTES
+   per GPU, held until the process ends, instead of context buffers, and it
+   calls no device allocation or free.  A subset needs
+   3*nmodes*nptk*12 bytes (plus a few hundred), so at most about
+   FP4_NOALLOC_SCRATCH_BYTES / (36*nptk) modes fit one call (227 at 16^3);
+   larger subsets are rejected with FP4_ERR_ARG and a message naming the
+   limit.  All calls on one GPU share the array: calls that overlap in time
+   on one GPU (from several threads or contexts) race and must not be made. */
+#define FP4_NOALLOC_SCRATCH_BYTES (32u * 1024u * 1024u)
+int fp4_np_4ph_modes_noalloc(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp,
+                             int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm);
+
 /* Ind_only_driver_4ph restricted to a strictly increasing subset of modes
    (mm, 1-based), for callers that split the mode range the way the
    reference's MPI chunks do (Src/processes.f90:3160-3203).  N_* receive the
