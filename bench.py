@@ -22,9 +22,11 @@ Library calls are synchronous (they return after their stream has drained),
 so a wall clock around the steps brackets exactly the device work.
 
 --workload iterative20 instead runs the diamond 20x20x20 iterative workload
-(4ph RTA + 3ph iterative kappa at 100/300/700 K) and prints its own line;
+(the 3ph+4ph operator assembled on the device, iterative kappa at
+100/300/700 K) and prints its own line;
 --workload sample16 runs the sampling estimator of RTA_driver_4ph on the
-16x16x16 crystal against the full rates.
+16x16x16 crystal against the full rates; --workload wurtzite12 runs every
+MFMA form on the 12x12x8 wurtzite against the dense FP64 form.
 """
 import argparse
 import json
@@ -168,10 +170,11 @@ def main():
     ap.add_argument("--precision", choices=list(FORMS), default="fp16x2",
                     help="MFMA form of the timed steps: FP16 with hi/lo phases (fp16x2), BF16 or FP16 hi/lo "
                          "(3 products), or FP32 MFMA; FP32 accumulation throughout")
-    ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16"], default="rta16",
+    ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16", "wurtzite12"], default="rta16",
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
-                         "4ph RTA + 3ph iterative kappa at 100/300/700 K; sample16: the sampling estimator on "
-                         "diamond 16^3 against the full rates")
+                         "iterative kappa with the assembled 3ph+4ph operator at 100/300/700 K; sample16: the sampling estimator on "
+                         "diamond 16^3 against the full rates; wurtzite12: every MFMA form on the 12x12x8 "
+                         "wurtzite against dense FP64 (per-mode and kappa errors)")
     ap.add_argument("--phase-profile", action="store_true",
                     help="load the profiling build (libfp4prof.so) and print the dense kernel's per-phase cycle "
                          "shares of the timed steps")
@@ -190,6 +193,9 @@ def main():
         return
     if args.workload == "sample16":
         run_sample16(args)
+        return
+    if args.workload == "wurtzite12":
+        run_wurtzite12(args)
         return
 
     from fourphonon_amd import crystal
@@ -443,11 +449,103 @@ def run_sample16(args):
                       "samples": recs}), flush=True)
 
 
+def run_wurtzite12(args):
+    """Precision form on the 4-atom wurtzite-like cell (a = 3.2 A, c = 5.2 A,
+    masses 26.98/14.01 amu, 12 branches) at 12x12x8 with second-shell FC4,
+    RTA at 300 K: every MFMA form over all modes against the dense FP64 form,
+    which is itself checked against the oracle's FP64 rates (golden fixture
+    cfg3.npz, its q' window) and against the per-process FP64 form on a
+    stratified sample.  Prints one JSON line with per-form time, processes/s,
+    per-mode max and p99 rate error and RTA kappa error."""
+    from fourphonon_amd import crystal
+    from fourphonon_amd import runtime as rt
+
+    t0 = time.time()
+    case = crystal.config_case(3)
+    m = case.modes
+    log(f"case built in {time.time() - t0:.1f}s: nptk={m.nptk} Nb={m.nbands} Nlist={m.Nlist} Ntri={case.ifc4.Ntri}")
+    ctx = rt.Context(case)
+    nm = ctx.nmodes
+    all_modes = np.arange(1, nm + 1, dtype=np.int32)
+    t1 = time.perf_counter()
+    Npp, _, Npm, _, Nmm, _ = ctx.NP_driver_4ph()
+    t_pref = time.perf_counter() - t1
+    nproc_total = int(Npp.sum() + Npm.sum() + Nmm.sum())
+    log(f"prefilter: {nproc_total} processes in {t_pref:.2f}s")
+    keys = ("rate_scatt_plusplus", "rate_scatt_plusminus", "rate_scatt_minusminus")
+
+    # dense FP64 over the whole mesh: the accuracy reference of the MFMA forms
+    t1 = time.perf_counter()
+    r64, n64 = ctx.RTA_driver_4ph_modes(all_modes, rt.PREC_FP64_DENSE)
+    t64 = time.perf_counter() - t1
+    b = r64["rate_scatt_4ph"].ravel(order="F")
+    nz = b != 0
+    k64 = np.trace(kappa_rta(case, b)) / 3
+    log(f"dense FP64: {t64:.1f}s, RTA kappa {k64:.4f}")
+
+    # the dense FP64 form against the oracle (golden window) and the
+    # per-process FP64 form (stratified sample, q' = 1..16)
+    checks = {}
+    gpath = os.path.join(ROOT, "tests", "golden", "cfg3.npz")
+    if os.path.exists(gpath):
+        g = np.load(gpath)
+        rm, lo, hi = g["rate_modes"], int(g["ii_lo"]), int(g["ii_hi"])
+        rw, _ = ctx.RTA_driver_4ph_modes(rm, rt.PREC_FP64_DENSE, ii_lo=lo, ii_hi=hi)
+        errs = []
+        for key, gk in zip(keys, ("rate_pp", "rate_pm", "rate_mm")):
+            got, want = rw[key].ravel(order="F")[rm - 1], g[gk]
+            ok = want != 0
+            errs.append(float(np.max(np.abs(got[ok] - want[ok]) / np.abs(want[ok]), initial=0.0)))
+        checks["dense_fp64_vs_oracle_golden"] = {"modes": [int(x) for x in rm], "q_window": [lo, hi],
+                                                 "max_rel_err": max(errs)}
+    tot = Npp + Npm + Nmm
+    act = np.nonzero(tot > 0)[0] + 1
+    sample = np.unique(np.concatenate([act[::97], [act[np.argmin(tot[act - 1])], act[np.argmax(tot[act - 1])]]]))
+    rp, np_s = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP64, ii_lo=1, ii_hi=16)
+    rd, _ = ctx.RTA_driver_4ph_modes(sample, rt.PREC_FP64_DENSE, ii_lo=1, ii_hi=16)
+    a_, b_ = (np.concatenate([r[k].ravel(order="F")[sample - 1] for k in keys]) for r in (rd, rp))
+    ok = b_ != 0
+    checks["dense_fp64_vs_per_process_fp64"] = {
+        "sample": f"{len(sample)} modes (every 97th active mode + min/max count), q'=1..16, {np_s} processes",
+        "max_rel_err": float(np.max(np.abs(a_[ok] - b_[ok]) / np.abs(b_[ok])))}
+    log(f"checks: {checks}")
+
+    forms = []
+    for name in FORMS:
+        prec = prec_of(rt, name)
+        ctx.reset_stats()
+        t1 = time.perf_counter()
+        ro, no = ctx.RTA_driver_4ph_modes(all_modes, prec)
+        to = time.perf_counter() - t1
+        k_ms, _ = ctx.kernel_stats(rt.Context.KS_DENSE)
+        a = ro["rate_scatt_4ph"].ravel(order="F")
+        err = np.abs(a[nz] - b[nz]) / np.abs(b[nz])
+        kk = np.trace(kappa_rta(case, a)) / 3
+        forms.append({"precision": name, "kernel": kernel_name(ctx, m.nbands), "seconds": to, "kernel_ms": k_ms,
+                      "value": no / to, "unit": "processes/s", "rate_max_rel_err": float(err.max()),
+                      "rate_p99_rel_err": float(np.percentile(err, 99)),
+                      "rate_median_rel_err": float(np.median(err)),
+                      "kappa_rel_err": float(abs(kk - k64) / abs(k64))})
+        log(f"{name}: {to:.1f}s, {no / to:.3e} proc/s, rate err max {err.max():.2e} p99 "
+            f"{np.percentile(err, 99):.2e}, kappa err {forms[-1]['kappa_rel_err']:.2e}")
+    ctx.close()
+    print(json.dumps({"workload": "wurtzite-like 4-atom cell 12x12x8, 2nd-shell FC4, 4ph RTA at 300 K, all "
+                                  f"{nm} modes: MFMA forms vs dense FP64",
+                      "processes": nproc_total, "prefilter_seconds": t_pref, "dense_fp64_seconds": t64,
+                      "kappa_rta_fp64": float(k64), "checks": checks, "forms": forms}), flush=True)
+
+
 def run_iterative20(args):
-    """Diamond 20x20x20 with 1st-shell FC3/FC4, the reference's default mode:
-    4ph RTA rates (MFMA form), 3ph lists and Gamma on the device, tau0 from
-    both, the 3ph iterative solution to eps = 1e-5 on the device, at 100, 300
-    and 700 K.  Prints one JSON line with the times and kappa per T."""
+    """Diamond 20x20x20 (8000 q, 1536 irreducible modes) with 1st-shell
+    FC3/FC4: the iterative BTE with the full 3ph+4ph scattering operator
+    (four_phonon_iteration = .true.) at 100, 300 and 700 K.  Per T the
+    operator is assembled on the device, aggregated by partner mode (4ph
+    Gamma in the MFMA form, 3ph Gamma in FP64; the 4.3e11 4ph processes are
+    never stored), tau0 comes from the same processes' rates, and the
+    reference's in-place iteration runs to a relative kappa change below
+    1e-5 with the FP64 kappa residual.  At 300 K the operator is assembled a
+    second time with FP64 dense 4ph Gamma, and the iterations and kappa of
+    the MFMA form are compared with it.  Prints one JSON line."""
     from fourphonon_amd import crystal
     from fourphonon_amd import runtime as rt
 
@@ -457,45 +555,61 @@ def run_iterative20(args):
     log(f"case built in {time.time() - t0:.1f}s: nptk={m.nptk} Nlist={m.Nlist} Ntri4={case.ifc4.Ntri} "
         f"Ntri3={case.ifc3.Ntri}")
     ctx = rt.Context(case)
-    nm = ctx.nmodes
-    all_modes = np.arange(1, nm + 1, dtype=np.int32)
     prec = prec_of(rt, args.precision)
-    N4 = [np.zeros(nm, np.int64) for _ in range(3)]
-    L4 = [[np.zeros(1, np.int64)] * 3 for _ in range(3)]
-    G4 = [np.zeros(1)] * 3
-    t1 = time.perf_counter()
-    n3 = ctx.NP_driver()
-    t_np3 = time.perf_counter() - t1
-    N3 = [n3[0], n3[2]]
+
+    def solve(precision):
+        ctx.reset_stats()
+        t1 = time.perf_counter()
+        # assembled in 8 pieces of whole irreducible points, a progress line each
+        nm = ctx.nmodes
+        r3 = np.zeros((m.nbands, m.Nlist), order="F")
+        r4 = np.zeros((m.nbands, m.Nlist), order="F")
+        n3 = n4 = 0
+        ctx.bte_operator_begin()
+        cuts = np.linspace(0, m.Nlist, 9).astype(int)
+        for p in range(8):
+            modes = np.arange(cuts[p] * m.nbands + 1, cuts[p + 1] * m.nbands + 1, dtype=np.int32)
+            if len(modes) == 0:
+                continue
+            a3, a4 = ctx.bte_operator_add(modes, precision, r3=r3, r4=r4)
+            n3, n4 = n3 + a3, n4 + a4
+            log(f"  operator piece {p + 1}/8: modes {modes[0]}..{modes[-1]} of {nm}, {time.perf_counter() - t1:.1f}s")
+        ctx.bte_operator_end()
+        ta = time.perf_counter() - t1
+        rate = r3 + r4
+        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+        t1 = time.perf_counter()
+        res = ctx.bte_solve(tau, maxiter=1000)
+        ts = time.perf_counter() - t1
+        sweep_ms, steps = ctx.kernel_stats(rt.Context.KS_ITER)
+        k = res["kappa_iter"]
+        return {"processes_3ph": int(n3), "processes_4ph": int(n4), "assemble_seconds": ta,
+                "solve_seconds": ts, "niter": int(res["niter"]), "final_relchange": float(res["relchange"][-1]),
+                "seconds_per_sweep": sweep_ms * 1e-3 / max(int(res["niter"]), 1), "steps_per_sweep": steps // max(int(res["niter"]), 1),
+                "kappa_rta": float(np.trace(k[:, :, 0]) / 3), "kappa": float(np.trace(k[:, :, -1]) / 3),
+                "kappa_tensor": k[:, :, -1].tolist()}
+
     per_T = []
     for T in (100.0, 300.0, 700.0):
         ctx.set_temperature(T)
-        t1 = time.perf_counter()
-        r4, n4 = ctx.RTA_driver_4ph_modes(all_modes, prec)
-        t4 = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        L3, G3, R3 = ctx.Ind_driver(*N3)
-        t3 = time.perf_counter() - t1
-        rate = R3["rate_scatt"] + r4["rate_scatt_4ph"]
-        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
-        t1 = time.perf_counter()
-        ctx.bte_set_operator(N4, L4, G4, N3, L3, G3)
-        res = ctx.bte_solve(tau, maxiter=1000)
-        ts = time.perf_counter() - t1
-        k = res["kappa_iter"]
-        rec = {"T": T, "processes_4ph": int(n4), "processes_3ph": int(sum(int(x.sum()) for x in N3)),
-               "rta_4ph_seconds": t4, "processes_4ph_per_s": n4 / t4, "ind_3ph_seconds": t3,
-               "solve_seconds": ts, "niter": int(res["niter"]), "final_relchange": float(res["relchange"][-1]),
-               "kappa_rta": float(np.trace(k[:, :, 0]) / 3), "kappa": float(np.trace(k[:, :, -1]) / 3)}
+        rec = {"T": T, **solve(prec)}
+        log(f"T={T:.0f} K: operator {rec['assemble_seconds']:.1f}s ({rec['processes_4ph']} 4ph + "
+            f"{rec['processes_3ph']} 3ph processes), solve {rec['solve_seconds']:.2f}s in {rec['niter']} iterations "
+            f"({rec['seconds_per_sweep']:.3f} s/sweep), kappa {rec['kappa']:.4f} W/m/K (RTA {rec['kappa_rta']:.4f})")
+        if T == 300.0 and not args.no_fp64:
+            ref = solve(rt.PREC_FP64_DENSE)
+            rec["fp64_dense_reference"] = {k: ref[k] for k in ("assemble_seconds", "niter", "kappa", "kappa_rta")}
+            rec["kappa_rel_err_vs_fp64_dense"] = abs(rec["kappa"] - ref["kappa"]) / abs(ref["kappa"])
+            rec["niter_fp64_dense"] = ref["niter"]
+            log(f"  FP64 dense operator: {ref['assemble_seconds']:.1f}s, {ref['niter']} iterations, kappa "
+                f"{ref['kappa']:.4f}; MFMA-form kappa rel err {rec['kappa_rel_err_vs_fp64_dense']:.2e}")
         per_T.append(rec)
-        log(f"T={T:.0f} K: 4ph {t4:.1f}s ({n4} processes), 3ph Ind {t3:.2f}s, solve {ts:.2f}s "
-            f"in {rec['niter']} iterations, kappa {rec['kappa']:.4f} W/m/K (RTA {rec['kappa_rta']:.4f})")
     ctx.close()
-    total = sum(r["rta_4ph_seconds"] + r["ind_3ph_seconds"] + r["solve_seconds"] for r in per_T) + t_np3
-    print(json.dumps({"workload": "diamond 20x20x20, 1st-shell FC3/FC4, 4ph RTA + 3ph iterative kappa "
-                                  "(reference default mode), eps 1e-5",
-                      "unit": "W/(m K)", "precision_4ph": args.precision, "np_3ph_seconds": t_np3,
-                      "total_seconds": total, "temperatures": per_T}), flush=True)
+    total = sum(r["assemble_seconds"] + r["solve_seconds"] for r in per_T)
+    print(json.dumps({"workload": "diamond 20x20x20, 1st-shell FC3/FC4, iterative BTE with the 3ph+4ph operator "
+                                  "assembled on the device (aggregated by partner mode), eps 1e-5",
+                      "unit": "W/(m K)", "precision_4ph": args.precision, "total_seconds": total,
+                      "temperatures": per_T}), flush=True)
 
 
 if __name__ == "__main__":

@@ -134,10 +134,14 @@ struct fp4_ctx {
   int ntri3 = 0;
   DevBuf clusters3, phi3;
   // scratch
-  DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g;
+  DevBuf modes_sub, cnt, psum, offsets, segN, segP, lists[3], blk, blk_g, blk_w, seg_first, seg_g, seg_w, proc_g,
+      seg_hash;
   DenseState *dense = nullptr;
   BteState *bte = nullptr;
   long long list_cap = 1LL << 26;   // processes per batch
+  int last_nblk = 0;                // rate blocks of the last batch (their descriptors are in blk)
+  double *w_op = nullptr;           // aggregated operator being assembled (between begin and end)
+  long long w_cols = 0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ks_ms[KS_N] = {};
@@ -364,6 +368,7 @@ int rate_common(fp4_ctx *c, const Plan &pl, const Batch &b, bool store_gamma, st
   std::vector<BlockDesc> blocks;
   std::vector<int> seg_first, piece_seg;
   plan_blocks(pl, b, blocks, seg_first, piece_seg);
+  c->last_nblk = (int)blocks.size();
   if (blocks.empty()) return 0;
   int rc;
   const int nblk = (int)blocks.size(), npc = (int)piece_seg.size();
@@ -628,7 +633,8 @@ int fp4_ctx_destroy(fp4_ctx *c) {
   DevBuf *bufs[] = {&c->energy, &c->vel, &c->vbound, &c->eig, &c->ijk, &c->list, &c->clusters, &c->phi, &c->roots,
                     &c->clusters3, &c->phi3,
                     &c->modes_sub, &c->cnt, &c->psum, &c->offsets, &c->segN, &c->segP, &c->lists[0], &c->lists[1],
-                    &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g};
+                    &c->lists[2], &c->blk, &c->blk_g, &c->blk_w, &c->seg_first, &c->seg_g, &c->seg_w, &c->proc_g,
+                    &c->seg_hash};
   for (DevBuf *b : bufs) b->release();
   dense_destroy(c->dense);
   c->dense = nullptr;
@@ -1104,6 +1110,43 @@ int fp4_np_4ph_modes_noalloc(fp4_ctx *c, int32_t nmodes, const int32_t *modes, i
                             c->stream));
     HIPCHECK(hipMemcpyAsync(P[ch], d_segP + (size_t)ch * nmodes, sizeof(double) * nmodes, hipMemcpyDeviceToHost,
                             c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return FP4_OK;
+}
+
+int fp4_list_hash_4ph_modes(fp4_ctx *c, int32_t nmodes, const int32_t *modes, int64_t *N_pp, int64_t *N_pm,
+                            int64_t *N_mm, uint64_t *H_pp, uint64_t *H_pm, uint64_t *H_mm) {
+  int rc;
+  if ((rc = check_ready(c, false))) return rc;
+  if (!N_pp || !N_pm || !N_mm || !H_pp || !H_pm || !H_mm) return fail(FP4_ERR_ARG, "null output");
+  Plan pl;
+  if ((rc = subset_plan(c, nmodes, modes, pl))) return rc;
+  if ((rc = count_pass(c, pl, true))) return rc;
+  const int nseg = 3 * nmodes;
+  if ((rc = c->seg_hash.ensure(sizeof(unsigned long long) * nseg))) return rc;
+  HIPCHECK(hipMemsetAsync(c->seg_hash.p, 0, sizeof(unsigned long long) * nseg, c->stream));
+  PrefilterArgs a{};
+  a.m = mode_view(c);
+  a.modes = c->modes_sub.as<int>();
+  a.nsub = nmodes;
+  a.nii = pl.nii;
+  a.ii_lo0 = pl.ii_lo0;
+  a.item_lo = 0;
+  a.item_hi = pl.items;
+  a.offsets = c->offsets.as<long long>();
+  a.out_base = 0;
+  a.seg_hash = c->seg_hash.as<unsigned long long>();
+  if ((rc = timed(c, KS_EMIT, [&] {
+         return launch_prefilter(a, 3, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
+       })))
+    return rc;
+  int64_t *N[3] = {N_pp, N_pm, N_mm};
+  uint64_t *H[3] = {H_pp, H_pm, H_mm};
+  for (int ch = 0; ch < 3; ch++) {
+    for (int m = 0; m < nmodes; m++) N[ch][m] = pl.segN[ch * nmodes + m];
+    HIPCHECK(hipMemcpyAsync(H[ch], c->seg_hash.as<unsigned long long>() + (size_t)ch * nmodes,
+                            sizeof(unsigned long long) * nmodes, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
   return FP4_OK;
@@ -1886,6 +1929,144 @@ int fp4_bte_set_operator(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const 
   BTE_CALL(bte_set_orbits(c->bte, m, Nequi, ALLEquiList, TypeofSymmetry));
   BTE_CALL(bte_set_operator(c->bte, m, N3, L3, G3, N4, L4, G4));
   return FP4_OK;
+}
+
+// Adds the per-process Gamma of the last rate batch (its block descriptors
+// in c->blk, lists in c->lists, Gamma in c->proc_g) to the operator W.
+static int scatter_batch(fp4_ctx *c, double *W, long long cols, int order) {
+  WScatterArgs a{};
+  a.blocks = c->blk.as<BlockDesc>();
+  a.nblocks = c->last_nblk;
+  for (int t = 0; t < 3; t++) a.lst[t] = c->lists[t].as<int>();
+  a.gamma = c->proc_g.as<double>();
+  a.W = W;
+  a.wcols = cols;
+  a.order = order;
+  launch_w_scatter(a, c->stream);
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+int fp4_bte_operator_begin(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                           const int32_t *TypeofSymmetry) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (nlist != c->nlist) return fail(FP4_ERR_ARG, "Nlist=%d differs from the modes' %d", nlist, c->nlist);
+  if (!TypeofSymmetry) return fail(FP4_ERR_ARG, "null TypeofSymmetry");
+  if ((long long)c->nptk * c->nb >= (1LL << 31)) return fail(FP4_ERR_UNSUPPORTED, "nptk*Nbands must be < 2^31");
+  const BteModes m = bte_modes(c);
+  BTE_CALL(bte_set_orbits(c->bte, m, Nequi, ALLEquiList, TypeofSymmetry));
+  double *W = nullptr;
+  long long cols = 0;
+  BTE_CALL(bte_w_begin(c->bte, m, &W, &cols));
+  c->w_op = W;
+  c->w_cols = cols;
+  return FP4_OK;
+}
+
+int fp4_bte_operator_add(fp4_ctx *c, int32_t precision, int32_t with_3ph, int32_t with_4ph, int32_t nmodes,
+                         const int32_t *modes, int64_t *nproc3, int64_t *nproc4, double *rate_3ph,
+                         double *rate_4ph) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (!c->w_op) return fail(FP4_ERR_STATE, "fp4_bte_operator_begin has not been called");
+  if (with_3ph && (rc = check_ready3(c, true))) return rc;
+  if (with_4ph && (rc = check_ready(c, true))) return rc;
+  Plan sub;
+  if ((rc = subset_plan(c, nmodes, modes, sub))) return rc;
+  double *W = c->w_op;
+  const long long cols = c->w_cols;
+  const int nm = c->nb * c->nlist;
+  long long n3 = 0, n4 = 0;
+  if (with_3ph) {
+    // Ind_driver's lists and per-process Gamma, batch by batch, straight into W
+    Plan pl;
+    pl.modes = sub.modes;
+    pl.win_lo0 = 0;
+    pl.win_n = c->nptk;
+    if ((rc = count3_pass(c, pl, true))) return rc;
+    if ((rc = counts3_fit(pl))) return rc;
+    std::vector<double> g(2 * (size_t)nmodes, 0.0), w(2 * (size_t)nmodes, 0.0);
+    for (const Batch &b : plan_batches(pl, c->list_cap)) {
+      if ((rc = emit3_batch(c, pl, b))) return rc;
+      if ((rc = rate3_batch(c, pl, b, true, g, w))) return rc;
+      if ((rc = scatter_batch(c, W, cols, 3))) return rc;
+    }
+    n3 = pl.off[pl.items];
+    if (rate_3ph) {
+      // Ind_driver's per-mode rates (sums of the per-process Gamma in THz)
+      std::vector<double> rp(nm), rm(nm), wp(nm), wm(nm);
+      write3_rates(c, pl, g, w, rate_3ph, rp.data(), rm.data(), wp.data(), wm.data(), false);
+    }
+  }
+  if (with_4ph) {
+    std::vector<double> t(7 * (size_t)nm);
+    if (precision == FP4_PREC_FP64) {
+      // Ind_driver_4ph's per-process FP64 Gamma, batch by batch
+      Plan pl = sub;
+      if ((rc = count_pass(c, pl, true))) return rc;
+      std::vector<double> g(3 * (size_t)nmodes, 0.0), w(3 * (size_t)nmodes, 0.0);
+      for (const Batch &b : plan_batches(pl, c->list_cap)) {
+        if ((rc = emit_batch(c, pl, b, false))) return rc;
+        if ((rc = rate_batch(c, pl, b, false, true, g, w))) return rc;
+        if ((rc = scatter_batch(c, W, cols, 4))) return rc;
+      }
+      n4 = pl.off[pl.items];
+      if (rate_4ph)
+        write_rates(c, pl, g, w, rate_4ph, &t[0], &t[nm], &t[2 * nm], &t[3 * nm], &t[4 * nm], &t[5 * nm], false);
+    } else if (precision == FP4_PREC_FP32 || precision == FP4_PREC_FP64_DENSE || precision == FP4_PREC_BF16X3 ||
+               precision == FP4_PREC_FP16X3 || precision == FP4_PREC_FP16X2) {
+      // the dense form computes every process's Gamma and adds it to W
+      DenseRun dr;
+      dr.m = mode_view(c);
+      dr.roots = c->roots.as<double2>();
+      dr.T = c->T;
+      dr.onlyharmonic = c->onlyharmonic;
+      dr.natoms = c->natoms;
+      dr.precision = precision;
+      dr.stream = c->stream;
+      dr.tuning = &c->dense_tuning;
+      dr.wop = W;
+      dr.wcols = cols;
+      double ms = 0.0;
+      std::vector<double> g, w;
+      std::vector<long long> n;
+      rc = dense_rates(c->dense, dr, sub.modes, 0, c->nptk, g, w, n, &ms);
+      if (rc) return fail(rc, "dense path: %s", dense_error());
+      c->ks_ms[KS_DENSE] += ms;
+      c->ks_n[KS_DENSE] += 1;
+      n4 = std::accumulate(n.begin(), n.end(), 0LL);
+      if (rate_4ph)
+        write_rates(c, sub, g, w, rate_4ph, &t[0], &t[nm], &t[2 * nm], &t[3 * nm], &t[4 * nm], &t[5 * nm], true);
+    } else {
+      return fail(FP4_ERR_ARG, "unknown precision %d", precision);
+    }
+  }
+  if (nproc3) *nproc3 = n3;
+  if (nproc4) *nproc4 = n4;
+  return FP4_OK;
+}
+
+int fp4_bte_operator_end(fp4_ctx *c) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  if (!c->w_op) return fail(FP4_ERR_STATE, "fp4_bte_operator_begin has not been called");
+  BTE_CALL(bte_w_finish(c->bte, bte_modes(c)));
+  c->w_op = nullptr;
+  return FP4_OK;
+}
+
+int fp4_bte_assemble_operator(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                              const int32_t *TypeofSymmetry, int32_t precision, int32_t with_3ph, int32_t with_4ph,
+                              int64_t *nproc3, int64_t *nproc4, double *rate_3ph, double *rate_4ph) {
+  int rc;
+  if ((rc = fp4_bte_operator_begin(c, nlist, Nequi, ALLEquiList, TypeofSymmetry))) return rc;
+  std::vector<int> modes((size_t)c->nb * c->nlist);
+  std::iota(modes.begin(), modes.end(), 1);
+  if ((rc = fp4_bte_operator_add(c, precision, with_3ph, with_4ph, (int)modes.size(), modes.data(), nproc3, nproc4,
+                                 rate_3ph, rate_4ph)))
+    return rc;
+  return fp4_bte_operator_end(c);
 }
 
 int fp4_bte_iterate(fp4_ctx *c, const double *tau_zero, double *F_n) {

@@ -799,6 +799,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
                   const double vr = (double)V[u].re, vi = (double)V[u].im;
                   g = gpre * wpv * (vr * vr + vi * vi);
                 }
+                if (a.wop && g != 0.0)
+                  wop_add(a, ll0 * NB + ib * IB + il, ch, ii0 * NB + j, s.jj[t] * NB + k, s.ss[t] * NB + l, g);
 #pragma unroll
                 for (int xi = 0; xi < IB; xi++)
                   if (xi == il) {
@@ -1342,6 +1344,9 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
     return derr(FP4_ERR_DEVICE, "dense: memset failed");
 #endif
   a.prof = s->prof;
+  a.wop = r.wop;
+  a.wcols = r.wcols;
+  a.wscale = (double)kGammaUnits * 1.e9 / r.m.nptk / r.m.nptk;
   a.out_g = og;
   a.out_w = ow;
   a.out_n = on;

@@ -58,6 +58,11 @@ struct DenseRun {
   int precision;          // FP4_PREC_FP32 / FP4_PREC_FP64_DENSE
   hipStream_t stream;
   DenseTuning *tuning;
+  // aggregated scattering operator (iterative solution), or null: each
+  // process also adds its Gamma (in THz) with its iteration_4ph sign to
+  // wop[irreducible mode][partner's q*Nb + b], wcols = nptk*Nb columns
+  double *wop = nullptr;
+  long long wcols = 0;
 };
 
 // Rates of the given modes over the q' window [ii_lo0, ii_lo0+nii): g, w and

@@ -1072,6 +1072,9 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
                     const double vr = (double)V[u].re, vi = (double)V[u].im;
                     g = gpre * wpv * (vr * vr + vi * vi);
                   }
+                  if (a.wop && g != 0.0)
+                    wop_add(a, ll0 * NB + ib * IB + il, ch, ii0 * NB + j, s.vj[t0 + t] * NB + k,
+                            s.vs[t0 + t] * NB + l, g);
                   const bool second = (il != il_lo);
                   acc_g[0] += second ? 0.0 : g;
                   acc_w[0] += second ? 0.0 : wpv;

@@ -61,7 +61,26 @@ struct DenseArgs {
   const int *gslot;      // phase-table slot of each coarse group
   int lmask;             // L - 1 when L is a power of two, else -1
   unsigned long long *prof;  // profiling build: [kDensePhases] cycle sums
+  // aggregated scattering operator (iterative solution): when wop is set,
+  // every process also adds its Gamma * wscale, with the sign of its term in
+  // iteration_4ph, to wop[row][col] (row = irreducible mode, col = the
+  // partner's full-mesh mode q*Nb + b, wcols columns)
+  double *wop;
+  long long wcols;
+  double wscale;
 };
+
+// Adds one process's Gamma to the aggregated operator with the signs of the
+// F_n terms of iteration_4ph (Src/iterations.f90:177-203): ++ -F(j,q2)
+// -F(k,q3) +F(l,q4); +- -F(j,q2) +F(k,q3) +F(l,q4); -- all +.  c2, c3, c4
+// are the partners' columns q*Nb + b.
+__device__ __forceinline__ void wop_add(const DenseArgs &a, int row, int ch, int c2, int c3, int c4, double g) {
+  double *r = a.wop + (size_t)row * (size_t)a.wcols;
+  const double gs = g * a.wscale;
+  unsafeAtomicAdd(r + c2, ch == 2 ? gs : -gs);
+  unsafeAtomicAdd(r + c3, ch == 0 ? -gs : gs);
+  unsafeAtomicAdd(r + c4, gs);
+}
 
 template <class T>
 __device__ __forceinline__ C2<T> cm(C2<T> a, C2<T> b) {

@@ -39,12 +39,13 @@ __global__ void __launch_bounds__(64) k_eigen_dm(EigenDMArgs a) {
   const int lane = threadIdx.x;
   const int q0 = blockIdx.x;
   const int i1 = q0 % a.n1, i2 = (q0 / a.n1) % a.n2, i3 = q0 / (a.n1 * a.n2);
-  // kspace of eigenDM: (i-1.0)/ngrid is a float32 quotient in the reference
-  const double f1 = (double)((float)i1 / (float)a.n1), f2 = (double)((float)i2 / (float)a.n2),
-               f3 = (double)((float)i3 / (float)a.n3);
+  // kspace of eigenDM, rlattvec(:,1)*(ii-1.0)/ngrid(1) + ...: evaluated left
+  // to right, so each term is (b * (i-1)) / N in double (Src/phonon_routines.f90:29-31)
   double k[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) k[c] = (a.b[c] * f1 + a.b[3 + c] * f2 + a.b[6 + c] * f3) * a.bohr2nm;
+  for (int c = 0; c < 3; c++)
+    k[c] = (((a.b[c] * (double)i1) / (double)a.n1 + (a.b[3 + c] * (double)i2) / (double)a.n2) +
+            (a.b[6 + c] * (double)i3) / (double)a.n3) * a.bohr2nm;
 
   // 1. D and dD
   for (int e = lane; e < NN; e += 64) {

@@ -19,6 +19,15 @@
 // bytes, so a gather touches one cache line); the lists are int32 (the
 // encoded (q-1)*Nb+branch fits for nptk*Nb < 2^31) and Gamma double, 20 B
 // per 4ph process.
+//
+// Aggregated operator.  The sum over a mode's processes is linear in F_n and
+// each term reads F_n at one partner mode, so it regroups by partner:
+// DeltaF(i, q_kk) = sum_c W[(ll,i)][c] F_n(b, ID_equi(S_kk, q), :) over the
+// full-mesh modes c = q*Nb + b, with W the signed Gamma summed over the
+// processes that have partner c (3ph and 4ph alike).  W has Nb*Nlist rows of
+// nptk*Nb doubles whatever the number of processes (590 MB at 20^3, Nb = 6,
+// against about 10 TB of lists), so it is assembled once per temperature
+// on the device and the in-place sweep keeps the reference's order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -93,6 +102,9 @@ struct BteState {
   DBuf e2[5], e3[5], e4[3], g[5];
   std::vector<long long> h_n[5], h_off[5];  // per mode
   DBuf F, stage, part, ticket, kappa, mode, tau, hist, rel, flag;
+  DBuf W;                      // aggregated operator [Nb*Nlist][nptk*Nb]
+  bool use_w = false;
+  long long wcols = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   long long last_launches = 0;
@@ -147,9 +159,24 @@ __global__ void __launch_bounds__(kStepThreads) k_iter_step(IterStep s) {
   double acc[3] = {0.0, 0.0, 0.0};
   const long long stride = (long long)gridDim.x * kStepThreads;
   const long long me = (long long)blockIdx.x * kStepThreads + tid;
-  for (int type = 0; type < 5; type++) {
-    const long long n = s.n[type], off = s.off[type];
-    for (long long p = me; p < n; p += stride) add_term(s, type, off + p, acc);
+  if (s.W) {
+    // aggregated operator: sum over partner modes c = q*Nb + b of
+    // W[mode][c] * F_n(b, ID_equi(sym, q), :)
+    const double *row = s.W + (size_t)s.wrow * (size_t)s.wcols;
+    for (long long c = me; c < s.wcols; c += stride) {
+      const double w = row[c];
+      if (w == 0.0) continue;
+      const int q = (int)(c / s.nb), b = (int)(c - (long long)q * s.nb);
+      const int qr = s.ideq[(size_t)s.sym + (size_t)s.nsymm * q] - 1;
+      const double *Fp = &FQ(s.F, s.nb, qr, b, 0);
+#pragma unroll
+      for (int a = 0; a < 3; a++) acc[a] = acc[a] + w * Fp[a];
+    }
+  } else {
+    for (int type = 0; type < 5; type++) {
+      const long long n = s.n[type], off = s.off[type];
+      for (long long p = me; p < n; p += stride) add_term(s, type, off + p, acc);
+    }
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) acc[a] = wave_sum(acc[a]);
@@ -399,7 +426,7 @@ BteState *bte_create() {
 void bte_destroy(BteState *s) {
   if (!s) return;
   DBuf *bufs[] = {&s->ideq, &s->symm, &s->crot, &s->nequi, &s->alleq, &s->F,    &s->stage, &s->part,
-                  &s->ticket, &s->kappa, &s->mode, &s->tau, &s->hist, &s->rel, &s->flag};
+                  &s->ticket, &s->kappa, &s->mode, &s->tau, &s->hist, &s->rel, &s->flag, &s->W};
   for (DBuf *b : bufs) b->release();
   for (int t = 0; t < 5; t++) {
     s->e2[t].release();
@@ -419,6 +446,7 @@ static int check_dims(BteState *s, const BteModes &m) {
   if (s->nptk && (s->nptk != m.nptk || s->nb != m.nb || s->nlist != m.nlist)) {
     // mode data changed shape: everything derived from it is stale
     s->have_symm = s->have_orbits = s->have_types = s->have_op = false;
+    s->use_w = false;
   }
   s->nptk = m.nptk;
   s->nb = m.nb;
@@ -520,6 +548,7 @@ int bte_set_operator(BteState *s, const BteModes &m, const int32_t *const N3[2],
   if ((long long)m.nptk * m.nb >= (1LL << 31)) return bfail(FP4_ERR_UNSUPPORTED, "nptk*Nbands must be < 2^31");
   const int nm = m.nb * m.nlist;
   s->have_op = false;
+  s->use_w = false;
   for (int t = 0; t < 5; t++) {
     s->h_n[t].assign(nm, 0);
     s->h_off[t].assign(nm + 1, 0);
@@ -596,6 +625,28 @@ int bte_get_F(BteState *s, const BteModes &m, double *F_n) {
   return 0;
 }
 
+int bte_w_begin(BteState *s, const BteModes &m, double **W, long long *cols) {
+  check_dims(s, m);
+  if (!s->have_orbits) return bfail(FP4_ERR_STATE, "orbit data not set");
+  s->use_w = false;
+  const long long rows = (long long)m.nb * m.nlist, nc = (long long)m.nptk * m.nb;
+  int rc;
+  if ((rc = s->W.ensure(sizeof(double) * (size_t)rows * (size_t)nc))) return rc;
+  BCHECK(hipMemsetAsync(s->W.p, 0, sizeof(double) * (size_t)rows * (size_t)nc, m.stream));
+  s->wcols = nc;
+  *W = s->W.as<double>();
+  *cols = nc;
+  return 0;
+}
+
+int bte_w_finish(BteState *s, const BteModes &m) {
+  check_dims(s, m);
+  if (!s->W.p || s->wcols != (long long)m.nptk * m.nb) return bfail(FP4_ERR_STATE, "operator not assembled");
+  BCHECK(hipStreamSynchronize(m.stream));
+  s->use_w = true;
+  return 0;
+}
+
 int bte_iteration0(BteState *s, const BteModes &m, const double *tau_zero) {
   check_dims(s, m);
   if (!s->have_orbits) return bfail(FP4_ERR_STATE, "orbit data not set");
@@ -610,7 +661,7 @@ int bte_iteration0(BteState *s, const BteModes &m, const double *tau_zero) {
 
 int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero) {
   check_dims(s, m);
-  if (!s->have_op || !s->have_orbits || !s->have_types || !s->have_symm)
+  if (!(s->have_op || s->use_w) || !s->have_orbits || !s->have_types || !s->have_symm)
     return bfail(FP4_ERR_STATE, "operator, orbits or symmetry data not set");
   if (!tau_zero) return bfail(FP4_ERR_ARG, "null tau_zero");
   if (!s->F.p) return bfail(FP4_ERR_STATE, "no F_n on the device");
@@ -637,16 +688,23 @@ int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero) {
   st.nb = m.nb;
   st.nptk = m.nptk;
   st.nsymm = s->nsymm;
+  st.W = s->use_w ? s->W.as<double>() : nullptr;
+  st.wcols = s->wcols;
   long long launches = 0;
   BCHECK(hipEventRecord(s->ev0, m.stream));
   for (int ll = 0; ll < m.nlist; ll++)
     for (int i = 0; i < m.nb; i++) {
       const int mm = ll * m.nb + i;
       long long tot = 0;
-      for (int t = 0; t < 5; t++) {
-        st.off[t] = s->h_off[t][mm];
-        st.n[t] = s->h_n[t][mm];
-        tot += st.n[t];
+      if (s->use_w) {
+        st.wrow = mm;
+        tot = s->wcols;
+      } else {
+        for (int t = 0; t < 5; t++) {
+          st.off[t] = s->h_off[t][mm];
+          st.n[t] = s->h_n[t][mm];
+          tot += st.n[t];
+        }
       }
       st.i = i;
       st.tau = tau_zero[i + (size_t)m.nb * ll];

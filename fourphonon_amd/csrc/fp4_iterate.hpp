@@ -24,6 +24,11 @@ struct IterStep {
   int nb, nptk, nsymm;
   int i, qk, sym;                    // 0-based branch, star member, symmetry operation
   double tau;
+  // aggregated operator instead of the lists when W is set: this mode's row
+  // of W[Nb*Nlist][nptk*Nb] (partner columns q*Nb + b)
+  const double *W;
+  long long wcols;
+  int wrow;
 };
 
 struct BteModes {
@@ -53,6 +58,12 @@ int bte_set_operator(BteState *s, const BteModes &m, const int32_t *const N3[2],
                      const double *const G3[2], const int64_t *const N4[3], const int64_t *const L4[3][3],
                      const double *const G4[3]);
 
+// Aggregated operator: bte_w_begin allocates W[Nb*Nlist][nptk*Nb] on the
+// device, zeroed (*W, *cols receive it; the caller adds the processes into
+// it), bte_w_finish makes the sweeps use it instead of the lists (until the
+// next bte_set_operator).  Needs the orbits (bte_set_orbits) first.
+int bte_w_begin(BteState *s, const BteModes &m, double **W, long long *cols);
+int bte_w_finish(BteState *s, const BteModes &m);
 int bte_iteration0(BteState *s, const BteModes &m, const double *tau_zero);  // tau_zero(Nb,Nlist) host
 int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero);       // one iteration_4ph
 int bte_symmetrize_all(BteState *s, const BteModes &m);                      // F_n(:,q,:) for every q

@@ -23,6 +23,9 @@ struct PrefilterArgs {
   long long out_base;        // offsets[item_lo]
   int *out32[3];
   long long *out64[3];
+  // hash pass: per-(channel, mode) order-sensitive list hash, accumulated
+  // with 64-bit atomic adds (zeroed by the caller)
+  unsigned long long *seg_hash;
 };
 
 int launch_prefilter(const PrefilterArgs &a, int mode, hipStream_t st);
@@ -77,6 +80,21 @@ struct RateArgs {
 };
 
 int launch_rate64(const RateArgs &a, hipStream_t st);
+
+// Aggregated scattering operator of the iterative solution: the per-process
+// Gamma of one batch (rate kernel with store_gamma) added to W[mode][col]
+// with the signs of the F_n terms of iteration_4ph, col = the partner's
+// (q-1)*Nb + branch - 1.  Block per BlockDesc, thread per process.
+struct WScatterArgs {
+  const BlockDesc *blocks;
+  int nblocks;
+  const int *lst[3];         // batch lists (reference encoding), int32
+  const double *gamma;       // per-process Gamma of the batch, THz
+  double *W;
+  long long wcols;
+  int order;                 // 3: lists +/- of three phonons; 4: ++/+-/--
+};
+int launch_w_scatter(const WScatterArgs &a, hipStream_t st);
 
 // Per-segment sums of block partials, in block order.
 void launch_seg_reduce(const double *blk_gamma, const double *blk_wp4, const int *seg_first, int nseg,

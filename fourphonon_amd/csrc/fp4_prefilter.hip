@@ -22,10 +22,20 @@
 // Emit pass: per-lane counts, a wavefront exclusive scan in lane (= jj)
 // order, then each lane writes its survivors at item_offset + lane_offset;
 // the concatenation is the reference's (ii, jj, j, k, l) order.
+// Hash pass: the emit pass's positions without the writes.  The hash of a
+// (channel, mode) list of n entries is H = sum_p key_p M^(n-1-p) mod 2^64,
+// key = (i2 << 42) ^ (i3 << 21) ^ i4.  A lane's run of c survivors starting
+// at list position p0 contributes M^(n-p0-c) times its Horner sum
+// (h = h*M + key over the run), so one candidate scan per lane suffices;
+// the contributions are added with 64-bit atomics, whose sum mod 2^64 does
+// not depend on their order.
 #include "fp4_device.hpp"
 #include "fp4_kernels.hpp"
 
 namespace fp4 {
+
+// multiplier of the order-sensitive list hash (the 64-bit golden ratio)
+constexpr unsigned long long kListHashM = 0x9E3779B97F4A7C15ULL;
 
 struct ItemDecode {
   int ch, mi, ii0;
@@ -228,6 +238,51 @@ k_prefilter(PrefilterArgs a) {
         a.cnt[item] = cnt;
         a.psum[item] = psum;
       }
+    } else if (MODE == 3) {
+      if (!active) continue;
+      const long s = item / a.nii;
+      const long long seg0 = a.offsets[s * a.nii], nseg = a.offsets[(s + 1) * a.nii] - seg0;
+      long long base = a.offsets[item] - seg0;
+      unsigned long long acc = 0;
+      const int niter = (m.nptk - jj_first + 63) / 64;
+      for (int it = 0; it < niter; it++) {
+        const int jj0 = jj_first + it * 64 + lane;
+        int c = 0;
+        unsigned long long h = 0;
+        bool ok = jj0 < m.nptk;
+        int ss0 = 0;
+        if (ok) {
+          int qdp[3];
+#pragma unroll
+          for (int t = 0; t < 3; t++) qdp[t] = m.ijk[3 * jj0 + t];
+          ss0 = partner(m, ch, q, qp, qdp);
+          if (ch != 0 && ss0 < jj0) ok = false;
+        }
+        if (ok)
+          scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int j, int k, int l, double) {
+            const unsigned long long v2 = (unsigned long long)ii0 * NB + j + 1;
+            const unsigned long long v3 = (unsigned long long)jj0 * NB + k + 1;
+            const unsigned long long v4 = (unsigned long long)ss0 * NB + l + 1;
+            h = h * kListHashM + ((v2 << 42) ^ (v3 << 21) ^ v4);
+            c++;
+          });
+        const int off = wave_excl_scan(c, lane);
+        const int tot = wave_sum_i(c);
+        unsigned long long contrib = 0;
+        if (c > 0) {
+          // M^(n - p0 - c) by square-and-multiply
+          unsigned long long e = (unsigned long long)(nseg - (base + off) - c), x = kListHashM, pw = 1;
+          while (e) {
+            if (e & 1) pw *= x;
+            x *= x;
+            e >>= 1;
+          }
+          contrib = h * pw;
+        }
+        acc += wave_sum_u64(contrib);
+        base += tot;
+      }
+      if (lane == 0 && acc) atomicAdd(a.seg_hash + s, acc);
     } else {
       if (!active) continue;
       // pass 1: survivors per lane
@@ -300,8 +355,10 @@ static void launch_nb(const PrefilterArgs &a, int mode, hipStream_t st) {
     hipLaunchKernelGGL((k_prefilter<NB, 0>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   else if (mode == 1)
     hipLaunchKernelGGL((k_prefilter<NB, 1>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-  else
+  else if (mode == 2)
     hipLaunchKernelGGL((k_prefilter<NB, 2>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_prefilter<NB, 3>), dim3((unsigned)blocks), dim3(256), 0, st, a);
 }
 
 int launch_prefilter(const PrefilterArgs &a, int mode, hipStream_t st) {

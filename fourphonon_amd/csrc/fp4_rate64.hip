@@ -200,4 +200,43 @@ int launch_vp64(const VpArgs &a, hipStream_t st) {
   return 0;
 }
 
+// Aggregated operator: iteration_4ph's sum over a mode's processes,
+// sum_p Gamma_p * (+-F(partner 2) +-F(partner 3) [+F(partner 4)]), regrouped
+// by partner: W[mode][partner] collects the signed Gamma of every process
+// that has that partner.  Signs: 3ph + -F(j)+F(k), 3ph - (F(j)+F(k))/2
+// (Src/iterations.f90:152-166); 4ph ++ -F(j)-F(k)+F(l), +- -F(j)+F(k)+F(l),
+// -- F(j)+F(k)+F(l) (:177-203).  The additions are atomic, so W agrees
+// between runs to rounding, not bit for bit.
+__global__ void __launch_bounds__(256) k_w_scatter(WScatterArgs a) {
+  const BlockDesc d = a.blocks[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t >= d.count) return;
+  const long long p = d.start + t;
+  const double g = a.gamma[p];
+  if (g == 0.0) return;
+  double *r = a.W + (size_t)(d.mm - 1) * (size_t)a.wcols;
+  const int c2 = a.lst[0][p] - 1, c3 = a.lst[1][p] - 1;
+  if (a.order == 3) {
+    if (d.ch == 0) {
+      unsafeAtomicAdd(r + c2, -g);
+      unsafeAtomicAdd(r + c3, g);
+    } else {
+      const double h = g * 5.e-1;
+      unsafeAtomicAdd(r + c2, h);
+      unsafeAtomicAdd(r + c3, h);
+    }
+  } else {
+    const int c4 = a.lst[2][p] - 1;
+    unsafeAtomicAdd(r + c2, d.ch == 2 ? g : -g);
+    unsafeAtomicAdd(r + c3, d.ch == 0 ? -g : g);
+    unsafeAtomicAdd(r + c4, g);
+  }
+}
+
+int launch_w_scatter(const WScatterArgs &a, hipStream_t st) {
+  if (a.nblocks <= 0) return 0;
+  hipLaunchKernelGGL(k_w_scatter, dim3((unsigned)a.nblocks), dim3(256), 0, st, a);
+  return 0;
+}
+
 }  // namespace fp4

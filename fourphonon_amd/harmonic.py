@@ -13,8 +13,14 @@
   (read3fc / read4fc, Src/input.f90:102-182): Angstrom vectors rounded to the
   nearest lattice vector and converted to nm.
 
-The polar (non-analytic) correction is not built: a non-zero Born charge
-raises ValueError.  Eigenvectors of degenerate modes are one orthonormal
+* phonopy ``FORCE_CONSTANTS_2ND`` (read2fc, Src/input.f90:18-79) to the same
+  quantities, following phonon_phonopy (Src/phonon_routines.f90:84-307):
+  nearest supercell images with the 1e-5 nm tie rule, and the non-analytic
+  (dielectric) correction from the CONTROL Born charges and dielectric
+  tensor when ``nonanalytic`` is set.
+
+The espresso path has no polar (non-analytic) term: a non-zero Born charge
+raises ValueError there.  Eigenvectors of degenerate modes are one orthonormal
 basis of their subspace, as from any Hermitian solver; gauge-invariant
 outputs (omega, v, |V|^2 summed over a degenerate set, rates, kappa) are
 what they determine.
@@ -40,6 +46,9 @@ ISOTOPES = {
     "Si": [(27.976928, 92.23), (28.976496, 4.67), (29.973772, 3.1)],
     "B": [(10.012937, 19.9), (11.009305, 80.1)],
     "As": [(74.921596, 100.0)],
+    "Li": [(6.015123, 7.42), (7.016005, 92.58)],
+    "Co": [(58.933198, 100.0)],
+    "O": [(15.994915, 99.76), (16.999131, 0.038), (17.999159, 0.2)],
 }
 
 
@@ -100,12 +109,13 @@ def read_espresso_ifc2(path):
 
 
 def grid_kpoints(rlattvec, ngrid):
-    """kspace of eigenDM (Src/phonon_routines.f90:26-33), in grid-index order;
-    (i-1.0)/ngrid(i) is a default-real (float32) quotient there."""
-    ijk = crystal.grid_ijk(ngrid)                              # (3, nptk), 0-based
-    frac = np.stack([(ijk[a].astype(np.float32) / np.float32(ngrid[a])).astype(np.float64) for a in range(3)])
-    b = np.asarray(rlattvec)
-    return (b[:, 0][:, None] * frac[0] + b[:, 1][:, None] * frac[1] + b[:, 2][:, None] * frac[2]).T
+    """kspace of eigenDM (Src/phonon_routines.f90:26-33), in grid-index order:
+    rlattvec(:,1)*(ii-1.0)/ngrid(1) + ... evaluates left to right, so each
+    term is (b * (i-1)) / N in double precision."""
+    ijk = crystal.grid_ijk(ngrid).astype(np.float64)          # (3, nptk), 0-based
+    b = np.asarray(rlattvec, np.float64)
+    t = [(b[:, a][:, None] * ijk[a]) / float(ngrid[a]) for a in range(3)]
+    return ((t[0] + t[1]) + t[2]).T
 
 
 def _ws_images(cell_r, scell, rr):
@@ -190,6 +200,214 @@ def modes_from_espresso(st: crystal.Structure, fc, scell, ngrid, born=None):
                          Nequi=Nequi, ALLEquiList=ALLEq,
                          TypeofSymmetry=TypeS, nsymm_rot=ns, crotations=Cr, qrotations=Qr, symmetrizers=symm,
                          ID_equi=ID)
+
+
+# ---------------------------------------------------------------------------
+# phonopy FORCE_CONSTANTS_2ND, with the non-analytic correction
+# ---------------------------------------------------------------------------
+
+# default-real (float32) literals of read2fc and phonon_phonopy
+# (Src/input.f90:13, Src/phonon_routines.f90:91)
+PHONOPY_UNITFACTOR = float(np.float32(9648.53336213))   # eV/(A^2 amu) -> (rad/ps)^2
+NAC_PREFACTOR = float(np.float32(1745.91429109))        # THz^2 amu nm^3
+
+
+def split_index(index, scell):
+    """Supercell atom number (1-based) -> (ix, iy, iz, iatom), 1-based, x
+    fastest (split_index, Src/input.f90:83-99)."""
+    t, ix = divmod(index - 1, scell[0])
+    t, iy = divmod(t, scell[1])
+    iatom, iz = divmod(t, scell[2])
+    return ix + 1, iy + 1, iz + 1, iatom + 1
+
+
+def read_phonopy_fc2(path, natoms, scell, masses, types):
+    """phonopy's FORCE_CONSTANTS_2ND (read2fc, Src/input.f90:18-79) ->
+    fc(natoms,3,s1,s2,s3,natoms,3) in (rad/ps)^2: the blocks whose first atom
+    lies in the home cell, divided by sqrt(m_i m_j) and scaled to (rad/ps)^2.
+    fc[a, i, cell, b, j] couples atom a (home cell, direction i) with atom b
+    in supercell cell `cell` (direction j)."""
+    s1, s2, s3 = (int(x) for x in scell)
+    with open(path) as f:
+        lines = f.read().split("\n")
+    pos = 0
+    while not lines[pos].strip():
+        pos += 1
+    ntot = int(lines[pos].split()[0])
+    pos += 1
+    if ntot != s1 * s2 * s3 * natoms:
+        raise ValueError(f"{path}: {ntot} atoms, scell and natoms give {s1 * s2 * s3 * natoms}")
+    fc = np.zeros((natoms, 3, s1, s2, s3, natoms, 3), order="F")
+    for _ in range(ntot * ntot):
+        while not lines[pos].strip():
+            pos += 1
+        a1, a2 = (int(x) for x in lines[pos].split()[:2])
+        ix1, iy1, iz1, i1 = split_index(a1, (s1, s2, s3))
+        ix2, iy2, iz2, i2 = split_index(a2, (s1, s2, s3))
+        if ix1 == 1 and iy1 == 1 and iz1 == 1:
+            for ip in range(3):
+                fc[i1 - 1, ip, ix2 - 1, iy2 - 1, iz2 - 1, i2 - 1, :] = [float(x) for x in lines[pos + 1 + ip].split()[:3]]
+        pos += 4
+    m = np.asarray(masses, np.float64)[np.asarray(types) - 1]
+    mm = np.sqrt(m[:, None] * m[None, :])
+    mm[np.diag_indices(natoms)] = m
+    fc = fc / mm[:, None, None, None, None, :, None]
+    return PHONOPY_UNITFACTOR * fc
+
+
+def _phonopy_images(lattvec, cart, scell):
+    """Per (atom 1, atom 2, supercell cell): the supercell images of
+    r = r1 - r2 + R_cell nearest to the origin, with the reference's 1e-5 nm
+    tie rule applied in its loop order (Src/phonon_routines.f90:216-240)
+    -> dict (a1, a2, ix, iy, iz) -> (neq, 3) array of rl + R_cell."""
+    nat = cart.shape[1]
+    out = {}
+    shifts = [(x, y, z) for x in range(-2, 3) for y in range(-2, 3) for z in range(-2, 3)]
+    for a1 in range(nat):
+        for a2 in range(nat):
+            for ix in range(scell[0]):
+                for iy in range(scell[1]):
+                    for iz in range(scell[2]):
+                        rcell = lattvec @ np.array([ix, iy, iz], np.float64)
+                        r = cart[:, a1] - cart[:, a2] + rcell
+                        dmin = np.finfo(np.float64).max
+                        rr = []
+                        for x, y, z in shifts:
+                            rl = (x * scell[0] * lattvec[:, 0] + y * scell[1] * lattvec[:, 1]
+                                  + z * scell[2] * lattvec[:, 2])
+                            rn = float(np.sqrt(np.sum((rl + r) ** 2)))
+                            if abs(rn - dmin) > 1e-5:
+                                if rn < dmin:
+                                    dmin = rn
+                                    rr = [rl + rcell]
+                            else:
+                                rr.append(rl + rcell)
+                        out[(a1, a2, ix, iy, iz)] = np.array(rr)
+    return out
+
+
+def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False, epsilon=None, born=None):
+    """omega (nk,Nb) rad/ps, velocity (nk,Nb,3) km/s and eigenvect (nk,Nb,Nb)
+    [mode, 3*(atom-1)+alpha] for the kpoints (nk,3) in nm^-1 from the
+    reduced constants of read_phonopy_fc2 (phonon_phonopy,
+    Src/phonon_routines.f90:84-307): D(q) = sum over cells and nearest
+    supercell images of e^{-i q.R}/neq fc, dD/dq alike, and, when
+    nonanalytic, the dielectric correction (Born charges born(3,3,natoms),
+    dielectric tensor epsilon(3,3)) added to every cell's constants and to
+    dD/dq, with q taken at its shortest image (no correction at Gamma)."""
+    lattvec = np.asarray(st.lattvec, np.float64)
+    rlatt = st.rlattvec
+    nat = st.natoms
+    nb = 3 * nat
+    s1, s2, s3 = (int(x) for x in scell)
+    ncell = s1 * s2 * s3
+    k = np.asarray(kpoints, np.float64)
+    nk = len(k)
+    m = np.asarray(st.masses, np.float64)[np.asarray(st.types) - 1]
+    mm = np.sqrt(m[:, None] * m[None, :])
+    mm[np.diag_indices(nat)] = m
+    cart = lattvec @ np.asarray(st.positions, np.float64).T         # (3, nat)
+    eps = np.eye(3) if epsilon is None else np.asarray(epsilon, np.float64)
+    bz = np.zeros((3, 3, nat)) if born is None else np.asarray(born, np.float64)
+    V = st.volume
+    # the 1st-BZ image of each q point (Src/phonon_routines.f90:143-159)
+    shortest = k.copy()
+    best = np.sqrt(np.sum(k * k, axis=1))
+    for x in range(-2, 3):
+        for y in range(-2, 3):
+            for z in range(-2, 3):
+                r = k + x * rlatt[:, 0] + y * rlatt[:, 1] + z * rlatt[:, 2]
+                d = np.sqrt(np.sum(r * r, axis=1))
+                better = d < best
+                best = np.where(better, d, best)
+                shortest[better] = r[better]
+    # dielectric correction per q: dyn_nac and ddyn_nac (:168-197)
+    dyn_nac = np.zeros((nk, nb, nb))
+    ddyn_nac = np.zeros((nk, nb, nb, 3))
+    nac_on = np.zeros(nk, bool)
+    if nonanalytic:
+        np_err = np.seterr(divide="ignore", invalid="ignore")   # Gamma rows are discarded below
+        nac_on = ~np.all(shortest == 0.0, axis=1)
+        sq = shortest
+        tmp3 = np.einsum("ka,ab,kb->k", sq, eps, sq)
+        t = np.einsum("ka,aib->kib", sq, bz)                      # t[k, i, atom] = q . born(:, i, atom)
+        es = sq @ eps.T                                             # (eps q)(ip)
+        for a1 in range(nat):
+            for a2 in range(nat):
+                for i in range(3):
+                    for j in range(3):
+                        t1, t2 = t[:, i, a1], t[:, j, a2]
+                        r_, c_ = 3 * a1 + i, 3 * a2 + j
+                        dyn_nac[:, r_, c_] = t1 * t2 / mm[a1, a2]
+                        for ip in range(3):
+                            ddyn_nac[:, r_, c_, ip] = (t1 * bz[ip, j, a2] + t2 * bz[ip, i, a1]
+                                                       - 2. * t1 * t2 * es[:, ip] / tmp3)
+                        ddyn_nac[:, r_, c_, :] /= mm[a1, a2]
+        safe = np.where(nac_on, tmp3, 1.0)
+        dyn_nac = np.where(nac_on[:, None, None], NAC_PREFACTOR * dyn_nac / safe[:, None, None] / V, 0.0)
+        ddyn_nac = np.where(nac_on[:, None, None, None], NAC_PREFACTOR * ddyn_nac / safe[:, None, None, None] / V,
+                            0.0)
+        np.seterr(**np_err)
+    fc_diel = dyn_nac / ncell                                       # the same for every cell (:198-208)
+    images = _phonopy_images(lattvec, cart, (s1, s2, s3))
+    dyn = np.zeros((nk, nb, nb), np.complex128)
+    ddyn = np.zeros((nk, nb, nb, 3), np.complex128)
+    kq_on = nonanalytic & ~np.all(k == 0.0, axis=1)
+    for a1 in range(nat):
+        for a2 in range(nat):
+            star_all = np.zeros(nk, np.complex128)
+            zr_all = np.zeros((nk, 3), np.complex128)
+            blk = np.zeros((nk, 3, 3), np.complex128)
+            dblk = np.zeros((nk, 3, 3, 3), np.complex128)
+            for ix in range(s1):
+                for iy in range(s2):
+                    for iz in range(s3):
+                        rr = images[(a1, a2, ix, iy, iz)]
+                        neq = len(rr)
+                        qr = k @ rr.T                                   # (nk, neq)
+                        z = (np.cos(-qr) + 1j * np.sin(-qr)) / neq       # phexp(-qr)/neq
+                        star = z.sum(axis=1)
+                        zr = z @ rr                                     # sum_p z_p rr_p
+                        # fc_total(iatom2, j, cell, iatom1, i) for (i, j)
+                        f = fc[a2, :, ix, iy, iz, a1, :].T              # [i, j]
+                        blk += star[:, None, None] * f[None]
+                        dblk -= 1j * zr[:, None, None, :] * f[None, :, :, None]
+                        star_all += star
+                        zr_all += zr
+            # the dielectric constants enter every cell's fc_total alike
+            fd = fc_diel[:, 3 * a2:3 * a2 + 3, 3 * a1:3 * a1 + 3].transpose(0, 2, 1)   # [k, i, j]
+            blk += star_all[:, None, None] * fd
+            dblk -= 1j * zr_all[:, None, None, :] * fd[..., None]
+            # nonanalytic part of dD/dq, star times ddyn_nac / ncell per cell (:249-257)
+            dn = ddyn_nac[:, 3 * a1:3 * a1 + 3, 3 * a2:3 * a2 + 3, :]
+            dblk += np.where(kq_on[:, None, None, None], star_all[:, None, None, None] * dn / ncell, 0.0)
+            dyn[:, 3 * a1:3 * a1 + 3, 3 * a2:3 * a2 + 3] = blk
+            ddyn[:, 3 * a1:3 * a1 + 3, 3 * a2:3 * a2 + 3, :] = dblk
+    w2, Vv = np.linalg.eigh(dyn, UPLO="U")
+    omega = np.sign(w2) * np.sqrt(np.abs(w2))
+    vel = np.einsum("kci,kcdj,kdi->kij", Vv.conj(), ddyn, Vv).real / (2.0 * omega[:, :, None])
+    eig = np.ascontiguousarray(np.transpose(Vv, (0, 2, 1)))
+    return omega, vel, eig
+
+
+def modes_from_phonopy(st: crystal.Structure, fc, scell, ngrid, nonanalytic=False, epsilon=None, born=None):
+    """eigenDM with the phonopy front end on the full grid (as
+    modes_from_espresso: q-point symmetrisers on v, Gamma acoustic omega and
+    v set to zero), with the irreducible wedge and symmetry tables."""
+    ngrid = np.asarray(ngrid, np.int32)
+    kp = grid_kpoints(st.rlattvec, ngrid)
+    omega, vel, eig = phonons_phonopy(kp, st, scell, fc, nonanalytic, epsilon, born)
+    Lst, Nequi, ALLEq, TypeS, ns, Cr, Qr, symm, ID = crystal.symmetry_tables(st, ngrid)
+    vel = np.einsum("abq,qnb->qna", symm, vel)
+    raw_gamma = omega[0, :3].copy()
+    omega[0, :3] = 0.0
+    vel[0, :3, :] = 0.0
+    modes = crystal.Modes(ngrid=ngrid, energy=np.asfortranarray(omega), velocity=np.asfortranarray(vel),
+                          eigenvect=np.asfortranarray(eig), IJK=crystal.grid_ijk(ngrid), Nlist=len(Lst), List=Lst,
+                          Nequi=Nequi, ALLEquiList=ALLEq, TypeofSymmetry=TypeS, nsymm_rot=ns, crotations=Cr,
+                          qrotations=Qr, symmetrizers=symm, ID_equi=ID)
+    modes.raw_gamma = raw_gamma
+    return modes
 
 
 # ---------------------------------------------------------------------------

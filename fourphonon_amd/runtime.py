@@ -37,14 +37,15 @@ ERRORS = {1: "FP4_ERR_ARG", 2: "FP4_ERR_DEVICE", 3: "FP4_ERR_NOMEM", 4: "FP4_ERR
 EXPORTS = [
     "fp4_last_error", "fp4_ctx_create", "fp4_ctx_destroy", "fp4_ctx_set_temperature", "fp4_ctx_set_modes",
     "fp4_ctx_set_ifc4", "fp4_np_4ph", "fp4_ind_only_4ph", "fp4_rta_4ph", "fp4_rta_4ph_modes",
-    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_np_4ph_modes_noalloc", "fp4_ind_only_4ph_modes",
+    "fp4_rta_4ph_using_ind", "fp4_ind_4ph", "fp4_vp_4ph", "fp4_np_4ph_modes", "fp4_np_4ph_modes_noalloc", "fp4_list_hash_4ph_modes", "fp4_ind_only_4ph_modes",
     "fp4_ctx_kernel_stats",
     "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape", "fp4_ctx_dense_kernel", "fp4_ctx_dense_candidates", "fp4_ctx_set_dense_shape", "fp4_ctx_dense_phase_profile",
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
     "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
     "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
-    "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense",
+    "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense", "fp4_bte_assemble_operator",
+    "fp4_bte_operator_begin", "fp4_bte_operator_add", "fp4_bte_operator_end",
 ]
 
 
@@ -209,6 +210,16 @@ class Context:
                                                _p(N[1]), _p(P[1]), _p(N[2]), _p(P[2])))
         return N[0], P[0], N[1], P[1], N[2], P[2]
 
+    def list_hash_4ph_modes(self, modes):
+        """Counts and order-sensitive list hashes of a mode subset, computed on the device without
+        materialising the lists -> (N[3, nmodes] int64, H[3, nmodes] uint64)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        N = np.zeros((3, len(modes)), np.int64)
+        H = np.zeros((3, len(modes)), np.uint64)
+        _check(load().fp4_list_hash_4ph_modes(self.h, C.c_int32(len(modes)), _p(modes), _p(N[0]), _p(N[1]),
+                                              _p(N[2]), _p(H[0]), _p(H[1]), _p(H[2])))
+        return N, H
+
     def Ind_only_driver_4ph(self, N_pp, N_pm, N_mm):
         """-> [[Indof2nd, Indof3rd, Indof4th] for ++, +-, --] (int64, reference encoding)."""
         Ns = [np.ascontiguousarray(n, np.int64) for n in (N_pp, N_pm, N_mm)]
@@ -357,6 +368,41 @@ class Context:
         keep, args = self._operator_args(N, lists, gammas, N3, lists3, gammas3)
         _check(load().fp4_bte_set_operator(self.h, *args))
         del keep
+
+    def bte_assemble_operator(self, precision=None, with_3ph=True, with_4ph=True, rates=False):
+        """Assembles the aggregated scattering operator on the device at the context's T (3ph from the
+        FP64 per-process Gamma, 4ph in the given form; default FP64 per-process) -> (n3, n4) processes,
+        plus (rate_3ph, rate_4ph) as (Nb, Nlist) arrays when rates=True."""
+        precision = PREC_FP64 if precision is None else precision
+        Nequi, alleq, types = self._orbits()
+        n3, n4 = C.c_int64(), C.c_int64()
+        r3 = np.zeros((self.nb, self.nlist), order="F")
+        r4 = np.zeros((self.nb, self.nlist), order="F")
+        _check(load().fp4_bte_assemble_operator(self.h, C.c_int32(self.nlist), _p(Nequi), _p(alleq), _p(types),
+                                                C.c_int32(precision), C.c_int32(int(bool(with_3ph))),
+                                                C.c_int32(int(bool(with_4ph))), C.byref(n3), C.byref(n4),
+                                                _p(r3) if rates else None, _p(r4) if rates else None))
+        if rates:
+            return n3.value, n4.value, r3, r4
+        return n3.value, n4.value
+
+    def bte_operator_begin(self):
+        Nequi, alleq, types = self._orbits()
+        _check(load().fp4_bte_operator_begin(self.h, C.c_int32(self.nlist), _p(Nequi), _p(alleq), _p(types)))
+
+    def bte_operator_add(self, modes, precision, with_3ph=True, with_4ph=True, r3=None, r4=None):
+        """Adds the processes of a mode subset to the operator being assembled; rates of those modes are
+        written into r3 / r4 ((Nb, Nlist) F-order arrays) when given -> (n3, n4)."""
+        modes = np.ascontiguousarray(np.sort(np.asarray(modes)), np.int32)
+        n3, n4 = C.c_int64(), C.c_int64()
+        _check(load().fp4_bte_operator_add(self.h, C.c_int32(precision), C.c_int32(int(bool(with_3ph))),
+                                           C.c_int32(int(bool(with_4ph))), C.c_int32(len(modes)), _p(modes),
+                                           C.byref(n3), C.byref(n4), _p(r3) if r3 is not None else None,
+                                           _p(r4) if r4 is not None else None))
+        return n3.value, n4.value
+
+    def bte_operator_end(self):
+        _check(load().fp4_bte_operator_end(self.h))
 
     def iteration_4ph(self, N, lists, gammas, tau_zero, F_n, N3=None, lists3=None, gammas3=None):
         """iteration_4ph on a copy of F_n; returns the updated F_n."""

@@ -160,6 +160,16 @@ int fp4_np_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t
 int fp4_np_4ph_modes_noalloc(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t *N_pp, double *P_pp,
                              int64_t *N_pm, double *P_pm, int64_t *N_mm, double *P_mm);
 
+/* Checks the lists of Ind_only_driver_4ph (Src/processes.f90:3080-3246) for
+   a mode subset without materialising them: per (channel, mode) the count
+   and the order-sensitive hash H = sum_p key_p*M^(n-1-p) mod 2^64 of the
+   list the reference would write, key_p = (i2<<42)^(i3<<21)^i4 of entry p
+   (Indof2nd/3rd/4thPhonon), M = 0x9E3779B97F4A7C15.  No counterpart in the
+   reference; it lets a caller compare a whole mesh's lists (8.1e10 entries
+   at 16^3) against stored hashes.  N_* and H_* receive nmodes entries. */
+int fp4_list_hash_4ph_modes(fp4_ctx *ctx, int32_t nmodes, const int32_t *modes, int64_t *N_pp, int64_t *N_pm,
+                            int64_t *N_mm, uint64_t *H_pp, uint64_t *H_pm, uint64_t *H_mm);
+
 /* Ind_only_driver_4ph restricted to a strictly increasing subset of modes
    (mm, 1-based), for callers that split the mode range the way the
    reference's MPI chunks do (Src/processes.f90:3160-3203).  N_* receive the
@@ -352,6 +362,45 @@ int fp4_iteration_4ph(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const i
                       const int64_t *Indof4thPhonon_mm, const double *Gamma_plus, const double *Gamma_minus,
                       const double *Gamma_pp, const double *Gamma_pm, const double *Gamma_mm,
                       const double *tau_zero, double *F_n);
+
+/* The scattering operator of iteration_4ph assembled on the device for the
+   context's T, for meshes whose process lists do not fit in memory.  No
+   counterpart in the reference, which stores the lists of Ind_driver /
+   Ind_driver_4ph (Src/processes.f90:524-707, 3551-3822) and sums over them
+   in iteration_4ph (Src/iterations.f90:95-215).  The sum over a mode's
+   processes is linear in F_n and each of its terms reads F_n at one partner
+   mode, so it is regrouped by partner: W[(ll,i)][(q,b)] = sum of the signed
+   Gamma of every process of mode (ll,i) with partner (q,b), and the sweep
+   computes DeltaF(i,q_kk) = sum W[(ll,i)][(q,b)] F_n(b, ID_equi(S_kk,q), :)
+   in the reference's (ll, i, kk) order.  W takes Nb*Nlist*nptk*Nb doubles
+   (590 MB at 20^3, Nb = 6) whatever the number of processes.  The 3ph part
+   (with_3ph, needs fp4_ctx_set_ifc3) comes from the FP64 per-process
+   Gamma of Ind_driver; the 4ph part (with_4ph) from the FP64 per-process
+   form (precision FP4_PREC_FP64) or a dense form (the MFMA forms, or
+   FP4_PREC_FP64_DENSE).  The additions are atomic, so W agrees between
+   runs to rounding, not bit for bit.  Orbits as in fp4_bte_set_operator;
+   later fp4_bte_iterate / fp4_bte_solve calls use W until the next
+   fp4_bte_set_operator.  *nproc3 / *nproc4 (may be NULL) receive the
+   number of processes added, rate_3ph / rate_4ph(Nb,Nlist) (may be NULL)
+   the per-mode scattering rates of the same processes (Ind_driver's and
+   RTA_driver_4ph's rate_scatt, for tau_zero), so no separate rate pass is
+   needed. */
+int fp4_bte_assemble_operator(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                              const int32_t *TypeofSymmetry, int32_t precision, int32_t with_3ph, int32_t with_4ph,
+                              int64_t *nproc3, int64_t *nproc4, double *rate_3ph, double *rate_4ph);
+
+/* fp4_bte_assemble_operator in parts, for callers that assemble a large
+   operator in pieces (progress reports, several callers sharing the mode
+   range): _begin sets the orbits and zeroes W, each _add adds the processes
+   of a strictly increasing subset of modes (mm, 1-based; rates written at
+   those modes' entries only), _end makes the sweeps use W.  Subsets must
+   not overlap. */
+int fp4_bte_operator_begin(fp4_ctx *ctx, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
+                           const int32_t *TypeofSymmetry);
+int fp4_bte_operator_add(fp4_ctx *ctx, int32_t precision, int32_t with_3ph, int32_t with_4ph, int32_t nmodes,
+                         const int32_t *modes, int64_t *nproc3, int64_t *nproc4, double *rate_3ph,
+                         double *rate_4ph);
+int fp4_bte_operator_end(fp4_ctx *ctx);
 
 /* One iteration_4ph sweep with the operator of the last
    fp4_bte_set_operator / fp4_iteration_4ph call; F_n updated in place. */

@@ -159,14 +159,9 @@ typedef struct {
 
 typedef void (*visit_fn)(const orc_case *c, const orc_hit *h, void *ud);
 
-/* The tests of one candidate (ii, jj, ss; j, k, l) of mode (i, qi) with
-   frequency omega in channel ch: nonzero frequencies, the +- pair
-   exclusions and the window |delta| <= sigma, then the 0.001 clamp.  Fills
-   h and returns 1 for a listed process. */
-static int test4(const orc_case *c, int ch, int i, int qi, double omega, int ii, int jj, int ss, int j, int k, int l,
-                 orc_hit *h) {
-  const double op = E(c, ii, j), odp = E(c, jj, k), otp = E(c, ss, l);
-  if (!((op != 0.0) && (odp != 0.0) && (otp != 0.0))) return 0;
+/* sigma of the pair (jj, k; ss, l) in channel ch: scalebroad*base_sigma of
+   -v(jj,k)+v(ss,l) (++) or v(jj,k)-v(ss,l) (+-, --), before the clamp. */
+static double pair_sigma4(const orc_case *c, int ch, int jj, int k, int ss, int l) {
   double dv[3];
   for (int a = 1; a <= 3; a++) {
     if (ch == 0)
@@ -174,7 +169,54 @@ static int test4(const orc_case *c, int ch, int i, int qi, double omega, int ii,
     else
       dv[a - 1] = VEL(c, jj, k, a) - VEL(c, ss, l, a);
   }
-  double sigma = c->scalebroad * orc_base_sigma(c, dv);
+  return c->scalebroad * orc_base_sigma(c, dv);
+}
+
+/* sig[(k-1)*nb + (l-1)] = pair_sigma4(c, ch, jj, k, ss, l) for all (k, l):
+   the same operations per entry, with the velocities gathered first. */
+static void pair_sigma4_table(const orc_case *c, int ch, int jj, int ss, double *sig) {
+  const int nb = c->nbands;
+  double vj[24][3], vs[24][3];
+  for (int b = 1; b <= nb; b++)
+    for (int a = 1; a <= 3; a++) {
+      vj[b - 1][a - 1] = VEL(c, jj, b, a);
+      vs[b - 1][a - 1] = VEL(c, ss, b, a);
+    }
+  const double *bv = c->rlattvec;
+  const double n0 = (double)c->ngrid[0], n1 = (double)c->ngrid[1], n2 = (double)c->ngrid[2];
+  const double sb = c->scalebroad;
+  for (int k = 0; k < nb; k++)
+    for (int l = 0; l < nb; l++) {
+      double dv0, dv1, dv2;
+      if (ch == 0) {
+        dv0 = -vj[k][0] + vs[l][0]; dv1 = -vj[k][1] + vs[l][1]; dv2 = -vj[k][2] + vs[l][2];
+      } else {
+        dv0 = vj[k][0] - vs[l][0]; dv1 = vj[k][1] - vs[l][1]; dv2 = vj[k][2] - vs[l][2];
+      }
+      /* orc_base_sigma's operations, in its order */
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+      d0 = d0 + bv[0] * dv0; d0 = d0 + bv[1] * dv1; d0 = d0 + bv[2] * dv2;
+      d1 = d1 + bv[3] * dv0; d1 = d1 + bv[4] * dv1; d1 = d1 + bv[5] * dv2;
+      d2 = d2 + bv[6] * dv0; d2 = d2 + bv[7] * dv1; d2 = d2 + bv[8] * dv2;
+      const double t0 = d0 / n0, t1 = d1 / n1, t2 = d2 / n2;
+      const double s0 = t0 * t0, s1 = t1 * t1, s2 = t2 * t2;
+      double m = s0;
+      if (s1 > m) m = s1;
+      if (s2 > m) m = s2;
+      sig[k * nb + l] = sb * sqrt(m / 2.0);
+    }
+}
+
+/* The tests of one candidate (ii, jj, ss; j, k, l) of mode (i, qi) with
+   frequency omega in channel ch: nonzero frequencies, the +- pair
+   exclusions and the window |delta| <= sigma, then the 0.001 clamp.  op,
+   odp, otp are E(ii,j), E(jj,k), E(ss,l); sigma is pair_sigma4(ch, jj, k,
+   ss, l), which does not depend on j, so enumerate() evaluates it once per
+   (jj, ss, k, l).  Fills h and returns 1 for a listed process. */
+static inline int test4(const orc_case *c, int ch, int i, int qi, double omega, int ii, int jj, int ss, int j, int k,
+                        int l, double op, double odp, double otp, double sigma, orc_hit *h) {
+  (void)c;
+  if (!((op != 0.0) && (odp != 0.0) && (otp != 0.0))) return 0;
   if (ch == 1) {
     if (!((qi != jj || i != k) && (qi != ss || i != l) && (ii != jj || j != k) && (ii != ss || j != l))) return 0;
   }
@@ -224,20 +266,33 @@ static void enumerate(const orc_case *c, int ch, int mm, int ii_lo, int ii_hi, v
   if (!(omega != 0.0)) return;
   if (ii_lo < 1) ii_lo = 1;
   if (ii_hi > nptk) ii_hi = nptk;
+  /* per (jj, ss): the sigma table and the three phonons' frequencies */
+  double *sig = (double *)malloc(sizeof(double) * ((size_t)nb * nb + 3 * (size_t)nb));
+  if (!sig) abort();
+  double *Eii = sig + (size_t)nb * nb, *Ejj = Eii + nb, *Ess = Ejj + nb;
   for (int ii = ii_lo; ii <= ii_hi; ii++) {
+    for (int b = 1; b <= nb; b++) Eii[b - 1] = E(c, ii, b);
     for (int jj = (ch == 1 ? 1 : ii); jj <= nptk; jj++) {
       const int ss = third_partner(c, ch, qi, ii, jj);
       if (ch == 0 ? !(ss >= 1) : !(ss >= jj)) continue;
+      for (int b = 1; b <= nb; b++) {
+        Ejj[b - 1] = E(c, jj, b);
+        Ess[b - 1] = E(c, ss, b);
+      }
+      pair_sigma4_table(c, ch, jj, ss, sig);
       for (int j = 1; j <= nb; j++) {
         const int kstart = (ch != 1 && jj == ii) ? j : 1;
         for (int k = kstart; k <= nb; k++) {
           const int lstart = (ch != 0 && ss == jj) ? k : 1;
           for (int l = lstart; l <= nb; l++)
-            if (test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, &h)) f(c, &h, ud);
+            if (test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, Eii[j - 1], Ejj[k - 1], Ess[l - 1],
+                      sig[(k - 1) * nb + (l - 1)], &h))
+              f(c, &h, ud);
         }
       }
     }
   }
+  free(sig);
 }
 
 /* ---------------------------------------------------------------- counts */
@@ -284,6 +339,36 @@ int64_t orc_ind_only_mode(const orc_case *c, int ch, int mm, int64_t cap, int64_
   a.n = 0; a.cap = cap; a.i2 = i2; a.i3 = i3; a.i4 = i4; a.nb = c->nbands;
   enumerate(c, ch, mm, 1, c->nptk, visit_list, &a);
   return a.n;
+}
+
+/* Order-sensitive hash of one mode's list, streamed (no list is kept):
+   h = sum_p key_p * M^(n-1-p) mod 2^64 with key = (i2 << 42) ^ (i3 << 21) ^ i4
+   and M = 0x9E3779B97F4A7C15, evaluated by Horner's rule h = h*M + key in
+   list order.  Also returns the count and phase space of the same pass. */
+typedef struct { uint64_t h; np_acc np; int nb; } hash_acc;
+
+static void visit_hash(const orc_case *c, const orc_hit *h, void *ud) {
+  hash_acc *a = (hash_acc *)ud;
+  const uint64_t i2 = (uint64_t)((h->ii - 1) * a->nb + h->j);
+  const uint64_t i3 = (uint64_t)((h->jj - 1) * a->nb + h->k);
+  const uint64_t i4 = (uint64_t)((h->ss - 1) * a->nb + h->l);
+  a->h = a->h * 0x9E3779B97F4A7C15ULL + ((i2 << 42) ^ (i3 << 21) ^ i4);
+  visit_np(c, h, &a->np);
+}
+
+uint64_t orc_list_hash_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P) {
+  hash_acc a;
+  const double n = (double)c->nptk;
+  a.h = 0;
+  a.nb = c->nbands;
+  a.np.n = 0;
+  a.np.p = 0.0;
+  a.np.dn3 = n * n * n;
+  a.np.dnb4 = (double)(c->nbands * c->nbands * c->nbands * c->nbands);
+  enumerate(c, ch, mm, 1, c->nptk, visit_hash, &a);
+  *N = a.np.n;
+  *P = a.np.p;
+  return a.h;
 }
 
 /* ------------------------------------------------------ matrix elements */
@@ -579,7 +664,9 @@ void orc_sample_mode(const orc_case *c, int ch, int mm, int32_t nsample, uint64_
       if (ch != 1 && !(jj >= ii && (jj != ii || k >= j))) continue;
       if (ch != 0 && !(ss >= jj && (ss != jj || l >= k))) continue;
       orc_hit h;
-      if (!test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, &h)) continue;
+      if (!test4(c, ch, i, qi, omega, ii, jj, ss, j, k, l, E(c, ii, j), E(c, jj, k), E(c, ss, l),
+                 pair_sigma4(c, ch, jj, k, ss, l), &h))
+        continue;
       visit_np(c, &h, &na);
       if (want_rates) visit_rta(c, &h, &ra);
     }

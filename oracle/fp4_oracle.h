@@ -57,6 +57,9 @@ double orc_base_sigma(const orc_case *c, const double v[3]);
 void orc_np_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P);
 int64_t orc_ind_only_mode(const orc_case *c, int ch, int mm, int64_t cap,
                           int64_t *i2, int64_t *i3, int64_t *i4);
+/* Streamed order-sensitive hash of the list orc_ind_only_mode would write
+   (sum_p key_p M^(n-1-p) mod 2^64), with its count and phase space. */
+uint64_t orc_list_hash_mode(const orc_case *c, int ch, int mm, int64_t *N, double *P);
 void orc_rta_mode(const orc_case *c, int ch, int mm, int32_t ii_lo, int32_t ii_hi,
                   double *gamma, double *wp4);
 int64_t orc_ind_mode(const orc_case *c, int ch, int mm, int64_t cap,

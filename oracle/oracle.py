@@ -124,6 +124,14 @@ class Oracle:
                                      *[_ptr(o) for o in out])
         return [o[:n] for o in out]
 
+    def list_hash_mode(self, ch, mm):
+        """(count, phase space, list hash) of one mode's list, streamed without storing the list."""
+        n = C.c_int64()
+        p = C.c_double()
+        self.L.orc_list_hash_mode.restype = C.c_uint64
+        h = self.L.orc_list_hash_mode(C.byref(self.c), C.c_int(ch), C.c_int(mm), C.byref(n), C.byref(p))
+        return n.value, p.value, int(h)
+
     def vp(self, ch, i, j, k, l, q, qp, qdp, qtp):
         out = np.zeros(2)
         self.L.orc_vp_idx(C.byref(self.c), C.c_int(ch), *[C.c_int(int(x)) for x in (i, j, k, l, q, qp, qdp, qtp)],

@@ -107,6 +107,70 @@ def part_cfg2():
                         **{k: v[rmodes - 1] for k, v in r.items()})
 
 
+def streamed_counts_and_hashes(o, modes):
+    """Counts, phase space and list hashes of the given modes from the
+    oracle's streamed hash (no list is stored), one mode per task."""
+    from concurrent.futures import ThreadPoolExecutor
+    nm = len(modes)
+    N = np.zeros((3, nm), np.int64)
+    P = np.zeros((3, nm))
+    H = np.zeros((3, nm), np.uint64)
+    done = [0]
+
+    def one(arg):
+        t, mm = arg
+        return t, [o.list_hash_mode(ch, int(mm)) for ch in range(3)]
+
+    with ThreadPoolExecutor(THREADS) as ex:
+        for t, res in ex.map(one, list(enumerate(modes))):
+            for ch, (n, p, h) in enumerate(res):
+                N[ch, t], P[ch, t], H[ch, t] = n, p, h
+            done[0] += 1
+            if done[0] % 29 == 0:
+                log(f"  {done[0]}/{nm} modes")
+    return N, P, H
+
+
+def part_cfg2lists():
+    """cfg2 at full size: counts, phase space and list hashes of all 870
+    modes in all three channels (the whole 8.1e10-process list, hashed)."""
+    case = crystal.config_case(2)
+    o = Oracle(case)
+    allm = np.arange(1, o.nmodes + 1, dtype=np.int32)
+    log("cfg2lists: counts + streamed hashes for", len(allm), "modes on", THREADS, "threads")
+    t0 = time.time()
+    N, P, H = streamed_counts_and_hashes(o, allm)
+    log("cfg2lists: took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "cfg2_lists.npz"), ngrid=np.array([16, 16, 16]), shell=2, workload=2,
+                        modes=allm, N=N, P=P, H=H, seconds=time.time() - t0, threads=THREADS)
+
+
+def part_iter6():
+    """The joint 3ph+4ph iterative solution on the 6x6x6 diamond (1st-shell
+    FC3/FC4, 300 K): the oracle's counts, tau0, iteration count, kappa per
+    iteration and converged F_n from its own lists and Gamma (5.9e7 4ph
+    processes; about ten minutes on eight cores)."""
+    from oracle.oracle import bte_solve
+    case = crystal.config_case(1, ngrid=(6, 6, 6))
+    o = Oracle(case)
+    t0 = time.time()
+    No3, _ = o.np3_driver(nthreads=THREADS)
+    wl3, wg3, wr3 = o.ind3_driver(No3, nthreads=THREADS)
+    No4, _ = o.np_driver(nthreads=THREADS)
+    log("iter6: 4ph lists and Gamma for", sum(int(x.sum()) for x in No4), "processes")
+    wl4, wg4, wr4 = o.ind_driver(No4, nthreads=THREADS)
+    rate = (wr3["rate"] + wr4["rate_4ph"]).reshape((case.modes.nbands, case.modes.Nlist), order="F")
+    tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+    log("iter6: iterating")
+    r = bte_solve(case, No4, wl4, wg4, tau, maxiter=60, N3=No3, lists3=wl3, gammas3=wg3)
+    log("iter6: niter", r["niter"], "took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "iter6.npz"), ngrid=np.array([6, 6, 6]), shell=1, workload=1, T=300.0,
+                        N3=np.array([No3[0], No3[1]]), N4=np.array([No4[0], No4[1], No4[2]]),
+                        rate_3ph=wr3["rate"], rate_4ph=wr4["rate_4ph"], tau=tau, niter=r["niter"],
+                        kappa_iter=r["kappa_iter"], kappa=r["kappa"], relchange=r["relchange"], F_n=r["F_n"],
+                        seconds=time.time() - t0, threads=THREADS)
+
+
 def part_cfg3():
     case = crystal.config_case(3)
     o = Oracle(case)

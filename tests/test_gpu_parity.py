@@ -251,6 +251,54 @@ def test_golden_diamond_16cube_shell2():
     c.close()
 
 
+def test_list_hash_pass_equals_hash_of_emitted_lists(ctx4, orc4):
+    """The device hash pass (no list written) against the hash of the lists
+    the emit pass writes and against the oracle's streamed hash, all modes."""
+    banner("list hash pass 4x4x4, all modes")
+    modes = np.arange(1, ctx4.nmodes + 1, dtype=np.int32)
+    N, H = ctx4.list_hash_4ph_modes(modes)
+    Np, Npm, Nmm, lists = ctx4.Ind_only_driver_4ph_modes(modes)
+    for ch, Nch in enumerate((Np, Npm, Nmm)):
+        assert np.array_equal(N[ch], Nch), ch
+        acc = np.concatenate([[0], np.cumsum(Nch)])
+        for t, mm in enumerate(modes):
+            h = list_hash(*(x[acc[t]:acc[t + 1]] for x in lists[ch]))
+            assert int(H[ch, t]) == h, (ch, int(mm))
+            if t % 7 == 0:
+                n, _, ho = orc4.list_hash_mode(ch, int(mm))
+                assert (n, ho) == (int(N[ch, t]), h), (ch, int(mm))
+
+
+def test_golden_diamond_8cube_all_lists():
+    g = _golden("cfg1")
+    banner("golden diamond 8x8x8: list hashes of all modes by the hash pass")
+    c = rt.Context(crystal.config_case(1))
+    N, H = c.list_hash_4ph_modes(g["modes"])
+    assert np.array_equal(N, g["N"])
+    assert np.array_equal(H, g["H"])
+    c.close()
+
+
+def test_golden_diamond_16cube_shell2_all_lists():
+    """Workload 2 at full size: the counts and order-sensitive hashes of the
+    lists of all 870 modes in all three channels (8.1e10 processes) equal the
+    oracle's, and the phase space agrees to 1e-10."""
+    g = _golden("cfg2_lists")
+    banner("golden diamond 16x16x16 shell 2: all 870 modes, counts and list hashes")
+    c = rt.Context(crystal.config_case(2))
+    modes = g["modes"]
+    assert len(modes) == c.nmodes
+    N, H = c.list_hash_4ph_modes(modes)
+    assert np.array_equal(N, g["N"])
+    bad = [(ch, int(modes[t])) for ch in range(3) for t in np.nonzero(H[ch] != g["H"][ch])[0]]
+    assert not bad, f"{len(bad)} list hashes differ, first {bad[:5]}"
+    Nd = c.NP_driver_4ph()
+    for ch in range(3):
+        assert np.array_equal(Nd[2 * ch], g["N"][ch]), ch
+        assert_close(Nd[2 * ch + 1].ravel(order="F"), g["P"][ch], RTOL, f"P {ch}")
+    c.close()
+
+
 def test_golden_wurtzite_12x12x8_shell2():
     g = _golden("cfg3")
     banner("golden wurtzite 12x12x8 shell 2 (workload 3, Nb = 12)")

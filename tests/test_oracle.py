@@ -21,6 +21,7 @@ import pytest
 
 from fourphonon_amd import crystal
 from oracle.oracle import Oracle, iteration0, iteration_4ph, tconduct
+from tests.golden.listhash import list_hash
 
 
 # ---------------------------------------------------------------------------
@@ -105,6 +106,10 @@ def test_lists_match_vectorised_enumeration_all_modes(ngrid):
             assert got.shape == ref.shape, (ch, mm)
             assert np.array_equal(got, ref), (ch, mm)
             assert o.np_mode(ch, mm)[0] == ref.shape[1]
+            # the streamed hash (golden fixtures of whole meshes use it)
+            n, p, h = o.list_hash_mode(ch, mm)
+            assert (n, p) == o.np_mode(ch, mm)
+            assert h == list_hash(*ref), (ch, mm)
 
 
 def test_lists_match_vectorised_enumeration_4cube(small_case, small_oracle):
