@@ -214,8 +214,14 @@ struct DenseSmem {
 
 // MF: 0 = VALU Psi, 1 = FP32 MFMA Psi, 2 = BF16 hi/lo MFMA Psi, 3 = FP16 hi/lo MFMA Psi,
 // 4 = FP16 Psi with hi/lo phases against one A' plane.
-template <class T, int NB, int IB, int TQ, int MF>
-__global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
+//
+// BATCH: one launch over several crystals with the same Nb (the 24-crystal
+// workload): the queue hands out items of every crystal, in the host's order
+// (largest first), and a block reloads the crystal's group vectors and
+// roots when its next item belongs to another crystal.  The LDS layout is
+// the launch's (the largest group, cluster and root counts of the crystals).
+template <class T, int NB, int IB, int TQ, int MF, bool BATCH = false>
+__global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs ka) {
   using S = DenseSmem<T, NB, IB, TQ>;
   constexpr int NA = NB / 3;
   constexpr int QMAX = S::QMAX;
@@ -227,13 +233,12 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   // FP16 form: A' is scaled by a_scale (a power of two that bounds it by 2^14)
   // and the phases by kPhaseScaleF16 before the split; Psi is unscaled after
   // the GEMM, exactly (powers of two)
-  const float psi_unscale = (MF >= 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
   static_assert(TQ * NB * NB * IB * NB <= 65536, "candidate codes must fit 16 bits");
   constexpr unsigned long long kJall = (1ull << NB) - 1ull;   // bits of all j of one i
   constexpr unsigned long long kRep = rep_mask(IB, NB);       // bit j = 0 of every i
   extern __shared__ __align__(16) unsigned char dyn[];
   S &s = *reinterpret_cast<S *>(dyn);
-  const DynLayout dl = dyn_layout<T, NB, IB, TQ, MF>(a.smem_static, a.ng, a.ntri, a.roots_lds ? a.L : 0);
+  const DynLayout dl = dyn_layout<T, NB, IB, TQ, MF>(ka.smem_static, ka.ng, ka.ntri, ka.roots_lds ? ka.L : 0);
   C2<T> *s_A = reinterpret_cast<C2<T> *>(dyn + dl.A);     // [ng][NOUT2]
   C2<T> *s_c = reinterpret_cast<C2<T> *>(dyn + dl.c);     // [ntri]
   C2<T> *s_ph = reinterpret_cast<C2<T> *>(dyn + dl.ph);   // [TQ][ng] (VALU path)
@@ -241,22 +246,41 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense(DenseArgs a) {
   double2 *s_rt = reinterpret_cast<double2 *>(dyn + dl.rt);
   int *s_gd = reinterpret_cast<int *>(dyn + dl.gd);       // [ng][3]
 
-  const ModeView &m = a.m;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const double hbarp = kHbar * (double)k1e22;
   const double gpre = hbarp * hbarp * kPi / 8.0;
-  for (int x = tid; x < 3 * a.ng; x += NT_DENSE) s_gd[x] = a.g[x / 3].d[x % 3];
-  if (a.roots_lds)
-    for (int x = tid; x < a.L; x += NT_DENSE) s_rt[x] = a.roots[x];
-  __syncthreads();
+  auto load_tables = [&](const DenseArgs &a) {
+    for (int x = tid; x < 3 * a.ng; x += NT_DENSE) s_gd[x] = a.g[x / 3].d[x % 3];
+    if (a.roots_lds)
+      for (int x = tid; x < a.L; x += NT_DENSE) s_rt[x] = a.roots[x];
+    __syncthreads();
+  };
+  if constexpr (!BATCH) load_tables(ka);
+  int cur_cr = -1;
 
   for (;;) {
-    if (tid == 0) s.item = atomicAdd(a.counter, 1);
+    if (tid == 0) s.item = atomicAdd(ka.counter, 1);
     __syncthreads();
-    const long item = s.item;
+    const long gitem = s.item;
     __syncthreads();
-    if (item >= a.nitems) break;
+    if (gitem >= ka.nitems) break;
+    long item = gitem;
+    int cr = 0;
+    if constexpr (BATCH) {
+      const long long o = ka.order[gitem];
+      cr = __builtin_amdgcn_readfirstlane((int)(o >> 40));
+      item = (long)(o & 0xFFFFFFFFFFLL);
+    }
+    const DenseArgs &a = dense_args_of<BATCH>(ka, cr);
+    if constexpr (BATCH) {
+      if (cr != cur_cr) {
+        load_tables(a);
+        cur_cr = cr;
+      }
+    }
+    const ModeView &m = a.m;
+    const float psi_unscale = (MF >= 3) ? 1.0f / (kPhaseScaleF16 * a.a_scale) : 1.0f;
     const int ii0 = (int)(item % a.nii) + a.ii_lo0;
     long r_ = item / a.nii;
     const int ib = (int)(r_ % a.nib);

@@ -68,7 +68,23 @@ struct DenseArgs {
   double *wop;
   long long wcols;
   double wscale;
+  // several crystals of one Nb in one launch: the per-crystal arguments in
+  // device memory, and the launch's work items in queue order, each packed
+  // as crystal << 40 | the crystal's own item index (nbatch = 0: one crystal,
+  // these arguments)
+  const DenseArgs *batch;
+  const long long *order;
+  int nbatch;
 };
+
+// The arguments of the crystal a work item belongs to.
+template <bool BATCH>
+__device__ __forceinline__ const DenseArgs &dense_args_of(const DenseArgs &ka, int cr) {
+  if constexpr (BATCH)
+    return ka.batch[cr];
+  else
+    return ka;
+}
 
 // Adds one process's Gamma to the aggregated operator with the signs of the
 // F_n terms of iteration_4ph (Src/iterations.f90:177-203): ++ -F(j,q2)
