@@ -170,11 +170,12 @@ def main():
     ap.add_argument("--precision", choices=list(FORMS), default="fp16x2",
                     help="MFMA form of the timed steps: FP16 with hi/lo phases (fp16x2), BF16 or FP16 hi/lo "
                          "(3 products), or FP32 MFMA; FP32 accumulation throughout")
-    ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16", "wurtzite12"], default="rta16",
+    ap.add_argument("--workload", choices=["rta16", "iterative20", "sample16", "wurtzite12", "batch24"], default="rta16",
                     help="rta16: the 4ph RTA throughput on diamond 16^3 (the metric); iterative20: diamond 20^3, "
                          "iterative kappa with the assembled 3ph+4ph operator at 100/300/700 K; sample16: the sampling estimator on "
                          "diamond 16^3 against the full rates; wurtzite12: every MFMA form on the 12x12x8 "
-                         "wurtzite against dense FP64 (per-mode and kappa errors)")
+                         "wurtzite against dense FP64 (per-mode and kappa errors); batch24: the 24-crystal batch, "
+                         "2- and 4-atom cells in one graph of one persistent launch per Nbands class")
     ap.add_argument("--phase-profile", action="store_true",
                     help="load the profiling build (libfp4prof.so) and print the dense kernel's per-phase cycle "
                          "shares of the timed steps")
@@ -196,6 +197,9 @@ def main():
         return
     if args.workload == "wurtzite12":
         run_wurtzite12(args)
+        return
+    if args.workload == "batch24":
+        run_batch24(args)
         return
 
     from fourphonon_amd import crystal
@@ -533,6 +537,75 @@ def run_wurtzite12(args):
                                   f"{nm} modes: MFMA forms vs dense FP64",
                       "processes": nproc_total, "prefilter_seconds": t_pref, "dense_fp64_seconds": t64,
                       "kappa_rta_fp64": float(k64), "checks": checks, "forms": forms}), flush=True)
+
+
+def run_batch24(args):
+    """The 24-crystal batch (fcc / zincblende / rocksalt-like cells of 2, 4,
+    6 and 8 atoms, meshes 6^3 to 14^3 as drawn): the RTA pass of every 2- and
+    4-atom crystal (Nb 6 and 12) through fp4_batch_rta_4ph -- one persistent
+    FP32-MFMA launch per Nbands class over one queue of all their work items,
+    costliest first by the prefilter counts, in a captured HIP graph -- timed
+    over --steps calls; each crystal's FP64 per-process rates on a sample
+    (three modes, q' = 1..4) give the per-mode error.  The 6- and 8-atom
+    crystals (Nb 18 and 24), which the dense kernel does not take, are
+    counted only.  Prints one JSON line."""
+    from fourphonon_amd import crystal
+    from fourphonon_amd import runtime as rt
+
+    t0 = time.time()
+    cases = [crystal.batch_case(c) for c in range(crystal.BATCH_SIZE)]
+    log(f"24 cases built in {time.time() - t0:.1f}s")
+    ctxs = [rt.Context(cs) for cs in cases]
+    small = [k for k, cs in enumerate(cases) if cs.structure.natoms <= 4]
+    big = [k for k in range(len(cases)) if k not in small]
+    counts = {}
+    for k in big:
+        t1 = time.perf_counter()
+        N = ctxs[k].NP_driver_4ph()
+        counts[k] = (int(N[0].sum() + N[2].sum() + N[4].sum()), time.perf_counter() - t1)
+        log(f"crystal {k} (Nb {cases[k].nbands}, {list(cases[k].modes.ngrid)}): {counts[k][0]} processes counted")
+    t1 = time.perf_counter()
+    b = rt.Batch([ctxs[k] for k in small])
+    t_create = time.perf_counter() - t1
+    prec = rt.PREC_FP32
+    for w in range(args.warmup):
+        b.RTA_driver_4ph(prec)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    t1 = time.perf_counter()
+    kms = []
+    for st in range(args.steps):
+        res, nproc, ms, nl = b.RTA_driver_4ph(prec)
+        kms.append(ms)
+        log(f"step {st + 1}/{args.steps} done: {int(nproc.sum())} processes, graph {ms:.1f} ms, {nl} kernels")
+    wall = (time.perf_counter() - t1) / args.steps
+    errs = []
+    for j, k in enumerate(small):
+        c = ctxs[k]
+        tot = res[j]["rate_scatt_4ph"].ravel(order="F")
+        act = np.nonzero(tot > 0)[0] + 1
+        modes = act[np.linspace(0, len(act) - 1, 3).astype(int)]
+        r64, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64, ii_lo=1, ii_hi=4)
+        rb, _ = c.RTA_driver_4ph_modes(modes, prec, ii_lo=1, ii_hi=4)
+        a_, b_ = rb["rate_scatt_4ph"].ravel(order="F")[modes - 1], r64["rate_scatt_4ph"].ravel(order="F")[modes - 1]
+        ok = b_ != 0
+        errs.append(float(np.max(np.abs(a_[ok] - b_[ok]) / np.abs(b_[ok]), initial=0.0)))
+    b.close()
+    for c in ctxs:
+        c.close()
+    n_small = int(nproc.sum())
+    out = {"workload": "24-crystal batch (2/4/6/8-atom fcc, zincblende, rocksalt-like cells, meshes 6^3..14^3), "
+                       "4ph RTA at 300 K: the 2- and 4-atom crystals in one graph of one persistent FP32-MFMA "
+                       "launch per Nbands class",
+           "precision": "fp32", "steps": args.steps, "warmup": args.warmup,
+           "crystals_batched": [{"crystal": k, "nbands": cases[k].nbands, "mesh": list(map(int, cases[k].modes.ngrid)),
+                                 "processes": int(nproc[j])} for j, k in enumerate(small)],
+           "processes_batched": n_small, "seconds_per_step": wall, "value": n_small / wall, "unit": "processes/s",
+           "graph_ms_per_step": float(np.mean(kms)), "kernels_per_step": nl, "batch_create_seconds": t_create,
+           "sample_rate_max_rel_err_vs_fp64": max(errs),
+           "crystals_counted_only": [{"crystal": k, "nbands": cases[k].nbands,
+                                      "mesh": list(map(int, cases[k].modes.ngrid)), "processes": counts[k][0],
+                                      "count_seconds": counts[k][1]} for k in big]}
+    print(json.dumps(out), flush=True)
 
 
 def run_iterative20(args):

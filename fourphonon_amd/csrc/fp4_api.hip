@@ -2123,6 +2123,135 @@ int fp4_bte_solve(fp4_ctx *c, double V, const double *tau_zero, int32_t maxiter,
   return FP4_OK;
 }
 
+// ---- batches of crystals ---------------------------------------------------
+}  // extern "C"
+
+struct fp4_batch {
+  std::vector<fp4_ctx *> ctx;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DenseBatchState *dense = nullptr;
+  std::vector<std::vector<int>> counts;   // per context: count pass per (channel, mode, q')
+};
+
+extern "C" {
+
+int fp4_batch_destroy(fp4_batch *b) {
+  if (!b) return FP4_OK;
+  (void)hipSetDevice(b->device);
+  dense_batch_destroy(b->dense);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FP4_OK;
+}
+
+int fp4_batch_create(int32_t n, fp4_ctx *const *ctxs, fp4_batch **out) {
+  if (!out || n < 1 || !ctxs) return fail(FP4_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  int rc;
+  for (int i = 0; i < n; i++) {
+    if ((rc = check_ready(ctxs[i], true))) return rc;
+    if (ctxs[i]->device != ctxs[0]->device) return fail(FP4_ERR_ARG, "contexts on different devices");
+  }
+  fp4_batch *b = new fp4_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->device = ctxs[0]->device;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      !(b->dense = dense_batch_create())) {
+    fp4_batch_destroy(b);
+    return fail(FP4_ERR_DEVICE, "batch: stream or event creation failed");
+  }
+  // the prefilter counts order the batch's work queue (they do not depend on T)
+  b->counts.resize(n);
+  for (int i = 0; i < n; i++) {
+    fp4_ctx *c = ctxs[i];
+    if (c->natoms > 4) continue;
+    Plan pl;
+    all_modes(c, pl);
+    if ((rc = count_pass(c, pl, false))) {
+      fp4_batch_destroy(b);
+      return rc;
+    }
+    b->counts[i].resize(pl.items);
+    if (hipMemcpy(b->counts[i].data(), c->cnt.p, sizeof(int) * pl.items, hipMemcpyDeviceToHost) != hipSuccess) {
+      fp4_batch_destroy(b);
+      return fail(FP4_ERR_DEVICE, "batch: copy of the counts failed");
+    }
+  }
+  *out = b;
+  return FP4_OK;
+}
+
+int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double *const *rates, int64_t *nproc,
+                      double *kernel_ms, int32_t *launches) {
+  if (!b || !rates) return fail(FP4_ERR_ARG, "bad arguments");
+  if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP16X2)
+    return fail(FP4_ERR_UNSUPPORTED, "batch: FP4_PREC_FP32 or FP4_PREC_FP16X2 only");
+  HIPCHECK(hipSetDevice(b->device));
+  const int n = (int)b->ctx.size();
+  for (int i = 0; i < n; i++)
+    if (!rates[i]) return fail(FP4_ERR_ARG, "null output of crystal %d", i);
+  std::vector<DenseBatchMember> mem;
+  std::vector<int> who;
+  for (int i = 0; i < n; i++) {
+    fp4_ctx *c = b->ctx[i];
+    if (c->natoms > 4) continue;
+    DenseBatchMember x;
+    x.s = c->dense;
+    x.r.m = mode_view(c);
+    x.r.roots = c->roots.as<double2>();
+    x.r.T = c->T;
+    x.r.onlyharmonic = c->onlyharmonic;
+    x.r.natoms = c->natoms;
+    x.r.precision = precision;
+    x.r.stream = b->stream;
+    x.r.tuning = nullptr;
+    x.modes.resize((size_t)c->nb * c->nlist);
+    std::iota(x.modes.begin(), x.modes.end(), 1);
+    x.ii_lo0 = 0;
+    x.nii = c->nptk;
+    x.counts = b->counts[i];
+    mem.push_back(std::move(x));
+    who.push_back(i);
+  }
+  double ms = 0.0;
+  int nl = 0;
+  if (!mem.empty()) {
+    const int rc = dense_rates_batch(b->dense, mem, precision, b->stream, use_graph != 0, &ms, &nl);
+    if (rc) return fail(rc, "batch: %s", dense_error());
+  }
+  for (size_t k = 0; k < mem.size(); k++) {
+    const int i = who[k];
+    fp4_ctx *c = b->ctx[i];
+    const size_t nm = (size_t)c->nb * c->nlist;
+    Plan pl;
+    all_modes(c, pl);
+    double *o = rates[i];
+    std::fill(o, o + 7 * nm, 0.0);
+    write_rates(c, pl, mem[k].g, mem[k].w, o, o + nm, o + 2 * nm, o + 3 * nm, o + 4 * nm, o + 5 * nm, o + 6 * nm,
+                true);
+    if (nproc) nproc[i] = std::accumulate(mem[k].n.begin(), mem[k].n.end(), 0LL);
+  }
+  // cells beyond the dense kernel's 4 atoms: the FP64 per-process form
+  for (int i = 0; i < n; i++) {
+    fp4_ctx *c = b->ctx[i];
+    if (c->natoms <= 4) continue;
+    const size_t nm = (size_t)c->nb * c->nlist;
+    std::vector<int> modes(nm);
+    std::iota(modes.begin(), modes.end(), 1);
+    double *o = rates[i];
+    std::fill(o, o + 7 * nm, 0.0);
+    int64_t np = 0;
+    int rc = fp4_rta_4ph_modes(c, FP4_PREC_FP64, (int)nm, modes.data(), 1, c->nptk, o, o + nm, o + 2 * nm, o + 3 * nm,
+                               o + 4 * nm, o + 5 * nm, o + 6 * nm, &np);
+    if (rc) return rc;
+    if (nproc) nproc[i] = np;
+  }
+  if (kernel_ms) *kernel_ms = ms;
+  if (launches) *launches = nl;
+  return FP4_OK;
+}
+
 // ---- diagnostics (no counterpart in the reference) -----------------------
 int fp4_ctx_kernel_stats(fp4_ctx *c, int32_t kind, double *total_ms, int64_t *launches) {
   if (!c || kind < 0 || kind >= KS_N) return fail(FP4_ERR_ARG, "bad arguments");

@@ -1197,6 +1197,41 @@ const DenseShape kShapes1[] = {
     FP4_SHAPE(double, FP4_PREC_FP64_DENSE, 12, 1, 2, 0),
 };
 
+template <class T, int NB, int IB, int TQ, int MF>
+void launch_shape_batch(const DenseArgs &a, int slots, size_t dyn, hipStream_t st) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  hipLaunchKernelGGL((k_dense<T, NB, IB, TQ, MF, true>), dim3(slots), dim3(NT_DENSE), dyn, st, a);
+}
+
+template <class T, int NB, int IB, int TQ, int MF>
+int per_cu_shape_batch(size_t dyn) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, reinterpret_cast<const void *>(&k_dense<T, NB, IB, TQ, MF, true>), NT_DENSE, dyn) != hipSuccess)
+    return 1;
+  return std::max(n, 1);
+}
+
+#define FP4_SHAPE_BATCH(T, P, NB, IB, TQ, MF)                                                                    \
+  {P, NB, IB, TQ, MF, 0, &lds_bytes_geom<T, NB, IB, TQ, MF>, &static_bytes<T, NB, IB, TQ>,                           \
+   &launch_shape_batch<T, NB, IB, TQ, MF>, &per_cu_shape_batch<T, NB, IB, TQ, MF>}
+
+// k_dense over several crystals in one launch (the batch workload): FP32
+// MFMA and FP16X2 MFMA shapes of the 2- and 4-atom cells.
+const DenseShape kShapesBatch[] = {
+    FP4_SHAPE_BATCH(float, FP4_PREC_FP32, 6, 6, 8, 1),   FP4_SHAPE_BATCH(float, FP4_PREC_FP32, 6, 2, 8, 1),
+    FP4_SHAPE_BATCH(float, FP4_PREC_FP32, 12, 1, 8, 1),  FP4_SHAPE_BATCH(float, FP4_PREC_FP16X2, 6, 6, 8, 4),
+    FP4_SHAPE_BATCH(float, FP4_PREC_FP16X2, 12, 1, 8, 4),
+};
+
+__global__ void k_zero_int(int *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0;
+}
+
 // All shapes in order of preference; the first whose LDS fits is used when
 // no tuning is given: k_dense2 with every branch of a point in one block
 // (IB = Nb) first, then k_dense, then the remaining k_dense2 shapes (fewer
@@ -1244,89 +1279,105 @@ int ensure(void *&p, size_t &bytes, size_t need) {
 
 }  // namespace
 
-int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
-                std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms) {
-  if (!s) return derr(FP4_ERR_STATE, "dense state missing");
-  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3 &&
-      r.precision != FP4_PREC_FP16X3 && r.precision != FP4_PREC_FP16X2)
-    return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
+// One crystal's launch arguments and the buffers its results go to.
+struct DensePrep {
+  DenseArgs a{};
+  const DenseShape *sh = nullptr;
+  DenseGeom geo{};
+  std::vector<int> lls;
+  int nls = 0, nib = 0, nii = 0;
+  long nitems = 0, nseg = 0;
+  double *og = nullptr, *ow = nullptr, *sg = nullptr, *sw = nullptr;
+  int *on = nullptr, *ot = nullptr;
+  long long *sn = nullptr;
+};
+
+namespace {
+
+// Chooses the kernel shape of one call (tuned, else the first that fits).
+const DenseShape *choose_shape(DenseState *s, const DenseRun &r, size_t ldsmax) {
   const int nb = r.m.nb;
-  int dev = 0, ldsmax = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (s->natoms > 4) return derr(FP4_ERR_UNSUPPORTED, "dense form: more than 4 atoms per cell");
   const bool tuned = r.tuning && r.tuning->shape_prec == r.precision;
-  const DenseShape *shp = nullptr;
   const DenseGeom geo = geom_of(s);
   const std::vector<DenseShape> &shapes = all_shapes();
   if (tuned && r.tuning->shape >= 0 && r.tuning->shape < (int)shapes.size()) {
     const DenseShape &c = shapes[r.tuning->shape];
-    if (c.prec == r.precision && c.nb == nb && c.bytes(geo) <= (size_t)ldsmax) shp = &c;
+    if (c.prec == r.precision && c.nb == nb && c.bytes(geo) <= ldsmax) return &c;
   }
-  if (!shp) shp = pick_shape(r.precision, nb, geo, (size_t)ldsmax);
-  if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
-  const DenseShape &sh = *shp;
+  return pick_shape(r.precision, nb, geo, ldsmax);
+}
+
+// Everything of a dense call up to the launch: mode selection, output
+// buffers, uploads, the Bose-Einstein table and the kernel arguments (all
+// on stream r.stream).
+int dense_prepare(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
+                  const DenseShape &sh, DensePrep &p) {
+  const int nb = r.m.nb;
+  p.sh = &sh;
+  p.geo = geom_of(s);
+  const DenseGeom &geo = p.geo;
   s->last_mfma = sh.mfma;
   s->last_tq = sh.TQ;
   s->last_ib = sh.IB;
   s->last_kg = sh.kg;
   const int nmodes_all = nb * r.m.nlist;
   std::vector<unsigned char> sel(nmodes_all, 0);
-  std::vector<int> lls;
+  p.lls.clear();
   for (int mm : modes) {
     sel[mm - 1] = 1;
     const int ll = (mm - 1) / nb;
-    if (lls.empty() || lls.back() != ll) lls.push_back(ll);
+    if (p.lls.empty() || p.lls.back() != ll) p.lls.push_back(ll);
   }
-  const int nls = (int)lls.size(), nib = nb / sh.IB;
+  const int nls = (int)p.lls.size(), nib = nb / sh.IB;
   const long nitems = 3L * nls * nib * nii;
-  int ncu = 0;
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  int slots = (tuned && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu);
-  if (slots > nitems) slots = (int)std::max(1L, nitems);
+  p.nls = nls;
+  p.nib = nib;
+  p.nii = nii;
+  p.nitems = nitems;
   int rc;
   const size_t per_item = sizeof(double) * 2 + sizeof(int);
   const long nseg = 3L * nls * nib * sh.IB;
+  p.nseg = nseg;
   // outputs: og, ow, on per item*IB, then sg, sw, sn per segment, then lls and sel
   const size_t need = per_item * nitems * sh.IB + sizeof(int) * nitems + (sizeof(double) * 2 + sizeof(long long)) * nseg +
                       sizeof(int) * (nls + s->ng) + nmodes_all + 1024;
   if ((rc = ensure(s->outb, s->out_bytes, need))) return rc;
-  char *p = reinterpret_cast<char *>(s->outb);
-  double *og = reinterpret_cast<double *>(p);
-  p += sizeof(double) * nitems * sh.IB;
-  double *ow = reinterpret_cast<double *>(p);
-  p += sizeof(double) * nitems * sh.IB;
-  double *sg = reinterpret_cast<double *>(p);
-  p += sizeof(double) * nseg;
-  double *sw = reinterpret_cast<double *>(p);
-  p += sizeof(double) * nseg;
-  long long *sn = reinterpret_cast<long long *>(p);
-  p += sizeof(long long) * nseg;
-  int *on = reinterpret_cast<int *>(p);
-  p += sizeof(int) * nitems * sh.IB;
-  int *ot = reinterpret_cast<int *>(p);
-  p += sizeof(int) * nitems;
-  int *dlls = reinterpret_cast<int *>(p);
-  p += sizeof(int) * nls;
-  int *dgslot = reinterpret_cast<int *>(p);
-  p += sizeof(int) * s->ng;
-  unsigned char *dsel = reinterpret_cast<unsigned char *>(p);
+  char *q = reinterpret_cast<char *>(s->outb);
+  p.og = reinterpret_cast<double *>(q);
+  q += sizeof(double) * nitems * sh.IB;
+  p.ow = reinterpret_cast<double *>(q);
+  q += sizeof(double) * nitems * sh.IB;
+  p.sg = reinterpret_cast<double *>(q);
+  q += sizeof(double) * nseg;
+  p.sw = reinterpret_cast<double *>(q);
+  q += sizeof(double) * nseg;
+  p.sn = reinterpret_cast<long long *>(q);
+  q += sizeof(long long) * nseg;
+  p.on = reinterpret_cast<int *>(q);
+  q += sizeof(int) * nitems * sh.IB;
+  p.ot = reinterpret_cast<int *>(q);
+  q += sizeof(int) * nitems;
+  int *dlls = reinterpret_cast<int *>(q);
+  q += sizeof(int) * nls;
+  int *dgslot = reinterpret_cast<int *>(q);
+  q += sizeof(int) * s->ng;
+  unsigned char *dsel = reinterpret_cast<unsigned char *>(q);
   // k_dense2: phase-table slots of the coarse groups for this k-step shape
   int pair_off[17] = {};
   std::vector<int> gslot(std::max(s->ng, 1), 0);
   if (sh.kg > 0) dense2_slots(geo, sh.kg, pair_off, gslot.data());
   hipStream_t st = r.stream;
-  if (hipMemcpyAsync(dlls, lls.data(), sizeof(int) * nls, hipMemcpyHostToDevice, st) != hipSuccess ||
+  if (hipMemcpyAsync(dlls, p.lls.data(), sizeof(int) * nls, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(dgslot, gslot.data(), sizeof(int) * s->ng, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(dsel, sel.data(), nmodes_all, hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemsetAsync(s->counter, 0, sizeof(int), st) != hipSuccess)
+      hipMemcpyAsync(dsel, sel.data(), nmodes_all, hipMemcpyHostToDevice, st) != hipSuccess)
     return derr(FP4_ERR_DEVICE, "dense: upload failed");
   const long nmt = (long)nb * r.m.nptk;
   if ((rc = ensure(s->fbose, s->fbose_bytes, sizeof(double) * nmt))) return rc;
   hipLaunchKernelGGL(k_bose, dim3((unsigned)((nmt + 255) / 256)), dim3(256), 0, st, r.m.energy, nmt, r.T,
                      reinterpret_cast<double *>(s->fbose));
   if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense: Bose-Einstein table launch failed");
-  DenseArgs a{};
+  DenseArgs &a = p.a;
+  a = DenseArgs{};
   a.m = r.m;
   a.roots = r.roots;
   a.L = s->L;
@@ -1334,7 +1385,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.f = s->f;
   a.c = s->c;
   a.phi = s->phi;
-  for (int q = 0; q < 17; q++) a.klf[q] = s->klf[q];
+  for (int k = 0; k < 17; k++) a.klf[k] = s->klf[k];
   a.roots_lds = s->L <= kRootsLds;
   a.fbose = reinterpret_cast<const double *>(s->fbose);
   a.ng = s->ng;
@@ -1357,7 +1408,7 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
     a.sbf = (float)std::fabs(r.m.scalebroad);
     a.a_scale = s->a_scale;
   }
-  for (int q = 0; q < 17; q++) a.pair_off[q] = pair_off[q];
+  for (int k = 0; k < 17; k++) a.pair_off[k] = pair_off[k];
   a.gpt = sh.kg > 0 ? dense2_gpt(geo, sh.kg) : 0;
   a.gslot = dgslot;
   a.lmask = (s->L > 0 && (s->L & (s->L - 1)) == 0) ? s->L - 1 : -1;
@@ -1371,26 +1422,34 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   a.wop = r.wop;
   a.wcols = r.wcols;
   a.wscale = (double)kGammaUnits * 1.e9 / r.m.nptk / r.m.nptk;
-  a.out_g = og;
-  a.out_w = ow;
-  a.out_n = on;
-  a.out_t = ot;
-  if (hipEventRecord(s->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  sh.launch(a, slots, sh.bytes(geo), st);
-  if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense kernel launch failed");
-  if (hipEventRecord(s->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  hipLaunchKernelGGL(k_dense_reduce, dim3((unsigned)((nseg + 127) / 128)), dim3(128), 0, st, og, ow, on, nseg, sh.IB,
-                     nii, sg, sw, sn);
+  a.out_g = p.og;
+  a.out_w = p.ow;
+  a.out_n = p.on;
+  a.out_t = p.ot;
+  return 0;
+}
+
+// The per-segment reduction after the launch (queued on st).
+void dense_reduce_launch(const DensePrep &p, hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_reduce, dim3((unsigned)((p.nseg + 127) / 128)), dim3(128), 0, st, p.og, p.ow, p.on,
+                     p.nseg, p.sh->IB, p.nii, p.sg, p.sw, p.sn);
+}
+
+// Copies one crystal's segment sums back and maps them to (channel, mode).
+int dense_collect(DenseState *s, const DensePrep &p, int nb, const std::vector<int> &modes, hipStream_t st,
+                  std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n) {
+  const long nseg = p.nseg, nitems = p.nitems;
+  const int nls = p.nls, nib = p.nib, IB = p.sh->IB;
   std::vector<double> hg(nseg), hw(nseg);
   std::vector<long long> hn(nseg);
-  if (hipMemcpyAsync(hg.data(), sg, sizeof(double) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(hw.data(), sw, sizeof(double) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(hn.data(), sn, sizeof(long long) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(hg.data(), p.sg, sizeof(double) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(hw.data(), p.sw, sizeof(double) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(hn.data(), p.sn, sizeof(long long) * nseg, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return derr(FP4_ERR_DEVICE, "dense: kernel or copy failed");
   {
     std::vector<int> ht(nitems);
-    if (hipMemcpy(ht.data(), ot, sizeof(int) * nitems, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(ht.data(), p.ot, sizeof(int) * nitems, hipMemcpyDeviceToHost) != hipSuccess)
       return derr(FP4_ERR_DEVICE, "dense: copy failed");
     long long tt = 0;
     for (int x : ht) tt += x;
@@ -1402,9 +1461,6 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
       return derr(FP4_ERR_DEVICE, "dense: copy failed");
 #endif
   }
-  float kms = 0.f;
-  (void)hipEventElapsedTime(&kms, s->e0, s->e1);
-  if (ms) *ms = kms;
   // segments are (ch, ls, ib, il); map to (ch, mode-in-subset)
   const int nsub = (int)modes.size();
   g.assign(3 * nsub, 0.0);
@@ -1415,15 +1471,298 @@ int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes,
   for (int ch = 0; ch < 3; ch++)
     for (int ls = 0; ls < nls; ls++)
       for (int ib = 0; ib < nib; ib++)
-        for (int il = 0; il < sh.IB; il++) {
-          const int mm = lls[ls] * nb + ib * sh.IB + il + 1;
+        for (int il = 0; il < IB; il++) {
+          const int mm = p.lls[ls] * nb + ib * IB + il + 1;
           auto it = mpos.find(mm);
           if (it == mpos.end()) continue;
-          const long sidx = (((long)ch * nls + ls) * nib + ib) * sh.IB + il;
+          const long sidx = (((long)ch * nls + ls) * nib + ib) * IB + il;
           g[ch * nsub + it->second] = hg[sidx];
           w[ch * nsub + it->second] = hw[sidx];
           n[ch * nsub + it->second] = hn[sidx];
         }
+  return 0;
+}
+
+}  // namespace
+
+int dense_rates(DenseState *s, const DenseRun &r, const std::vector<int> &modes, int ii_lo0, int nii,
+                std::vector<double> &g, std::vector<double> &w, std::vector<long long> &n, double *ms) {
+  if (!s) return derr(FP4_ERR_STATE, "dense state missing");
+  if (r.precision != FP4_PREC_FP32 && r.precision != FP4_PREC_FP64_DENSE && r.precision != FP4_PREC_BF16X3 &&
+      r.precision != FP4_PREC_FP16X3 && r.precision != FP4_PREC_FP16X2)
+    return derr(FP4_ERR_UNSUPPORTED, "dense form: precision not built");
+  int dev = 0, ldsmax = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (s->natoms > 4) return derr(FP4_ERR_UNSUPPORTED, "dense form: more than 4 atoms per cell");
+  const DenseShape *shp = choose_shape(s, r, (size_t)ldsmax);
+  if (!shp) return derr(FP4_ERR_UNSUPPORTED, "dense form: no kernel shape fits this Nbands / cluster set in LDS");
+  const bool tuned = r.tuning && r.tuning->shape_prec == r.precision;
+  DensePrep p;
+  int rc;
+  if ((rc = dense_prepare(s, r, modes, ii_lo0, nii, *shp, p))) return rc;
+  int slots = (tuned && r.tuning->slots > 0) ? r.tuning->slots : std::max(1, ncu);
+  if (slots > p.nitems) slots = (int)std::max(1L, p.nitems);
+  hipStream_t st = r.stream;
+  if (hipMemsetAsync(s->counter, 0, sizeof(int), st) != hipSuccess) return derr(FP4_ERR_DEVICE, "dense: memset failed");
+  if (hipEventRecord(s->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
+  shp->launch(p.a, slots, shp->bytes(p.geo), st);
+  if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "dense kernel launch failed");
+  if (hipEventRecord(s->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
+  dense_reduce_launch(p, st);
+  if ((rc = dense_collect(s, p, r.m.nb, modes, st, g, w, n))) return rc;
+  float kms = 0.f;
+  (void)hipEventElapsedTime(&kms, s->e0, s->e1);
+  if (ms) *ms = kms;
+  return 0;
+}
+
+// ------------------------------------------------------------ batches
+struct DenseBatchState {
+  struct Class {
+    int nb = 0;
+    const DenseShape *sh = nullptr;
+    DenseArgs ka{};
+    size_t dyn = 0;
+    int slots = 0;
+    std::vector<int> members;
+    DenseArgs *d_args = nullptr;
+    long long *d_order = nullptr;
+    size_t args_cap = 0, order_cap = 0;
+    std::vector<DenseArgs> h_args;       // as uploaded (graph replays are valid while unchanged)
+    std::vector<long long> h_order;
+  };
+  std::vector<Class> classes;
+  int *counters = nullptr;               // one queue counter per class
+  int ncounters = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_prec = -1;
+  bool graph_valid = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+DenseBatchState *dense_batch_create() {
+  DenseBatchState *b = new DenseBatchState();
+  if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+    dense_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void dense_batch_destroy(DenseBatchState *b) {
+  if (!b) return;
+  for (auto &c : b->classes) {
+    if (c.d_args) (void)hipFree(c.d_args);
+    if (c.d_order) (void)hipFree(c.d_order);
+  }
+  if (b->counters) (void)hipFree(b->counters);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  delete b;
+}
+
+int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, int precision, hipStream_t st,
+                      bool use_graph, double *kernel_ms, int *launches) {
+  if (!b) return derr(FP4_ERR_STATE, "batch state missing");
+  if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP16X2)
+    return derr(FP4_ERR_UNSUPPORTED, "batch: FP32 or FP16X2 MFMA forms only");
+  int dev = 0, ldsmax = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ldsmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  // classes by Nbands, each with the first batch shape that fits the
+  // largest group / cluster / root counts of its crystals
+  std::vector<DenseBatchState::Class> cls;
+  for (int i = 0; i < (int)mem.size(); i++) {
+    DenseBatchMember &x = mem[i];
+    if (!x.s || x.s->natoms > 4)
+      return derr(FP4_ERR_UNSUPPORTED, ("batch: crystal " + std::to_string(i) + " has more than 4 atoms").c_str());
+    const int nb = x.r.m.nb;
+    auto it = std::find_if(cls.begin(), cls.end(), [&](const DenseBatchState::Class &c) { return c.nb == nb; });
+    if (it == cls.end()) {
+      cls.emplace_back();
+      cls.back().nb = nb;
+      it = cls.end() - 1;
+    }
+    it->members.push_back(i);
+  }
+  for (auto &c : cls) {
+    DenseGeom gmax{};
+    for (int i : c.members) {
+      const DenseGeom g = geom_of(mem[i].s);
+      gmax.ng = std::max(gmax.ng, g.ng);
+      gmax.ntri = std::max(gmax.ntri, g.ntri);
+      gmax.L = std::max(gmax.L, g.L);
+    }
+    // the roots go to LDS only when every crystal's table fits
+    const bool all_lds = gmax.L <= kRootsLds;
+    for (const DenseShape &sh : kShapesBatch)
+      if (sh.prec == precision && sh.nb == c.nb && sh.bytes(gmax) <= (size_t)ldsmax) {
+        c.sh = &sh;
+        break;
+      }
+    if (!c.sh)
+      return derr(FP4_ERR_UNSUPPORTED,
+                  ("batch: no batch shape of Nbands=" + std::to_string(c.nb) + " fits in LDS").c_str());
+    c.dyn = c.sh->bytes(gmax);
+    c.ka = DenseArgs{};
+    c.ka.ng = gmax.ng;
+    c.ka.ntri = gmax.ntri;
+    c.ka.L = gmax.L;
+    c.ka.roots_lds = all_lds ? 1 : 0;
+    c.ka.smem_static = (int)c.sh->stat();
+    c.slots = std::max(1, ncu) * std::min(c.sh->per_cu(c.dyn), 2);
+  }
+  // per crystal: uploads, Bose-Einstein table and arguments (outside the graph)
+  std::vector<DensePrep> prep(mem.size());
+  int rc;
+  for (auto &c : cls)
+    for (int i : c.members) {
+      DenseBatchMember &x = mem[i];
+      DenseRun r = x.r;
+      r.stream = st;
+      r.precision = precision;
+      if ((rc = dense_prepare(x.s, r, x.modes, x.ii_lo0, x.nii, *c.sh, prep[i]))) return rc;
+      prep[i].a.roots_lds = c.ka.roots_lds;
+    }
+  // device argument tables and the largest-first work order
+  bool changed = !b->graph_valid || b->graph_prec != precision || b->classes.size() != cls.size();
+  if ((int)cls.size() > b->ncounters) {
+    if (b->counters) (void)hipFree(b->counters);
+    b->counters = nullptr;
+    if (hipMalloc(&b->counters, sizeof(int) * cls.size()) != hipSuccess) return derr(FP4_ERR_NOMEM, "batch: hipMalloc failed");
+    b->ncounters = (int)cls.size();
+    changed = true;
+  }
+  for (size_t k = 0; k < cls.size(); k++) {
+    auto &c = cls[k];
+    DenseBatchState::Class *old = (k < b->classes.size()) ? &b->classes[k] : nullptr;
+    c.h_args.clear();
+    for (int i : c.members) c.h_args.push_back(prep[i].a);
+    // cost of an item (channel, point, mode block, q'): its processes (the
+    // prefilter counts of its modes at that q') plus the filter work every
+    // candidate q'' costs, about one process per q'' and mode block
+    std::vector<std::pair<double, long long>> items;
+    for (size_t mi = 0; mi < c.members.size(); mi++) {
+      const int i = c.members[mi];
+      const DenseBatchMember &x = mem[i];
+      const DensePrep &p = prep[i];
+      const int nsub = (int)x.modes.size(), IB = c.sh->IB;
+      const bool have = x.counts.size() == (size_t)3 * nsub * x.nii;
+      std::map<int, int> mpos;
+      for (int t = 0; t < nsub; t++) mpos[x.modes[t]] = t;
+      for (long t = 0; t < p.nitems; t++) {
+        const int ii = (int)(t % p.nii);
+        long r_ = t / p.nii;
+        const int ib = (int)(r_ % p.nib);
+        r_ /= p.nib;
+        const int ls = (int)(r_ % p.nls), ch = (int)(r_ / p.nls);
+        double cost = (double)x.r.m.nptk;
+        if (have)
+          for (int il = 0; il < IB; il++) {
+            auto it = mpos.find(p.lls[ls] * c.nb + ib * IB + il + 1);
+            if (it != mpos.end()) cost += (double)x.counts[((size_t)ch * nsub + it->second) * x.nii + ii];
+          }
+        items.push_back({cost, ((long long)mi << 40) | (long long)t});
+      }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const auto &u, const auto &v) { return u.first > v.first; });
+    c.h_order.resize(items.size());
+    for (size_t t = 0; t < items.size(); t++) c.h_order[t] = items[t].second;
+    // keep the device tables of the previous call when they fit
+    if (old) {
+      c.d_args = old->d_args;
+      c.d_order = old->d_order;
+      c.args_cap = old->args_cap;
+      c.order_cap = old->order_cap;
+      old->d_args = nullptr;
+      old->d_order = nullptr;
+    }
+    if (c.h_args.size() > c.args_cap) {
+      if (c.d_args) (void)hipFree(c.d_args);
+      c.d_args = nullptr;
+      if (hipMalloc(&c.d_args, sizeof(DenseArgs) * c.h_args.size()) != hipSuccess)
+        return derr(FP4_ERR_NOMEM, "batch: hipMalloc failed");
+      c.args_cap = c.h_args.size();
+      changed = true;
+    }
+    if (c.h_order.size() > c.order_cap) {
+      if (c.d_order) (void)hipFree(c.d_order);
+      c.d_order = nullptr;
+      if (hipMalloc(&c.d_order, sizeof(long long) * c.h_order.size()) != hipSuccess)
+        return derr(FP4_ERR_NOMEM, "batch: hipMalloc failed");
+      c.order_cap = c.h_order.size();
+      changed = true;
+    }
+    if (!old || old->h_args.size() != c.h_args.size() ||
+        std::memcmp(old->h_args.data(), c.h_args.data(), sizeof(DenseArgs) * c.h_args.size()) != 0 ||
+        old->h_order != c.h_order || old->sh != c.sh)
+      changed = true;
+    if (hipMemcpyAsync(c.d_args, c.h_args.data(), sizeof(DenseArgs) * c.h_args.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(c.d_order, c.h_order.data(), sizeof(long long) * c.h_order.size(), hipMemcpyHostToDevice,
+                       st) != hipSuccess)
+      return derr(FP4_ERR_DEVICE, "batch: upload failed");
+    c.ka.batch = c.d_args;
+    c.ka.order = c.d_order;
+    c.ka.nbatch = (int)c.members.size();
+    c.ka.nitems = (long)c.h_order.size();
+    c.ka.counter = b->counters + k;
+  }
+  for (auto &c : b->classes) {
+    if (c.d_args) (void)hipFree(c.d_args);
+    if (c.d_order) (void)hipFree(c.d_order);
+  }
+  b->classes = std::move(cls);
+  // the device work of a step: queue counters, one persistent launch per
+  // class, the per-crystal reductions
+  auto enqueue = [&](hipStream_t s2) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s2, b->counters, (int)b->classes.size());
+    for (auto &c : b->classes)
+      if (c.ka.nitems > 0) c.sh->launch(c.ka, (int)std::min<long>(c.slots, c.ka.nitems), c.dyn, s2);
+    for (auto &c : b->classes)
+      for (int i : c.members) dense_reduce_launch(prep[i], s2);
+  };
+  int nl = 1;
+  for (auto &c : b->classes) nl += (c.ka.nitems > 0 ? 1 : 0) + (int)c.members.size();
+  if (hipEventRecord(b->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
+  if (use_graph) {
+    if (changed) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      b->graph_valid = false;
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return derr(FP4_ERR_DEVICE, "batch: stream capture failed");
+      enqueue(st);
+      if (hipStreamEndCapture(st, &b->graph) != hipSuccess ||
+          hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+        return derr(FP4_ERR_DEVICE, "batch: graph capture failed");
+      b->graph_valid = true;
+      b->graph_prec = precision;
+    }
+    if (hipGraphLaunch(b->exec, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "batch: graph launch failed");
+  } else {
+    enqueue(st);
+    b->graph_valid = false;
+  }
+  if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "batch: launch failed");
+  if (hipEventRecord(b->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
+  for (auto &c : b->classes)
+    for (int i : c.members) {
+      DenseBatchMember &x = mem[i];
+      if ((rc = dense_collect(x.s, prep[i], x.r.m.nb, x.modes, st, x.g, x.w, x.n))) return rc;
+    }
+  float kms = 0.f;
+  (void)hipEventElapsedTime(&kms, b->e0, b->e1);
+  if (kernel_ms) *kernel_ms = kms;
+  if (launches) *launches = nl;
   return 0;
 }
 

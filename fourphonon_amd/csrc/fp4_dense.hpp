@@ -91,6 +91,28 @@ int dense_candidates(const DenseState *s, int cap, int *kg, int *ib, int *tq, in
 // per CU); FP4_ERR_UNSUPPORTED when no such shape is built or fits in LDS.
 int dense_pick(const DenseState *s, int precision, int nb, int kg, int ib, int slots, DenseTuning *out);
 
+// Several crystals' dense rates in one persistent launch per Nbands class
+// (FP32 or FP16X2 MFMA; cells of at most 4 atoms): every crystal's work items
+// go into one queue, costliest first by the prefilter counts (counts:
+// processes per (channel, mode-in-subset, q') in the count pass's layout, or
+// empty), and
+// with use_graph the device work of a call (queue counters, the class
+// launches, the per-crystal reductions) is a captured HIP graph, replayed
+// while the crystals' arguments are unchanged.
+struct DenseBatchMember {
+  DenseState *s = nullptr;
+  DenseRun r;
+  std::vector<int> modes;
+  int ii_lo0 = 0, nii = 0;
+  std::vector<int> counts;
+  std::vector<double> g, w;              // results, as dense_rates
+  std::vector<long long> n;
+};
+struct DenseBatchState;
+DenseBatchState *dense_batch_create();
+void dense_batch_destroy(DenseBatchState *b);
+int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, int precision, hipStream_t st,
+                      bool use_graph, double *kernel_ms, int *launches);
 const char *dense_error();
 
 }  // namespace fp4

@@ -46,6 +46,7 @@ EXPORTS = [
     "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
     "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense", "fp4_bte_assemble_operator",
     "fp4_bte_operator_begin", "fp4_bte_operator_add", "fp4_bte_operator_end",
+    "fp4_batch_create", "fp4_batch_destroy", "fp4_batch_rta_4ph",
 ]
 
 
@@ -596,3 +597,36 @@ class Context:
 
     def set_batch_capacity(self, processes):
         _check(load().fp4_ctx_set_batch_capacity(self.h, C.c_int64(processes)))
+
+
+class Batch:
+    """Several crystals' RTA_driver_4ph in one persistent launch per Nbands class (fp4_batch_*)."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        h = C.c_void_p()
+        _check(load().fp4_batch_create(C.c_int32(len(self.contexts)), arr, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().fp4_batch_destroy(self.h)
+            self.h = None
+
+    def RTA_driver_4ph(self, precision, use_graph=True):
+        """-> (list of per-crystal dicts of (Nb, Nlist) arrays, nproc per crystal, kernel ms, launches)."""
+        outs = [np.zeros(7 * c.nb * c.nlist) for c in self.contexts]
+        ptrs = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+        nproc = np.zeros(len(outs), np.int64)
+        ms = C.c_double()
+        nl = C.c_int32()
+        _check(load().fp4_batch_rta_4ph(self.h, C.c_int32(precision), C.c_int32(int(bool(use_graph))), ptrs,
+                                        _p(nproc), C.byref(ms), C.byref(nl)))
+        keys = ("rate_scatt_4ph", "rate_scatt_plusplus", "rate_scatt_plusminus", "rate_scatt_minusminus",
+                "WP4_plusplus", "WP4_plusminus", "WP4_minusminus")
+        res = []
+        for c, o in zip(self.contexts, outs):
+            nm = c.nb * c.nlist
+            res.append({k: o[t * nm:(t + 1) * nm].reshape((c.nb, c.nlist), order="F") for t, k in enumerate(keys)})
+        return res, nproc, ms.value, nl.value

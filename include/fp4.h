@@ -402,6 +402,30 @@ int fp4_bte_operator_add(fp4_ctx *ctx, int32_t precision, int32_t with_3ph, int3
                          double *rate_4ph);
 int fp4_bte_operator_end(fp4_ctx *ctx);
 
+/* ---- batches of crystals ---------------------------------------------------
+   RTA_driver_4ph (Src/processes.f90:5174-5311) for many small crystals at
+   once.  No counterpart in the reference, which runs one crystal per job
+   (its nearest analogue is the gang loop over one crystal's modes,
+   Src/processes.f90:4202-4243).  The contexts (one per crystal, modes and
+   IFCs set, one device) are taken as they are; fp4_batch_create runs each
+   crystal's count pass once, to order the work.  fp4_batch_rta_4ph then runs
+   the dense form of every crystal of at most 4 atoms in one persistent
+   launch per Nbands class, over a single queue of all their work items
+   (channel, point, mode block, q') ordered costliest first by the prefilter
+   counts; with use_graph that device work is a captured HIP graph, replayed
+   while the crystals' data and T are unchanged.  precision is
+   FP4_PREC_FP32 or FP4_PREC_FP16X2 (MFMA).  Crystals of more than 4 atoms
+   run the FP64 per-process form, one after the other.  rates[c] receives
+   7 arrays of (Nb,Nlist) doubles back to back (rate_4ph, rate_pp, rate_pm,
+   rate_mm, WP4_pp, WP4_pm, WP4_mm) for crystal c, nproc[c] (may be NULL)
+   its processes; *kernel_ms the device time of the batched launches and
+   *launches the number of kernels in them. */
+typedef struct fp4_batch fp4_batch;
+int fp4_batch_create(int32_t n, fp4_ctx *const *ctxs, fp4_batch **out);
+int fp4_batch_destroy(fp4_batch *b);
+int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double *const *rates, int64_t *nproc,
+                      double *kernel_ms, int32_t *launches);
+
 /* One iteration_4ph sweep with the operator of the last
    fp4_bte_set_operator / fp4_iteration_4ph call; F_n updated in place. */
 int fp4_bte_iterate(fp4_ctx *ctx, const double *tau_zero, double *F_n);

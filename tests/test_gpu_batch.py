@@ -112,3 +112,49 @@ def test_batch_golden(c):
                 assert abs(got - want) <= 1e-3 * abs(want), (prec, got, want)
     finally:
         ctx.close()
+
+
+# the 2- and 4-atom crystals of the batch (Nb 6 and 12), at small meshes
+SMALL = [(c, (5, 5, 5) if crystal.batch_crystal(c)["natoms"] == 2 else (4, 4, 4))
+         for c in range(crystal.BATCH_SIZE) if crystal.batch_crystal(c)["natoms"] <= 4]
+
+
+def test_batch_one_launch_per_class_matches_each_crystal():
+    """All 2- and 4-atom crystals through fp4_batch_rta_4ph (one persistent
+    launch per Nbands class over one cost-ordered queue, in a HIP graph)
+    against each crystal's own dense call in the same form (the same kernel
+    shape and per-item sums, so the rates agree to rounding of nothing:
+    <= 1e-12), against the FP64 per-process form (FP32 MFMA: 1e-3), and
+    graph replays, with other device work between them, repeat the first
+    result bit for bit."""
+    banner(f"batch of {len(SMALL)} crystals (Nb 6 and 12) in one graph")
+    ctxs = [rt.Context(crystal.batch_case(c, ngrid=mesh)) for c, mesh in SMALL]
+    b = rt.Batch(ctxs)
+    try:
+        first, nproc, ms, nl = b.RTA_driver_4ph(rt.PREC_FP32)
+        print(f"batch: {int(nproc.sum())} processes, {ms:.1f} ms, {nl} kernels", flush=True)
+        assert nl == 1 + 2 + len(SMALL)      # counters, one launch per class, one reduction per crystal
+        for k, ((c, mesh), ctx) in enumerate(zip(SMALL, ctxs)):
+            own, n_own = ctx.RTA_driver_4ph_modes(np.arange(1, ctx.nmodes + 1), rt.PREC_FP32)
+            assert int(nproc[k]) == int(n_own), c
+            for key in ("rate_scatt_4ph", "rate_scatt_plusminus", "WP4_plusplus"):
+                assert_close(first[k][key], own[key], 1e-12, f"crystal {c} {key}")
+            r64, _ = ctx.RTA_driver_4ph_modes(np.arange(1, ctx.nmodes + 1), rt.PREC_FP64)
+            assert_close(first[k]["rate_scatt_4ph"], r64["rate_scatt_4ph"], 1e-3, f"crystal {c} vs FP64")
+        for rep in range(3):
+            ctxs[0].NP_driver_4ph()           # other device work between replays
+            again, nproc2, _, _ = b.RTA_driver_4ph(rt.PREC_FP32)
+            assert np.array_equal(nproc2, nproc)
+            for k in range(len(SMALL)):
+                assert np.array_equal(again[k]["rate_scatt_4ph"], first[k]["rate_scatt_4ph"]), (rep, k)
+        # a changed temperature re-captures the graph
+        for ctx in ctxs:
+            ctx.set_temperature(500.0)
+        hot, _, _, _ = b.RTA_driver_4ph(rt.PREC_FP32)
+        own, _ = ctxs[1].RTA_driver_4ph_modes(np.arange(1, ctxs[1].nmodes + 1), rt.PREC_FP32)
+        assert_close(hot[1]["rate_scatt_4ph"], own["rate_scatt_4ph"], 1e-12, "after T change")
+        assert not np.array_equal(hot[1]["rate_scatt_4ph"], first[1]["rate_scatt_4ph"])
+    finally:
+        b.close()
+        for ctx in ctxs:
+            ctx.close()
