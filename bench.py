@@ -544,8 +544,9 @@ def run_batch24(args):
     6 and 8 atoms, meshes 6^3 to 14^3 as drawn): the RTA pass of every 2- and
     4-atom crystal (Nb 6 and 12) through fp4_batch_rta_4ph -- one persistent
     FP32-MFMA launch per Nbands class over one queue of all their work items,
-    costliest first by the prefilter counts, in a captured HIP graph -- timed
-    over --steps calls; each crystal's FP64 per-process rates on a sample
+    costliest first by the prefilter counts, in a captured HIP graph; a step
+    is 16 such graphs, one per q' sixteenth (progress shows between them) --
+    timed over --steps steps; each crystal's FP64 per-process rates on a sample
     (three modes, q' = 1..4) give the per-mode error.  The 6- and 8-atom
     crystals (Nb 18 and 24), which the dense kernel does not take, are
     counted only.  Prints one JSON line."""
@@ -568,15 +569,30 @@ def run_batch24(args):
     b = rt.Batch([ctxs[k] for k in small])
     t_create = time.perf_counter() - t1
     prec = rt.PREC_FP32
+    nparts = 16    # q' windows per step, one graph each, so that progress shows between them
+
+    def step(tag):
+        tot, np_tot, ms_tot = None, 0, 0.0
+        for part in range(nparts):
+            res, nproc, ms, nl = b.RTA_driver_4ph(prec, part=part, nparts=nparts)
+            if tot is None:
+                tot = [{k: v.copy() for k, v in r.items()} for r in res]
+            else:
+                for t_, r in zip(tot, res):
+                    for k in r:
+                        t_[k] += r[k]
+            np_tot = np_tot + nproc
+            ms_tot += ms
+            log(f"{tag} part {part + 1}/{nparts}: {int(nproc.sum())} processes, graph {ms:.1f} ms, {nl} kernels")
+        return tot, np_tot, ms_tot, nl
+
     for w in range(args.warmup):
-        b.RTA_driver_4ph(prec)
-        log(f"warmup {w + 1}/{args.warmup} done")
+        step(f"warmup {w + 1}/{args.warmup}")
     t1 = time.perf_counter()
     kms = []
     for st in range(args.steps):
-        res, nproc, ms, nl = b.RTA_driver_4ph(prec)
+        res, nproc, ms, nl = step(f"step {st + 1}/{args.steps}")
         kms.append(ms)
-        log(f"step {st + 1}/{args.steps} done: {int(nproc.sum())} processes, graph {ms:.1f} ms, {nl} kernels")
     wall = (time.perf_counter() - t1) / args.steps
     errs = []
     for j, k in enumerate(small):
@@ -600,7 +616,8 @@ def run_batch24(args):
            "crystals_batched": [{"crystal": k, "nbands": cases[k].nbands, "mesh": list(map(int, cases[k].modes.ngrid)),
                                  "processes": int(nproc[j])} for j, k in enumerate(small)],
            "processes_batched": n_small, "seconds_per_step": wall, "value": n_small / wall, "unit": "processes/s",
-           "graph_ms_per_step": float(np.mean(kms)), "kernels_per_step": nl, "batch_create_seconds": t_create,
+           "graph_ms_per_step": float(np.mean(kms)), "graphs_per_step": nparts, "kernels_per_graph": nl,
+           "batch_create_seconds": t_create,
            "sample_rate_max_rel_err_vs_fp64": max(errs),
            "crystals_counted_only": [{"crystal": k, "nbands": cases[k].nbands,
                                       "mesh": list(map(int, cases[k].modes.ngrid)), "processes": counts[k][0],

@@ -2182,9 +2182,10 @@ int fp4_batch_create(int32_t n, fp4_ctx *const *ctxs, fp4_batch **out) {
   return FP4_OK;
 }
 
-int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double *const *rates, int64_t *nproc,
-                      double *kernel_ms, int32_t *launches) {
+int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_t part, int32_t nparts,
+                      double *const *rates, int64_t *nproc, double *kernel_ms, int32_t *launches) {
   if (!b || !rates) return fail(FP4_ERR_ARG, "bad arguments");
+  if (nparts < 1 || part < 0 || part >= nparts) return fail(FP4_ERR_ARG, "part %d of %d", part, nparts);
   if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP16X2)
     return fail(FP4_ERR_UNSUPPORTED, "batch: FP4_PREC_FP32 or FP4_PREC_FP16X2 only");
   HIPCHECK(hipSetDevice(b->device));
@@ -2208,16 +2209,24 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double
     x.r.tuning = nullptr;
     x.modes.resize((size_t)c->nb * c->nlist);
     std::iota(x.modes.begin(), x.modes.end(), 1);
-    x.ii_lo0 = 0;
-    x.nii = c->nptk;
-    x.counts = b->counts[i];
+    // q' window of this part: [part*nptk/nparts, (part+1)*nptk/nparts)
+    const long lo = (long)part * c->nptk / nparts, hi = (long)(part + 1) * c->nptk / nparts;
+    x.ii_lo0 = (int)lo;
+    x.nii = (int)(hi - lo);
+    {
+      // this window's counts, in the count pass's (channel, mode, q') layout
+      const size_t nm = (size_t)c->nb * c->nlist;
+      x.counts.resize(3 * nm * (size_t)x.nii);
+      for (size_t cm = 0; cm < 3 * nm; cm++)
+        for (int t = 0; t < x.nii; t++) x.counts[cm * x.nii + t] = b->counts[i][cm * c->nptk + lo + t];
+    }
     mem.push_back(std::move(x));
     who.push_back(i);
   }
   double ms = 0.0;
   int nl = 0;
   if (!mem.empty()) {
-    const int rc = dense_rates_batch(b->dense, mem, precision, b->stream, use_graph != 0, &ms, &nl);
+    const int rc = dense_rates_batch(b->dense, mem, precision, b->stream, use_graph != 0, part, &ms, &nl);
     if (rc) return fail(rc, "batch: %s", dense_error());
   }
   for (size_t k = 0; k < mem.size(); k++) {
@@ -2242,7 +2251,9 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double
     double *o = rates[i];
     std::fill(o, o + 7 * nm, 0.0);
     int64_t np = 0;
-    int rc = fp4_rta_4ph_modes(c, FP4_PREC_FP64, (int)nm, modes.data(), 1, c->nptk, o, o + nm, o + 2 * nm, o + 3 * nm,
+    const int lo = (int)((long)part * c->nptk / nparts), hi = (int)((long)(part + 1) * c->nptk / nparts);
+    if (hi <= lo) continue;
+    int rc = fp4_rta_4ph_modes(c, FP4_PREC_FP64, (int)nm, modes.data(), lo + 1, hi, o, o + nm, o + 2 * nm, o + 3 * nm,
                                o + 4 * nm, o + 5 * nm, o + 6 * nm, &np);
     if (rc) return rc;
     if (nproc) nproc[i] = np;

@@ -1533,13 +1533,17 @@ struct DenseBatchState {
     std::vector<DenseArgs> h_args;       // as uploaded (graph replays are valid while unchanged)
     std::vector<long long> h_order;
   };
-  std::vector<Class> classes;
+  // one set of device tables and one graph per q' part of the crystals
+  struct Part {
+    std::vector<Class> classes;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_prec = -1;
+    bool graph_valid = false;
+  };
+  std::map<int, Part> parts;
   int *counters = nullptr;               // one queue counter per class
   int ncounters = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_prec = -1;
-  bool graph_valid = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -1554,21 +1558,24 @@ DenseBatchState *dense_batch_create() {
 
 void dense_batch_destroy(DenseBatchState *b) {
   if (!b) return;
-  for (auto &c : b->classes) {
-    if (c.d_args) (void)hipFree(c.d_args);
-    if (c.d_order) (void)hipFree(c.d_order);
+  for (auto &kv : b->parts) {
+    for (auto &c : kv.second.classes) {
+      if (c.d_args) (void)hipFree(c.d_args);
+      if (c.d_order) (void)hipFree(c.d_order);
+    }
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
   }
   if (b->counters) (void)hipFree(b->counters);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
   delete b;
 }
 
 int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, int precision, hipStream_t st,
-                      bool use_graph, double *kernel_ms, int *launches) {
+                      bool use_graph, int part, double *kernel_ms, int *launches) {
   if (!b) return derr(FP4_ERR_STATE, "batch state missing");
+  DenseBatchState::Part &P = b->parts[part];
   if (precision != FP4_PREC_FP32 && precision != FP4_PREC_FP16X2)
     return derr(FP4_ERR_UNSUPPORTED, "batch: FP32 or FP16X2 MFMA forms only");
   int dev = 0, ldsmax = 0, ncu = 0;
@@ -1631,7 +1638,7 @@ int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, in
       prep[i].a.roots_lds = c.ka.roots_lds;
     }
   // device argument tables and the largest-first work order
-  bool changed = !b->graph_valid || b->graph_prec != precision || b->classes.size() != cls.size();
+  bool changed = !P.graph_valid || P.graph_prec != precision || P.classes.size() != cls.size();
   if ((int)cls.size() > b->ncounters) {
     if (b->counters) (void)hipFree(b->counters);
     b->counters = nullptr;
@@ -1641,7 +1648,7 @@ int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, in
   }
   for (size_t k = 0; k < cls.size(); k++) {
     auto &c = cls[k];
-    DenseBatchState::Class *old = (k < b->classes.size()) ? &b->classes[k] : nullptr;
+    DenseBatchState::Class *old = (k < P.classes.size()) ? &P.classes[k] : nullptr;
     c.h_args.clear();
     for (int i : c.members) c.h_args.push_back(prep[i].a);
     // cost of an item (channel, point, mode block, q'): its processes (the
@@ -1714,47 +1721,47 @@ int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, in
     c.ka.nitems = (long)c.h_order.size();
     c.ka.counter = b->counters + k;
   }
-  for (auto &c : b->classes) {
+  for (auto &c : P.classes) {
     if (c.d_args) (void)hipFree(c.d_args);
     if (c.d_order) (void)hipFree(c.d_order);
   }
-  b->classes = std::move(cls);
+  P.classes = std::move(cls);
   // the device work of a step: queue counters, one persistent launch per
   // class, the per-crystal reductions
   auto enqueue = [&](hipStream_t s2) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s2, b->counters, (int)b->classes.size());
-    for (auto &c : b->classes)
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s2, b->counters, (int)P.classes.size());
+    for (auto &c : P.classes)
       if (c.ka.nitems > 0) c.sh->launch(c.ka, (int)std::min<long>(c.slots, c.ka.nitems), c.dyn, s2);
-    for (auto &c : b->classes)
+    for (auto &c : P.classes)
       for (int i : c.members) dense_reduce_launch(prep[i], s2);
   };
   int nl = 1;
-  for (auto &c : b->classes) nl += (c.ka.nitems > 0 ? 1 : 0) + (int)c.members.size();
+  for (auto &c : P.classes) nl += (c.ka.nitems > 0 ? 1 : 0) + (int)c.members.size();
   if (hipEventRecord(b->e0, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
   if (use_graph) {
     if (changed) {
-      if (b->exec) (void)hipGraphExecDestroy(b->exec);
-      if (b->graph) (void)hipGraphDestroy(b->graph);
-      b->exec = nullptr;
-      b->graph = nullptr;
-      b->graph_valid = false;
+      if (P.exec) (void)hipGraphExecDestroy(P.exec);
+      if (P.graph) (void)hipGraphDestroy(P.graph);
+      P.exec = nullptr;
+      P.graph = nullptr;
+      P.graph_valid = false;
       if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return derr(FP4_ERR_DEVICE, "batch: stream capture failed");
       enqueue(st);
-      if (hipStreamEndCapture(st, &b->graph) != hipSuccess ||
-          hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+      if (hipStreamEndCapture(st, &P.graph) != hipSuccess ||
+          hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0) != hipSuccess)
         return derr(FP4_ERR_DEVICE, "batch: graph capture failed");
-      b->graph_valid = true;
-      b->graph_prec = precision;
+      P.graph_valid = true;
+      P.graph_prec = precision;
     }
-    if (hipGraphLaunch(b->exec, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "batch: graph launch failed");
+    if (hipGraphLaunch(P.exec, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "batch: graph launch failed");
   } else {
     enqueue(st);
-    b->graph_valid = false;
+    P.graph_valid = false;
   }
   if (hipGetLastError() != hipSuccess) return derr(FP4_ERR_DEVICE, "batch: launch failed");
   if (hipEventRecord(b->e1, st) != hipSuccess) return derr(FP4_ERR_DEVICE, "event record failed");
-  for (auto &c : b->classes)
+  for (auto &c : P.classes)
     for (int i : c.members) {
       DenseBatchMember &x = mem[i];
       if ((rc = dense_collect(x.s, prep[i], x.r.m.nb, x.modes, st, x.g, x.w, x.n))) return rc;

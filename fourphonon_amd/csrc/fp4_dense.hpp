@@ -111,8 +111,10 @@ struct DenseBatchMember {
 struct DenseBatchState;
 DenseBatchState *dense_batch_create();
 void dense_batch_destroy(DenseBatchState *b);
+// part: the key of the cached device tables and graph (one per q' window
+// the caller cycles through).
 int dense_rates_batch(DenseBatchState *b, std::vector<DenseBatchMember> &mem, int precision, hipStream_t st,
-                      bool use_graph, double *kernel_ms, int *launches);
+                      bool use_graph, int part, double *kernel_ms, int *launches);
 const char *dense_error();
 
 }  // namespace fp4

@@ -248,6 +248,15 @@ module fp4_processes
             c2(*), c3(*), c4(*)
        real(c_double), intent(in) :: gp(*), gm(*), gpp(*), gpm(*), gmm(*)
      end function c_set_operator
+     integer(c_int) function c_assemble(ctx, nlist, Nequi, alleq, types, prec, w3, w4, n3, n4, r3, r4) &
+          bind(C, name="fp4_bte_assemble_operator")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nlist, prec, w3, w4
+       integer(c_int32_t), intent(in) :: Nequi(*), alleq(*), types(*)
+       integer(c_int64_t), intent(out) :: n3, n4
+       real(c_double), intent(out) :: r3(*), r4(*)
+     end function c_assemble
      integer(c_int) function c_bte_iterate(ctx, tau, F) bind(C, name="fp4_bte_iterate")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
@@ -291,7 +300,7 @@ module fp4_processes
   public :: NP_driver_4ph, Ind_only_driver_4ph, RTA_driver_4ph, RTA_driver_4ph_GPU_using_Ind, Ind_driver_4ph
   public :: Vp_pp, Vp_pm, Vp_mm
   public :: NP_driver, Ind_only_driver, RTA_driver, RTA_driver_GPU_using_Ind, Ind_driver, Vp_plus, Vp_minus
-  public :: fp4_init_iterations, fp4_keep_operator, fp4_set_operator_4ph, fp4_solve_4ph
+  public :: fp4_init_iterations, fp4_keep_operator, fp4_set_operator_4ph, fp4_solve_4ph, fp4_assemble_operator
   public :: iteration0, iteration, iteration_4ph, TConduct
 
 contains
@@ -849,6 +858,24 @@ contains
     real(kind=8), intent(out) :: ThConductivity(:,:,:), ThConductivityMode(:,:,:,:)
     call check(c_tconduct(ctx, volume_, F_n, ThConductivity, ThConductivityMode), "TConduct")
   end subroutine TConduct
+
+  ! The 3ph+4ph operator of iteration_4ph assembled on the device at the
+  ! current temperature, aggregated by partner mode instead of stored as
+  ! Ind_driver / Ind_driver_4ph lists (which do not fit memory at large
+  ! meshes), with the 4ph Gamma in the precision of fp4_set_precision and
+  ! the 3ph Gamma in FP64.  rate_3ph and rate_4ph (Nb,Nlist) receive the
+  ! rates of the same processes (for tau_zero).  Later iteration_4ph calls
+  ! and fp4_solve_4ph use this operator (as with fp4_keep_operator(.true.))
+  ! until the next fp4_set_operator_4ph or fp4_keep_operator call.
+  subroutine fp4_assemble_operator(Nlist, Nequi, ALLEquiList, TypeofSymmetry, rate_3ph, rate_4ph)
+    integer(kind=4), intent(in) :: Nlist, Nequi(:), ALLEquiList(:,:), TypeofSymmetry(:,:)
+    real(kind=8), intent(out) :: rate_3ph(:,:), rate_4ph(:,:)
+    integer(c_int64_t) :: n3, n4
+    call check(c_assemble(ctx, Nlist, Nequi, ALLEquiList, TypeofSymmetry, precision, 1_c_int32_t, 1_c_int32_t, &
+         n3, n4, rate_3ph, rate_4ph), "fp4_assemble_operator")
+    keep_operator_ = .true.
+    operator_valid_ = .true.
+  end subroutine fp4_assemble_operator
 
   ! The convergence loop of Src/ShengBTE.f90:1250-1307 on the device, with
   ! the operator of the last fp4_set_operator_4ph / iteration_4ph call:

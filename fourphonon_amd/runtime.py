@@ -614,15 +614,17 @@ class Batch:
             load().fp4_batch_destroy(self.h)
             self.h = None
 
-    def RTA_driver_4ph(self, precision, use_graph=True):
-        """-> (list of per-crystal dicts of (Nb, Nlist) arrays, nproc per crystal, kernel ms, launches)."""
+    def RTA_driver_4ph(self, precision, use_graph=True, part=0, nparts=1):
+        """-> (list of per-crystal dicts of (Nb, Nlist) arrays, nproc per crystal, kernel ms, launches) for
+        q' window part of nparts (the parts' rates add up to the full rates)."""
         outs = [np.zeros(7 * c.nb * c.nlist) for c in self.contexts]
         ptrs = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
         nproc = np.zeros(len(outs), np.int64)
         ms = C.c_double()
         nl = C.c_int32()
-        _check(load().fp4_batch_rta_4ph(self.h, C.c_int32(precision), C.c_int32(int(bool(use_graph))), ptrs,
-                                        _p(nproc), C.byref(ms), C.byref(nl)))
+        _check(load().fp4_batch_rta_4ph(self.h, C.c_int32(precision), C.c_int32(int(bool(use_graph))),
+                                        C.c_int32(part), C.c_int32(nparts), ptrs, _p(nproc), C.byref(ms),
+                                        C.byref(nl)))
         keys = ("rate_scatt_4ph", "rate_scatt_plusplus", "rate_scatt_plusminus", "rate_scatt_minusminus",
                 "WP4_plusplus", "WP4_plusminus", "WP4_minusminus")
         res = []

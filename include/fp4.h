@@ -415,7 +415,12 @@ int fp4_bte_operator_end(fp4_ctx *ctx);
    counts; with use_graph that device work is a captured HIP graph, replayed
    while the crystals' data and T are unchanged.  precision is
    FP4_PREC_FP32 or FP4_PREC_FP16X2 (MFMA).  Crystals of more than 4 atoms
-   run the FP64 per-process form, one after the other.  rates[c] receives
+   run the FP64 per-process form, one after the other.  A call covers the
+   part-th of nparts q' windows of every crystal ([part*nptk/nparts,
+   (part+1)*nptk/nparts)); the rates of the parts add up to the full rates,
+   so a caller that cannot wait for a whole pass cycles through the parts,
+   each with its own cached graph (part = 0, nparts = 1: everything).
+   rates[c] receives
    7 arrays of (Nb,Nlist) doubles back to back (rate_4ph, rate_pp, rate_pm,
    rate_mm, WP4_pp, WP4_pm, WP4_mm) for crystal c, nproc[c] (may be NULL)
    its processes; *kernel_ms the device time of the batched launches and
@@ -423,8 +428,8 @@ int fp4_bte_operator_end(fp4_ctx *ctx);
 typedef struct fp4_batch fp4_batch;
 int fp4_batch_create(int32_t n, fp4_ctx *const *ctxs, fp4_batch **out);
 int fp4_batch_destroy(fp4_batch *b);
-int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, double *const *rates, int64_t *nproc,
-                      double *kernel_ms, int32_t *launches);
+int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_t part, int32_t nparts,
+                      double *const *rates, int64_t *nproc, double *kernel_ms, int32_t *launches);
 
 /* One iteration_4ph sweep with the operator of the last
    fp4_bte_set_operator / fp4_iteration_4ph call; F_n updated in place. */

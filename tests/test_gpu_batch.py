@@ -122,9 +122,9 @@ SMALL = [(c, (5, 5, 5) if crystal.batch_crystal(c)["natoms"] == 2 else (4, 4, 4)
 def test_batch_one_launch_per_class_matches_each_crystal():
     """All 2- and 4-atom crystals through fp4_batch_rta_4ph (one persistent
     launch per Nbands class over one cost-ordered queue, in a HIP graph)
-    against each crystal's own dense call in the same form (the same kernel
-    shape and per-item sums, so the rates agree to rounding of nothing:
-    <= 1e-12), against the FP64 per-process form (FP32 MFMA: 1e-3), and
+    against each crystal's own dense call with the same kernel shape
+    (k_dense, FP32 MFMA, IB = 6 at Nb 6 and 1 at Nb 12: the same per-item
+    sums, so <= 1e-12), against the FP64 per-process form (1e-3), and
     graph replays, with other device work between them, repeat the first
     result bit for bit."""
     banner(f"batch of {len(SMALL)} crystals (Nb 6 and 12) in one graph")
@@ -134,6 +134,8 @@ def test_batch_one_launch_per_class_matches_each_crystal():
         first, nproc, ms, nl = b.RTA_driver_4ph(rt.PREC_FP32)
         print(f"batch: {int(nproc.sum())} processes, {ms:.1f} ms, {nl} kernels", flush=True)
         assert nl == 1 + 2 + len(SMALL)      # counters, one launch per class, one reduction per crystal
+        for ctx in ctxs:
+            ctx.set_dense_shape(rt.PREC_FP32, 0, 6 if ctx.nb == 6 else 1)
         for k, ((c, mesh), ctx) in enumerate(zip(SMALL, ctxs)):
             own, n_own = ctx.RTA_driver_4ph_modes(np.arange(1, ctx.nmodes + 1), rt.PREC_FP32)
             assert int(nproc[k]) == int(n_own), c
@@ -147,6 +149,12 @@ def test_batch_one_launch_per_class_matches_each_crystal():
             assert np.array_equal(nproc2, nproc)
             for k in range(len(SMALL)):
                 assert np.array_equal(again[k]["rate_scatt_4ph"], first[k]["rate_scatt_4ph"]), (rep, k)
+        # q' parts add up to the whole pass
+        parts = [b.RTA_driver_4ph(rt.PREC_FP32, part=p, nparts=3) for p in range(3)]
+        assert np.array_equal(sum(p[1] for p in parts), nproc)
+        for k in range(len(SMALL)):
+            assert_close(sum(p[0][k]["rate_scatt_4ph"] for p in parts), first[k]["rate_scatt_4ph"], 1e-12,
+                         f"parts of crystal {k}")
         # a changed temperature re-captures the graph
         for ctx in ctxs:
             ctx.set_temperature(500.0)
