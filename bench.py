@@ -560,16 +560,21 @@ def run_batch24(args):
     small = [k for k, cs in enumerate(cases) if cs.structure.natoms <= 4]
     big = [k for k in range(len(cases)) if k not in small]
     counts = {}
-    for k in big:
+    for k in range(len(cases)):
         t1 = time.perf_counter()
         N = ctxs[k].NP_driver_4ph()
         counts[k] = (int(N[0].sum() + N[2].sum() + N[4].sum()), time.perf_counter() - t1)
-        log(f"crystal {k} (Nb {cases[k].nbands}, {list(cases[k].modes.ngrid)}): {counts[k][0]} processes counted")
+        log(f"crystal {k} (Nb {cases[k].nbands}, mesh {int(cases[k].modes.ngrid[0])}^3): {counts[k][0]} processes "
+            f"counted in {counts[k][1]:.1f}s")
+    n_small_counted = sum(counts[k][0] for k in small)
     t1 = time.perf_counter()
     b = rt.Batch([ctxs[k] for k in small])
     t_create = time.perf_counter() - t1
+    log(f"batch of {len(small)} crystals created in {t_create:.1f}s ({n_small_counted} processes per pass)")
     prec = rt.PREC_FP32
-    nparts = 16    # q' windows per step, one graph each, so that progress shows between them
+    # q' windows per step, one graph each, about 4e10 processes per window, so
+    # that progress shows between them
+    nparts = int(max(16, min(256, np.ceil(n_small_counted / 4e10))))
 
     def step(tag):
         tot, np_tot, ms_tot = None, 0, 0.0
