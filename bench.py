@@ -544,10 +544,13 @@ def run_batch24(args):
     6 and 8 atoms, meshes 6^3 to 14^3 as drawn): the RTA pass of every 2- and
     4-atom crystal (Nb 6 and 12) through fp4_batch_rta_4ph -- one persistent
     FP32-MFMA launch per Nbands class over one queue of all their work items,
-    costliest first by the prefilter counts, in a captured HIP graph; a step
-    is 16 such graphs, one per q' sixteenth (progress shows between them) --
-    timed over --steps steps; each crystal's FP64 per-process rates on a sample
-    (three modes, q' = 1..4) give the per-mode error.  The 6- and 8-atom
+    costliest first by the prefilter counts, in a captured HIP graph.  The
+    whole pass is about 3e12 processes, so a step is a sample: 8 of its 64 q'
+    windows spread over the q' range, one graph each, timed over --steps
+    steps (a warm-up step runs each window's count pass and captures its
+    graph); the batch's rates of the first window against each crystal's
+    FP64 per-process rates there (three modes) give the per-mode error.  The
+    6- and 8-atom
     crystals (Nb 18 and 24), which the dense kernel does not take, are
     counted only.  Prints one JSON line."""
     from fourphonon_amd import crystal
@@ -560,25 +563,24 @@ def run_batch24(args):
     small = [k for k, cs in enumerate(cases) if cs.structure.natoms <= 4]
     big = [k for k in range(len(cases)) if k not in small]
     counts = {}
-    for k in range(len(cases)):
+    for k in big:
         t1 = time.perf_counter()
         N = ctxs[k].NP_driver_4ph()
         counts[k] = (int(N[0].sum() + N[2].sum() + N[4].sum()), time.perf_counter() - t1)
         log(f"crystal {k} (Nb {cases[k].nbands}, mesh {int(cases[k].modes.ngrid[0])}^3): {counts[k][0]} processes "
             f"counted in {counts[k][1]:.1f}s")
-    n_small_counted = sum(counts[k][0] for k in small)
     t1 = time.perf_counter()
     b = rt.Batch([ctxs[k] for k in small])
     t_create = time.perf_counter() - t1
-    log(f"batch of {len(small)} crystals created in {t_create:.1f}s ({n_small_counted} processes per pass)")
     prec = rt.PREC_FP32
-    # q' windows per step, one graph each, about 4e10 processes per window, so
-    # that progress shows between them
-    nparts = int(max(16, min(256, np.ceil(n_small_counted / 4e10))))
+    # a step is a sample of the pass: 8 of its 64 q' windows, spread over the
+    # q' range, one graph each (the whole pass is about 3e12 processes)
+    nparts = 64
+    parts = [k * nparts // 8 for k in range(8)]
 
     def step(tag):
         tot, np_tot, ms_tot = None, 0, 0.0
-        for part in range(nparts):
+        for part in parts:
             res, nproc, ms, nl = b.RTA_driver_4ph(prec, part=part, nparts=nparts)
             if tot is None:
                 tot = [{k: v.copy() for k, v in r.items()} for r in res]
@@ -588,7 +590,8 @@ def run_batch24(args):
                         t_[k] += r[k]
             np_tot = np_tot + nproc
             ms_tot += ms
-            log(f"{tag} part {part + 1}/{nparts}: {int(nproc.sum())} processes, graph {ms:.1f} ms, {nl} kernels")
+            log(f"{tag} q' window {part + 1}/{nparts}: {int(nproc.sum())} processes, graph {ms:.1f} ms, "
+                f"{nl} kernels")
         return tot, np_tot, ms_tot, nl
 
     for w in range(args.warmup):
@@ -599,17 +602,21 @@ def run_batch24(args):
         res, nproc, ms, nl = step(f"step {st + 1}/{args.steps}")
         kms.append(ms)
     wall = (time.perf_counter() - t1) / args.steps
+    # the batch's rates of window 1 against each crystal's FP64 per-process
+    # rates over the same q' window, three modes per crystal
+    res0, _, _, _ = b.RTA_driver_4ph(prec, part=0, nparts=nparts)
     errs = []
     for j, k in enumerate(small):
         c = ctxs[k]
-        tot = res[j]["rate_scatt_4ph"].ravel(order="F")
+        tot = res0[j]["rate_scatt_4ph"].ravel(order="F")
         act = np.nonzero(tot > 0)[0] + 1
         modes = act[np.linspace(0, len(act) - 1, 3).astype(int)]
-        r64, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64, ii_lo=1, ii_hi=4)
-        rb, _ = c.RTA_driver_4ph_modes(modes, prec, ii_lo=1, ii_hi=4)
-        a_, b_ = rb["rate_scatt_4ph"].ravel(order="F")[modes - 1], r64["rate_scatt_4ph"].ravel(order="F")[modes - 1]
+        hi = c.nptk // nparts
+        r64, _ = c.RTA_driver_4ph_modes(modes, rt.PREC_FP64, ii_lo=1, ii_hi=hi)
+        a_, b_ = tot[modes - 1], r64["rate_scatt_4ph"].ravel(order="F")[modes - 1]
         ok = b_ != 0
         errs.append(float(np.max(np.abs(a_[ok] - b_[ok]) / np.abs(b_[ok]), initial=0.0)))
+        log(f"crystal {k}: batch vs FP64 per-process over q' = 1..{hi}: max rel err {errs[-1]:.2e}")
     b.close()
     for c in ctxs:
         c.close()
@@ -617,10 +624,12 @@ def run_batch24(args):
     out = {"workload": "24-crystal batch (2/4/6/8-atom fcc, zincblende, rocksalt-like cells, meshes 6^3..14^3), "
                        "4ph RTA at 300 K: the 2- and 4-atom crystals in one graph of one persistent FP32-MFMA "
                        "launch per Nbands class",
+           "sample": f"a step is {len(parts)} of the pass's {nparts} q' windows (windows {[p + 1 for p in parts]})",
            "precision": "fp32", "steps": args.steps, "warmup": args.warmup,
            "crystals_batched": [{"crystal": k, "nbands": cases[k].nbands, "mesh": list(map(int, cases[k].modes.ngrid)),
                                  "processes": int(nproc[j])} for j, k in enumerate(small)],
-           "processes_batched": n_small, "seconds_per_step": wall, "value": n_small / wall, "unit": "processes/s",
+           "processes_batched_per_step": n_small, "seconds_per_step": wall, "value": n_small / wall,
+           "unit": "processes/s",
            "graph_ms_per_step": float(np.mean(kms)), "graphs_per_step": nparts, "kernels_per_graph": nl,
            "batch_create_seconds": t_create,
            "sample_rate_max_rel_err_vs_fp64": max(errs),

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -2131,7 +2132,9 @@ struct fp4_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   DenseBatchState *dense = nullptr;
-  std::vector<std::vector<int>> counts;   // per context: count pass per (channel, mode, q')
+  // per (part, nparts) and context: the count pass over the part's q'
+  // window, per (channel, mode, q'); it orders the queue and does not depend on T
+  std::map<std::pair<int, int>, std::vector<std::vector<int>>> counts;
 };
 
 extern "C" {
@@ -2161,23 +2164,6 @@ int fp4_batch_create(int32_t n, fp4_ctx *const *ctxs, fp4_batch **out) {
     fp4_batch_destroy(b);
     return fail(FP4_ERR_DEVICE, "batch: stream or event creation failed");
   }
-  // the prefilter counts order the batch's work queue (they do not depend on T)
-  b->counts.resize(n);
-  for (int i = 0; i < n; i++) {
-    fp4_ctx *c = ctxs[i];
-    if (c->natoms > 4) continue;
-    Plan pl;
-    all_modes(c, pl);
-    if ((rc = count_pass(c, pl, false))) {
-      fp4_batch_destroy(b);
-      return rc;
-    }
-    b->counts[i].resize(pl.items);
-    if (hipMemcpy(b->counts[i].data(), c->cnt.p, sizeof(int) * pl.items, hipMemcpyDeviceToHost) != hipSuccess) {
-      fp4_batch_destroy(b);
-      return fail(FP4_ERR_DEVICE, "batch: copy of the counts failed");
-    }
-  }
   *out = b;
   return FP4_OK;
 }
@@ -2190,8 +2176,32 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
     return fail(FP4_ERR_UNSUPPORTED, "batch: FP4_PREC_FP32 or FP4_PREC_FP16X2 only");
   HIPCHECK(hipSetDevice(b->device));
   const int n = (int)b->ctx.size();
-  for (int i = 0; i < n; i++)
+  for (int i = 0; i < n; i++) {
     if (!rates[i]) return fail(FP4_ERR_ARG, "null output of crystal %d", i);
+    std::fill(rates[i], rates[i] + 7 * (size_t)b->ctx[i]->nb * b->ctx[i]->nlist, 0.0);
+    if (nproc) nproc[i] = 0;
+  }
+  // the prefilter counts of this part's windows, once per (part, nparts)
+  auto key = std::make_pair((int)part, (int)nparts);
+  if (!b->counts.count(key)) {
+    std::vector<std::vector<int>> cs(n);
+    for (int i = 0; i < n; i++) {
+      fp4_ctx *c = b->ctx[i];
+      if (c->natoms > 4) continue;
+      const long lo = (long)part * c->nptk / nparts, hi = (long)(part + 1) * c->nptk / nparts;
+      if (hi <= lo) continue;
+      Plan pl;
+      all_modes(c, pl);
+      pl.ii_lo0 = (int)lo;
+      pl.nii = (int)(hi - lo);
+      int rc;
+      if ((rc = count_pass(c, pl, false))) return rc;
+      cs[i].resize(pl.items);
+      HIPCHECK(hipMemcpy(cs[i].data(), c->cnt.p, sizeof(int) * pl.items, hipMemcpyDeviceToHost));
+    }
+    b->counts[key] = std::move(cs);
+  }
+  const std::vector<std::vector<int>> &counts = b->counts[key];
   std::vector<DenseBatchMember> mem;
   std::vector<int> who;
   for (int i = 0; i < n; i++) {
@@ -2211,15 +2221,10 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
     std::iota(x.modes.begin(), x.modes.end(), 1);
     // q' window of this part: [part*nptk/nparts, (part+1)*nptk/nparts)
     const long lo = (long)part * c->nptk / nparts, hi = (long)(part + 1) * c->nptk / nparts;
+    if (hi <= lo) continue;
     x.ii_lo0 = (int)lo;
     x.nii = (int)(hi - lo);
-    {
-      // this window's counts, in the count pass's (channel, mode, q') layout
-      const size_t nm = (size_t)c->nb * c->nlist;
-      x.counts.resize(3 * nm * (size_t)x.nii);
-      for (size_t cm = 0; cm < 3 * nm; cm++)
-        for (int t = 0; t < x.nii; t++) x.counts[cm * x.nii + t] = b->counts[i][cm * c->nptk + lo + t];
-    }
+    x.counts = counts[i];     // (channel, mode, q') over the window, the count pass's layout
     mem.push_back(std::move(x));
     who.push_back(i);
   }

@@ -407,8 +407,9 @@ int fp4_bte_operator_end(fp4_ctx *ctx);
    once.  No counterpart in the reference, which runs one crystal per job
    (its nearest analogue is the gang loop over one crystal's modes,
    Src/processes.f90:4202-4243).  The contexts (one per crystal, modes and
-   IFCs set, one device) are taken as they are; fp4_batch_create runs each
-   crystal's count pass once, to order the work.  fp4_batch_rta_4ph then runs
+   IFCs set, one device) are taken as they are.  fp4_batch_rta_4ph runs each
+   crystal's count pass over the call's q' window once (it orders the work
+   and does not depend on T), then
    the dense form of every crystal of at most 4 atoms in one persistent
    launch per Nbands class, over a single queue of all their work items
    (channel, point, mode block, q') ordered costliest first by the prefilter
