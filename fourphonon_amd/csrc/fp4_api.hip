@@ -2187,7 +2187,7 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
     std::vector<std::vector<int>> cs(n);
     for (int i = 0; i < n; i++) {
       fp4_ctx *c = b->ctx[i];
-      if (c->natoms > 4) continue;
+      if (c->natoms > kDenseMaxAtoms) continue;
       const long lo = (long)part * c->nptk / nparts, hi = (long)(part + 1) * c->nptk / nparts;
       if (hi <= lo) continue;
       Plan pl;
@@ -2206,7 +2206,7 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
   std::vector<int> who;
   for (int i = 0; i < n; i++) {
     fp4_ctx *c = b->ctx[i];
-    if (c->natoms > 4) continue;
+    if (c->natoms > kDenseMaxAtoms) continue;
     DenseBatchMember x;
     x.s = c->dense;
     x.r.m = mode_view(c);
@@ -2246,10 +2246,10 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
                 true);
     if (nproc) nproc[i] = std::accumulate(mem[k].n.begin(), mem[k].n.end(), 0LL);
   }
-  // cells beyond the dense kernel's 4 atoms: the FP64 per-process form
+  // cells beyond the dense kernel's 8 atoms: the FP64 per-process form
   for (int i = 0; i < n; i++) {
     fp4_ctx *c = b->ctx[i];
-    if (c->natoms <= 4) continue;
+    if (c->natoms <= kDenseMaxAtoms) continue;
     const size_t nm = (size_t)c->nb * c->nlist;
     std::vector<int> modes(nm);
     std::iota(modes.begin(), modes.end(), 1);

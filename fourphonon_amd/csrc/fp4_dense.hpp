@@ -8,6 +8,11 @@
 
 namespace fp4 {
 
+// Atoms per cell the dense kernels take (8: Nbands up to 24), and the (K, L)
+// atom pairs that follow from it.
+constexpr int kDenseMaxAtoms = 8;
+constexpr int kDenseMaxPairs = kDenseMaxAtoms * kDenseMaxAtoms;
+
 // Coarse phase group: all clusters whose third and fourth atoms are (K, L)
 // and whose R_k - R_l is the same lattice vector d (in phase units).
 struct DGroup {

@@ -33,7 +33,7 @@ struct DenseArgs {
   const FGroup *f;
   const DCluster *c;
   const double *phi;
-  int klf[17];           // first coarse group of each (K, L) pair, then ng
+  int klf[kDenseMaxPairs + 1];  // first coarse group of each (K, L) pair, then ng
   const double *fbose;   // [Nb][nptk] Bose-Einstein factor of every mode at T
   int roots_lds;         // the root table is copied to LDS (L <= kRootsLds)
   int ng;
@@ -75,6 +75,10 @@ struct DenseArgs {
   const DenseArgs *batch;
   const long long *order;
   int nbatch;
+  // cells of more than 4 atoms: A' of every resident block in device memory,
+  // ag_stride bytes per block (the table does not fit LDS beside the tile)
+  void *ag;
+  long long ag_stride;
 };
 
 // The arguments of the crystal a work item belongs to.
@@ -218,7 +222,7 @@ constexpr float kPhaseScaleF16 = 16384.0f;
 struct DenseGeom {
   int ng, ntri, L;
   int npairs;            // (K, L) atom pairs
-  int klf[17];           // first coarse group of each pair, then ng
+  int klf[kDenseMaxPairs + 1];  // first coarse group of each pair, then ng
 };
 
 // One kernel shape: precision, Nbands, modes per block (IB), q'' per tile

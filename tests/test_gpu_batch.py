@@ -2,9 +2,8 @@
 and 8 atoms, Nb = 6 ... 24, meshes 6^3 ... 14^3) against the oracle.
 
 Bars: counts and lists bit-exact; FP64 rates within 1e-10 relative; the
-dense FP32 and BF16 hi/lo forms (cells of up to 4 atoms) within 1e-3
-relative per mode of the FP64 form.  Cells of 6 and 8 atoms run the FP64
-form only: the dense kernel takes at most 4 atoms per cell.
+dense FP32 MFMA form (every cell size) and the BF16 / FP16 hi/lo forms
+(cells of up to 4 atoms) within 1e-3 relative per mode of the FP64 form.
 """
 import os
 
@@ -103,37 +102,37 @@ def test_batch_golden(c):
         r, n64 = ctx.RTA_driver_4ph_modes([rm], rt.PREC_FP64, ii_lo=lo, ii_hi=hi)
         for key, wk in RATE_KEYS:
             assert_close(r[key].ravel(order="F")[rm - 1], g[wk][c], RTOL, key)
-        if case.structure.natoms <= 4:
-            want = g["rate_4ph"][c]
-            for prec in (rt.PREC_FP32, rt.PREC_BF16X3, rt.PREC_FP16X3):
-                rd, nd = ctx.RTA_driver_4ph_modes([rm], prec, ii_lo=lo, ii_hi=hi)
-                got = rd["rate_scatt_4ph"].ravel(order="F")[rm - 1]
-                assert nd == n64
-                assert abs(got - want) <= 1e-3 * abs(want), (prec, got, want)
+        want = g["rate_4ph"][c]
+        precs = (rt.PREC_FP32, rt.PREC_BF16X3, rt.PREC_FP16X3) if case.structure.natoms <= 4 else (rt.PREC_FP32,)
+        for prec in precs:
+            rd, nd = ctx.RTA_driver_4ph_modes([rm], prec, ii_lo=lo, ii_hi=hi)
+            got = rd["rate_scatt_4ph"].ravel(order="F")[rm - 1]
+            assert nd == n64
+            assert abs(got - want) <= 1e-3 * abs(want), (prec, got, want)
     finally:
         ctx.close()
 
 
-# the 2- and 4-atom crystals of the batch (Nb 6 and 12), at small meshes
-SMALL = [(c, (5, 5, 5) if crystal.batch_crystal(c)["natoms"] == 2 else (4, 4, 4))
-         for c in range(crystal.BATCH_SIZE) if crystal.batch_crystal(c)["natoms"] <= 4]
+# all 24 crystals of the batch (Nb 6, 12, 18 and 24) at small meshes
+SMALL_MESH = {2: (5, 5, 5), 4: (4, 4, 4), 6: (3, 3, 3), 8: (3, 3, 3)}
+SMALL = [(c, SMALL_MESH[crystal.batch_crystal(c)["natoms"]]) for c in range(crystal.BATCH_SIZE)]
 
 
-def test_batch_one_launch_per_class_matches_each_crystal():
-    """All 2- and 4-atom crystals through fp4_batch_rta_4ph (one persistent
-    launch per Nbands class over one cost-ordered queue, in a HIP graph)
-    against each crystal's own dense call with the same kernel shape
-    (k_dense, FP32 MFMA, IB = 6 at Nb 6 and 1 at Nb 12: the same per-item
-    sums, so <= 1e-12), against the FP64 per-process form (1e-3), and
-    graph replays, with other device work between them, repeat the first
-    result bit for bit."""
-    banner(f"batch of {len(SMALL)} crystals (Nb 6 and 12) in one graph")
+def test_batch_one_launch_matches_each_crystal():
+    """All 24 crystals through fp4_batch_rta_4ph (one persistent launch over
+    one cost-ordered queue of every crystal's items, in a HIP graph) against
+    each crystal's own dense call with the same kernel shape (k_dense, FP32
+    MFMA, IB = 6 at Nb 6 and 1 at Nb 12, 18, 24: the same per-item sums, so
+    <= 1e-12), against the FP64 per-process form (1e-3), and graph replays,
+    with other device work between them, repeat the first result bit for
+    bit."""
+    banner(f"batch of {len(SMALL)} crystals (Nb 6 to 24) in one graph")
     ctxs = [rt.Context(crystal.batch_case(c, ngrid=mesh)) for c, mesh in SMALL]
     b = rt.Batch(ctxs)
     try:
         first, nproc, ms, nl = b.RTA_driver_4ph(rt.PREC_FP32)
         print(f"batch: {int(nproc.sum())} processes, {ms:.1f} ms, {nl} kernels", flush=True)
-        assert nl == 1 + 2 + len(SMALL)      # counters, one launch per class, one reduction per crystal
+        assert nl == 1 + 1 + len(SMALL)      # counter, one dense launch, one reduction per crystal
         for ctx in ctxs:
             ctx.set_dense_shape(rt.PREC_FP32, 0, 6 if ctx.nb == 6 else 1)
         for k, ((c, mesh), ctx) in enumerate(zip(SMALL, ctxs)):
@@ -143,6 +142,11 @@ def test_batch_one_launch_per_class_matches_each_crystal():
                 assert_close(first[k][key], own[key], 1e-12, f"crystal {c} {key}")
             r64, _ = ctx.RTA_driver_4ph_modes(np.arange(1, ctx.nmodes + 1), rt.PREC_FP64)
             assert_close(first[k]["rate_scatt_4ph"], r64["rate_scatt_4ph"], 1e-3, f"crystal {c} vs FP64")
+        # the FP16X2 batch: 2- and 4-atom classes in FP16X2, 6- and 8-atom ones in FP32
+        h16, n16, _, nl16 = b.RTA_driver_4ph(rt.PREC_FP16X2)
+        assert nl16 == nl and np.array_equal(n16, nproc)
+        for k, ((c, mesh), ctx) in enumerate(zip(SMALL, ctxs)):
+            assert_close(h16[k]["rate_scatt_4ph"], first[k]["rate_scatt_4ph"], 1e-3, f"crystal {c} FP16X2")
         for rep in range(3):
             ctxs[0].NP_driver_4ph()           # other device work between replays
             again, nproc2, _, _ = b.RTA_driver_4ph(rt.PREC_FP32)
