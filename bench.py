@@ -175,7 +175,7 @@ def main():
                          "iterative kappa with the assembled 3ph+4ph operator at 100/300/700 K; sample16: the sampling estimator on "
                          "diamond 16^3 against the full rates; wurtzite12: every MFMA form on the 12x12x8 "
                          "wurtzite against dense FP64 (per-mode and kappa errors); batch24: the 24-crystal batch, "
-                         "2- and 4-atom cells in one graph of one persistent launch per Nbands class")
+                         "all 24 crystals (Nb 6 to 24) in one graph of one persistent launch")
     ap.add_argument("--phase-profile", action="store_true",
                     help="load the profiling build (libfp4prof.so) and print the dense kernel's per-phase cycle "
                          "shares of the timed steps")
@@ -541,18 +541,16 @@ def run_wurtzite12(args):
 
 def run_batch24(args):
     """The 24-crystal batch (fcc / zincblende / rocksalt-like cells of 2, 4,
-    6 and 8 atoms, meshes 6^3 to 14^3 as drawn): the RTA pass of every 2- and
-    4-atom crystal (Nb 6 and 12) through fp4_batch_rta_4ph -- one persistent
-    FP32-MFMA launch per Nbands class over one queue of all their work items,
+    6 and 8 atoms, Nb 6 to 24, meshes 6^3 to 14^3 as drawn): the RTA pass of
+    every crystal through fp4_batch_rta_4ph -- one persistent FP32-MFMA
+    launch (k_dense_multi) over one queue of all 24 crystals' work items,
     costliest first by the prefilter counts, in a captured HIP graph.  The
-    whole pass is about 3e12 processes, so a step is a sample: 8 of its 64 q'
-    windows spread over the q' range, one graph each, timed over --steps
+    whole pass is about 6.8e12 processes, so a step is a sample: 4 of its 64
+    q' windows spread over the q' range, one graph each, timed over --steps
     steps (a warm-up step runs each window's count pass and captures its
     graph); the batch's rates of the first window against each crystal's
-    FP64 per-process rates there (three modes) give the per-mode error.  The
-    6- and 8-atom
-    crystals (Nb 18 and 24), which the dense kernel does not take, are
-    counted only.  Prints one JSON line."""
+    FP64 per-process rates there (three modes per crystal) give the per-mode
+    error.  Prints one JSON line."""
     from fourphonon_amd import crystal
     from fourphonon_amd import runtime as rt
 
@@ -560,55 +558,39 @@ def run_batch24(args):
     cases = [crystal.batch_case(c) for c in range(crystal.BATCH_SIZE)]
     log(f"24 cases built in {time.time() - t0:.1f}s")
     ctxs = [rt.Context(cs) for cs in cases]
-    small = [k for k, cs in enumerate(cases) if cs.structure.natoms <= 4]
-    big = [k for k in range(len(cases)) if k not in small]
-    counts = {}
-    for k in big:
-        t1 = time.perf_counter()
-        N = ctxs[k].NP_driver_4ph()
-        counts[k] = (int(N[0].sum() + N[2].sum() + N[4].sum()), time.perf_counter() - t1)
-        log(f"crystal {k} (Nb {cases[k].nbands}, mesh {int(cases[k].modes.ngrid[0])}^3): {counts[k][0]} processes "
-            f"counted in {counts[k][1]:.1f}s")
     t1 = time.perf_counter()
-    b = rt.Batch([ctxs[k] for k in small])
+    b = rt.Batch(ctxs)
     t_create = time.perf_counter() - t1
     prec = rt.PREC_FP32
-    # a step is a sample of the pass: 8 of its 64 q' windows, spread over the
-    # q' range, one graph each (the whole pass is about 3e12 processes)
+    # a step is a sample of the pass: 4 of its 64 q' windows, spread over the
+    # q' range, one graph each (the whole pass is about 6.8e12 processes)
     nparts = 64
-    parts = [k * nparts // 8 for k in range(8)]
+    parts = [k * nparts // 4 for k in range(4)]
 
     def step(tag):
-        tot, np_tot, ms_tot = None, 0, 0.0
+        np_tot, ms_tot = 0, 0.0
         for part in parts:
             res, nproc, ms, nl = b.RTA_driver_4ph(prec, part=part, nparts=nparts)
-            if tot is None:
-                tot = [{k: v.copy() for k, v in r.items()} for r in res]
-            else:
-                for t_, r in zip(tot, res):
-                    for k in r:
-                        t_[k] += r[k]
             np_tot = np_tot + nproc
             ms_tot += ms
             log(f"{tag} q' window {part + 1}/{nparts}: {int(nproc.sum())} processes, graph {ms:.1f} ms, "
                 f"{nl} kernels")
-        return tot, np_tot, ms_tot, nl
+        return np_tot, ms_tot, nl
 
     for w in range(args.warmup):
         step(f"warmup {w + 1}/{args.warmup}")
     t1 = time.perf_counter()
     kms = []
     for st in range(args.steps):
-        res, nproc, ms, nl = step(f"step {st + 1}/{args.steps}")
+        nproc, ms, nl = step(f"step {st + 1}/{args.steps}")
         kms.append(ms)
     wall = (time.perf_counter() - t1) / args.steps
     # the batch's rates of window 1 against each crystal's FP64 per-process
     # rates over the same q' window, three modes per crystal
     res0, _, _, _ = b.RTA_driver_4ph(prec, part=0, nparts=nparts)
     errs = []
-    for j, k in enumerate(small):
-        c = ctxs[k]
-        tot = res0[j]["rate_scatt_4ph"].ravel(order="F")
+    for k, c in enumerate(ctxs):
+        tot = res0[k]["rate_scatt_4ph"].ravel(order="F")
         act = np.nonzero(tot > 0)[0] + 1
         modes = act[np.linspace(0, len(act) - 1, 3).astype(int)]
         hi = c.nptk // nparts
@@ -616,26 +598,27 @@ def run_batch24(args):
         a_, b_ = tot[modes - 1], r64["rate_scatt_4ph"].ravel(order="F")[modes - 1]
         ok = b_ != 0
         errs.append(float(np.max(np.abs(a_[ok] - b_[ok]) / np.abs(b_[ok]), initial=0.0)))
-        log(f"crystal {k}: batch vs FP64 per-process over q' = 1..{hi}: max rel err {errs[-1]:.2e}")
+        log(f"crystal {k} (Nb {cases[k].nbands}): batch vs FP64 per-process over q' = 1..{hi}: "
+            f"max rel err {errs[-1]:.2e}")
     b.close()
     for c in ctxs:
         c.close()
-    n_small = int(nproc.sum())
-    out = {"workload": "24-crystal batch (2/4/6/8-atom fcc, zincblende, rocksalt-like cells, meshes 6^3..14^3), "
-                       "4ph RTA at 300 K: the 2- and 4-atom crystals in one graph of one persistent FP32-MFMA "
-                       "launch per Nbands class",
+    n_all = int(nproc.sum())
+    by_nb = {}
+    for k, cs in enumerate(cases):
+        by_nb[cs.nbands] = by_nb.get(cs.nbands, 0) + int(nproc[k])
+    out = {"workload": "24-crystal batch (2/4/6/8-atom fcc, zincblende, rocksalt-like cells, Nb 6..24, meshes "
+                       "6^3..14^3), 4ph RTA at 300 K: all crystals in one graph of one persistent FP32-MFMA launch",
            "sample": f"a step is {len(parts)} of the pass's {nparts} q' windows (windows {[p + 1 for p in parts]})",
            "precision": "fp32", "steps": args.steps, "warmup": args.warmup,
-           "crystals_batched": [{"crystal": k, "nbands": cases[k].nbands, "mesh": list(map(int, cases[k].modes.ngrid)),
-                                 "processes": int(nproc[j])} for j, k in enumerate(small)],
-           "processes_batched_per_step": n_small, "seconds_per_step": wall, "value": n_small / wall,
+           "crystals": [{"crystal": k, "nbands": cases[k].nbands, "mesh": list(map(int, cases[k].modes.ngrid)),
+                         "processes": int(nproc[k])} for k in range(len(cases))],
+           "processes_per_step_by_nbands": {str(k): v for k, v in sorted(by_nb.items())},
+           "processes_per_step": n_all, "seconds_per_step": wall, "value": n_all / wall,
            "unit": "processes/s",
            "graph_ms_per_step": float(np.mean(kms)), "graphs_per_step": len(parts), "kernels_per_graph": nl,
            "batch_create_seconds": t_create,
-           "sample_rate_max_rel_err_vs_fp64": max(errs),
-           "crystals_counted_only": [{"crystal": k, "nbands": cases[k].nbands,
-                                      "mesh": list(map(int, cases[k].modes.ngrid)), "processes": counts[k][0],
-                                      "count_seconds": counts[k][1]} for k in big]}
+           "sample_rate_max_rel_err_vs_fp64": max(errs)}
     print(json.dumps(out), flush=True)
 
 

@@ -512,6 +512,14 @@ __device__ __forceinline__ void dense_item(const DenseArgs &ka, int cr, long ite
 
     // ------------- q'' loop
     const int jj_first = (ch == 1) ? 0 : ii0;
+    // range of the item's x1 for the triple screen
+    double xlo = s.x1[0][0], xhi = s.x1[0][0];
+    for (int il = 0; il < IB; il++)
+      for (int j = 0; j < NB; j++) {
+        xlo = fmin(xlo, s.x1[il][j]);
+        xhi = fmax(xhi, s.x1[il][j]);
+      }
+    const double xabs = fmax(fabs(xlo), fabs(xhi));
     for (int base = jj_first; base < m.nptk; base += NT_DENSE) {
       const int jj = base + tid;
       int ok = 0, ssv = 0;
@@ -520,7 +528,7 @@ __device__ __forceinline__ void dense_item(const DenseArgs &ka, int cr, long ite
 #pragma unroll
         for (int t = 0; t < 3; t++) n3[t] = m.ijk[3 * jj + t];
         ssv = partner(m, ch, n1, n2, n3);
-        ok = (ch == 0) || (ssv >= jj);
+        ok = ((ch == 0) || (ssv >= jj)) && triple_may_survive<NB>(m, ch, jj, ssv, xlo, xhi, xabs);
       }
       int tot;
       const int pos = block_excl_scan<NW>(ok, s.wsum, tot);

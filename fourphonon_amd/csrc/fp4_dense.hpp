@@ -96,13 +96,13 @@ int dense_candidates(const DenseState *s, int cap, int *kg, int *ib, int *tq, in
 // per CU); FP4_ERR_UNSUPPORTED when no such shape is built or fits in LDS.
 int dense_pick(const DenseState *s, int precision, int nb, int kg, int ib, int slots, DenseTuning *out);
 
-// Several crystals' dense rates in one persistent launch per Nbands class
-// (FP32 or FP16X2 MFMA; cells of at most 4 atoms): every crystal's work items
-// go into one queue, costliest first by the prefilter counts (counts:
+// Several crystals' dense rates in one persistent launch (k_dense_multi;
+// FP32 or FP16X2 MFMA, cells of up to 8 atoms, Nbands 6, 12, 18, 24): every
+// crystal's work items go into one queue, costliest first by the prefilter
+// counts (counts:
 // processes per (channel, mode-in-subset, q') in the count pass's layout, or
-// empty), and
-// with use_graph the device work of a call (queue counters, the class
-// launches, the per-crystal reductions) is a captured HIP graph, replayed
+// empty), and with use_graph the device work of a call (the queue counter,
+// the launch, the per-crystal reductions) is a captured HIP graph, replayed
 // while the crystals' arguments are unchanged.
 struct DenseBatchMember {
   DenseState *s = nullptr;

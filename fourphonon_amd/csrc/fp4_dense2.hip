@@ -805,6 +805,8 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
       // run between the same two barriers (double-buffered tile data)
       const int jj_first = (ch == 1) ? 0 : ii0;
       int cur = 0;
+      // range of the item's x1 for the triple screen
+      const double xlo = s.xs[0], xhi = s.xs[NIJ - 1], xabs = fmax(fabs(xlo), fabs(xhi));
       for (int base = jj_first; base < m.nptk; base += NT_DENSE) {
         const int jj = base + tid;
         int ok = 0, ssv = 0;
@@ -813,7 +815,7 @@ __global__ void __launch_bounds__(NT_DENSE) k_dense2(DenseArgs a) {
 #pragma unroll
           for (int t = 0; t < 3; t++) n3[t] = m.ijk[3 * jj + t];
           ssv = partner(m, ch, n1, n2, n3);
-          ok = (ch == 0) || (ssv >= jj);
+          ok = ((ch == 0) || (ssv >= jj)) && triple_may_survive<NB>(m, ch, jj, ssv, xlo, xhi, xabs);
         }
         int tot;
         const int pos = block_excl_scan<NW>(ok, s.wsum, tot);

@@ -600,7 +600,7 @@ class Context:
 
 
 class Batch:
-    """Several crystals' RTA_driver_4ph in one persistent launch per Nbands class (fp4_batch_*)."""
+    """Several crystals' RTA_driver_4ph in one persistent launch over all of them (fp4_batch_*)."""
 
     def __init__(self, contexts):
         self.contexts = list(contexts)

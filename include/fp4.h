@@ -410,13 +410,14 @@ int fp4_bte_operator_end(fp4_ctx *ctx);
    IFCs set, one device) are taken as they are.  fp4_batch_rta_4ph runs each
    crystal's count pass over the call's q' window once (it orders the work
    and does not depend on T), then
-   the dense form of every crystal of at most 4 atoms in one persistent
-   launch per Nbands class, over a single queue of all their work items
+   the dense form of every crystal (up to 8 atoms, Nbands 6, 12, 18 or 24)
+   in one persistent launch over a single queue of all their work items
    (channel, point, mode block, q') ordered costliest first by the prefilter
    counts; with use_graph that device work is a captured HIP graph, replayed
    while the crystals' data and T are unchanged.  precision is
-   FP4_PREC_FP32 or FP4_PREC_FP16X2 (MFMA).  Crystals of more than 4 atoms
-   run the FP64 per-process form, one after the other.  A call covers the
+   FP4_PREC_FP32 or FP4_PREC_FP16X2 (MFMA; in the FP16X2 batch the 6- and
+   8-atom crystals run FP32 MFMA).  Crystals of more than 8 atoms run the
+   FP64 per-process form, one after the other.  A call covers the
    part-th of nparts q' windows of every crystal ([part*nptk/nparts,
    (part+1)*nptk/nparts)); the rates of the parts add up to the full rates,
    so a caller that cannot wait for a whole pass cycles through the parts,

@@ -251,6 +251,36 @@ def part_cfg4():
                         rate_modes=rmode, ii_lo=2, ii_hi=2, **rates)
 
 
+def part_cfg4r():
+    """The 24-crystal batch at its drawn meshes: FP64 rates (per channel, and
+    WP4) of the three fixture modes of every crystal over the q' window
+    ii = 1..3, from the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    g = np.load(os.path.join(OUT, "cfg4.npz"))
+    nc = crystal.BATCH_SIZE
+    lo, hi = 1, 3
+
+    def rate_job(c):
+        case = crystal.batch_case(c)
+        o = Oracle(case)
+        mm = np.asarray(g["modes"][c], np.int32)
+        t1 = time.time()
+        r = o.rta_driver(modes=mm, ii_lo=lo, ii_hi=hi, nthreads=1)
+        log(f"cfg4r: crystal {c} Nb {case.nbands} modes {list(mm)} in {time.time() - t1:.1f}s")
+        return c, {k: v[mm - 1] for k, v in r.items()}
+
+    rates = {}
+    t0 = time.time()
+    order = sorted(range(nc), key=lambda c: -int(g["nbands"][c]) ** 3 * int(np.prod(g["ngrid"][c])))
+    with ThreadPoolExecutor(THREADS) as ex:
+        for c, r in ex.map(rate_job, order):
+            for k, v in r.items():
+                rates.setdefault(k, np.zeros((nc, 3)))[c] = v
+    log("cfg4r: rates took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "cfg4r.npz"), workload=4, modes=g["modes"], ii_lo=lo, ii_hi=hi,
+                        **rates)
+
+
 if __name__ == "__main__":
     parts = sys.argv[1:] or ["cfg1", "cfg2", "cfg3"]
     for p in parts:

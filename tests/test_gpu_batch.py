@@ -79,7 +79,8 @@ def _golden():
 @pytest.mark.parametrize("c", range(crystal.BATCH_SIZE))
 def test_batch_golden(c):
     """Each crystal at its own mesh: counts, phase space and list hashes of three
-    modes, FP64 rates of one mode over one q' point, and the dense forms there."""
+    modes, FP64 rates of one mode over one q' point and of all three over
+    q' = 1..3, and the dense forms there."""
     g = _golden()
     case = crystal.batch_case(c)
     assert list(case.modes.ngrid) == list(g["ngrid"][c]) and case.nbands == int(g["nbands"][c])
@@ -109,6 +110,21 @@ def test_batch_golden(c):
             got = rd["rate_scatt_4ph"].ravel(order="F")[rm - 1]
             assert nd == n64
             assert abs(got - want) <= 1e-3 * abs(want), (prec, got, want)
+        # the three fixture modes over q' = 1..3 (cfg4r.npz): FP64 and the FP32 MFMA form
+        path = os.path.join(GOLDEN, "cfg4r.npz")
+        if os.path.exists(path):
+            gr = np.load(path)
+            mm = np.asarray(gr["modes"][c])
+            lo, hi = int(gr["ii_lo"]), int(gr["ii_hi"])
+            r, _ = ctx.RTA_driver_4ph_modes(mm, rt.PREC_FP64, ii_lo=lo, ii_hi=hi)
+            for key, wk in RATE_KEYS:
+                assert_close(r[key].ravel(order="F")[mm - 1], gr[wk][c], RTOL, f"{key} q'={lo}..{hi}")
+            rd, _ = ctx.RTA_driver_4ph_modes(mm, rt.PREC_FP32, ii_lo=lo, ii_hi=hi)
+            want = gr["rate_4ph"][c]
+            got = rd["rate_scatt_4ph"].ravel(order="F")[mm - 1]
+            ok = want != 0
+            assert ok.any()
+            assert np.all(np.abs(got[ok] - want[ok]) <= 1e-3 * np.abs(want[ok])), (got, want)
     finally:
         ctx.close()
 
