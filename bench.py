@@ -644,7 +644,7 @@ def run_iterative20(args):
     ctx = rt.Context(case)
     prec = prec_of(rt, args.precision)
 
-    def solve(precision):
+    def solve(precision, check_sweep=False):
         ctx.reset_stats()
         t1 = time.perf_counter()
         # assembled in 8 pieces of whole irreducible points, a progress line each
@@ -670,16 +670,32 @@ def run_iterative20(args):
         ts = time.perf_counter() - t1
         sweep_ms, steps = ctx.kernel_stats(rt.Context.KS_ITER)
         k = res["kappa_iter"]
-        return {"processes_3ph": int(n3), "processes_4ph": int(n4), "assemble_seconds": ta,
+        extra = {}
+        if check_sweep:
+            # the same solve with one launch per (ll, i, kk) step: bit-identical F_n and kappa
+            ctx.reset_stats()
+            ctx.bte_set_sweep(False)
+            t1 = time.perf_counter()
+            res1 = ctx.bte_solve(tau, maxiter=1000)
+            ts1 = time.perf_counter() - t1
+            ctx.bte_set_sweep(True)
+            ms1, steps1 = ctx.kernel_stats(rt.Context.KS_ITER)
+            same = bool(np.array_equal(res1["F_n"], res["F_n"]) and np.array_equal(res1["kappa_iter"], k))
+            extra = {"one_launch_per_step": {"solve_seconds": ts1, "seconds_per_sweep": ms1 * 1e-3 / max(int(res1["niter"]), 1),
+                                             "launches_per_sweep": steps1 // max(int(res1["niter"]), 1),
+                                             "niter": int(res1["niter"]), "identical_to_persistent": same}}
+            log(f"  one launch per step: {ts1:.2f}s, {ms1 * 1e-3 / max(int(res1['niter']), 1):.3f} s/sweep, "
+                f"bit-identical to the persistent sweep: {same}")
+        return {**extra, "processes_3ph": int(n3), "processes_4ph": int(n4), "assemble_seconds": ta,
                 "solve_seconds": ts, "niter": int(res["niter"]), "final_relchange": float(res["relchange"][-1]),
-                "seconds_per_sweep": sweep_ms * 1e-3 / max(int(res["niter"]), 1), "steps_per_sweep": steps // max(int(res["niter"]), 1),
+                "seconds_per_sweep": sweep_ms * 1e-3 / max(int(res["niter"]), 1), "launches_per_sweep": steps // max(int(res["niter"]), 1),
                 "kappa_rta": float(np.trace(k[:, :, 0]) / 3), "kappa": float(np.trace(k[:, :, -1]) / 3),
                 "kappa_tensor": k[:, :, -1].tolist()}
 
     per_T = []
     for T in (100.0, 300.0, 700.0):
         ctx.set_temperature(T)
-        rec = {"T": T, **solve(prec)}
+        rec = {"T": T, **solve(prec, check_sweep=(T == 300.0))}
         log(f"T={T:.0f} K: operator {rec['assemble_seconds']:.1f}s ({rec['processes_4ph']} 4ph + "
             f"{rec['processes_3ph']} 3ph processes), solve {rec['solve_seconds']:.2f}s in {rec['niter']} iterations "
             f"({rec['seconds_per_sweep']:.3f} s/sweep), kappa {rec['kappa']:.4f} W/m/K (RTA {rec['kappa_rta']:.4f})")

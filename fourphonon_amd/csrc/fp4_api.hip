@@ -2082,6 +2082,13 @@ int fp4_bte_iterate(fp4_ctx *c, const double *tau_zero, double *F_n) {
   return FP4_OK;
 }
 
+int fp4_bte_set_sweep(fp4_ctx *c, int32_t persistent) {
+  int rc;
+  if ((rc = bte_ready(c))) return rc;
+  BTE_CALL(bte_set_persistent(c->bte, persistent != 0));
+  return FP4_OK;
+}
+
 int fp4_iteration_4ph(fp4_ctx *c, int32_t nlist, const int32_t *Nequi, const int32_t *ALLEquiList,
                       const int32_t *TypeofSymmetry, const int32_t *N_plus, const int32_t *N_minus,
                       int32_t Ntotal_plus, int32_t Ntotal_minus, const int64_t *N_pp, const int64_t *N_pm,

@@ -104,6 +104,8 @@ struct BteState {
   DBuf F, stage, part, ticket, kappa, mode, tau, hist, rel, flag;
   DBuf W;                      // aggregated operator [Nb*Nlist][nptk*Nb]
   bool use_w = false;
+  bool persistent = true;      // aggregated operator: one cooperative launch per sweep
+  DBuf steps, taus, bar;       // persistent sweep: step table, tau per step, barrier counter and error flag
   long long wcols = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -222,6 +224,133 @@ __global__ void __launch_bounds__(kStepThreads) k_iter_step(IterStep s) {
     }
     for (int a = 0; a < 3; a++) FQ(s.F, s.nb, s.qk, b, a) = y[a];
   }
+}
+
+// A whole iteration_4ph sweep with the aggregated operator in one
+// cooperative launch of G blocks (the G of k_iter_step for this operator):
+// the (ll, i, kk) steps run in order, each block computing the same
+// partial sums as k_iter_step's block of that index.  A grid barrier per
+// step (a monotonic arrival counter, device-scope fences around it) makes
+// the partials visible; every block then adds them in block order, as
+// k_iter_step's last block does, and writes the same F_n(:, q_kk, :)
+// update itself, so the next step's gathers read the new values without a
+// second barrier (every block writes bit-identical values).  The old row
+// F_n(:, q_kk, :) the symmetriser reads is loaded before the block arrives
+// at the step's barrier, when no block can have written the step's update
+// yet; after the barrier a block computes only from that copy and the
+// partials (reading the row again could see another block's finished update
+// and apply the symmetriser to it twice).  Partials are double-buffered by
+// step parity.  The barrier waits are bounded: after
+// spin_limit polls a block sets *err and every block leaves at its next
+// barrier.  Results are bitwise those of one k_iter_step launch per step.
+struct SweepW {
+  const int4 *steps;      // per step: W row (ll*Nb + i), i, q_kk, symmetry operation (0-based)
+  const double *tau;      // tau_zero of the step's mode
+  int nsteps;
+  const double *W;
+  long long wcols;
+  const int *ideq;
+  const double *symm, *omega, *vel;
+  double *F;
+  double *part;           // [2][G][3]
+  unsigned int *bar;      // arrivals, 0 at launch
+  int *err;               // set when a barrier wait exceeds spin_limit
+  int nb, nptk, nsymm;
+  long long spin_limit;
+};
+
+__device__ __forceinline__ bool sweep_barrier(const SweepW &a, unsigned target) {
+  __threadfence();
+  atomicAdd(a.bar, 1u);
+  long long n = 0;
+  while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (++n > a.spin_limit) {
+      atomicExch(a.err, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __threadfence();
+  return true;
+}
+
+constexpr int kSweepMaxNb = 96;   // branches of one F_n row a block keeps in LDS
+
+__global__ void __launch_bounds__(kStepThreads) k_sweep_w(SweepW a) {
+  __shared__ double red[kStepThreads / 64][3];
+  __shared__ double row0[kSweepMaxNb * 3];   // F_n(:, q_kk, :) before the step
+  __shared__ int stop;
+  const int tid = threadIdx.x;
+  const int G = (int)gridDim.x;
+  const long long stride = (long long)G * kStepThreads;
+  const long long me = (long long)blockIdx.x * kStepThreads + tid;
+  for (int st = 0; st < a.nsteps; st++) {
+    const int4 sp = a.steps[st];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const double *row = a.W + (size_t)sp.x * (size_t)a.wcols;
+    for (long long c = me; c < a.wcols; c += stride) {
+      const double w = row[c];
+      if (w == 0.0) continue;
+      const int q = (int)(c / a.nb), b = (int)(c - (long long)q * a.nb);
+      const int qr = a.ideq[(size_t)sp.w + (size_t)a.nsymm * q] - 1;
+      const double *Fp = &FQ(a.F, a.nb, qr, b, 0);
+#pragma unroll
+      for (int x = 0; x < 3; x++) acc[x] = acc[x] + w * Fp[x];
+    }
+#pragma unroll
+    for (int x = 0; x < 3; x++) acc[x] = wave_sum(acc[x]);
+    if ((tid & 63) == 0)
+#pragma unroll
+      for (int x = 0; x < 3; x++) red[tid >> 6][x] = acc[x];
+    __syncthreads();
+    if (tid == 0) {
+      double *part = a.part + (size_t)(st & 1) * 3 * G;
+      for (int x = 0; x < 3; x++) {
+        double v = red[0][x];
+        for (int w = 1; w < kStepThreads / 64; w++) v = v + red[w][x];
+        part[3 * (size_t)blockIdx.x + x] = v;
+      }
+      for (int b = 0; b < a.nb; b++)
+        for (int x = 0; x < 3; x++) row0[3 * b + x] = FQ(a.F, a.nb, sp.z, b, x);
+      stop = sweep_barrier(a, (unsigned)(st + 1) * (unsigned)G) ? 0 : 1;
+      if (!stop) {
+        double dF[3];
+        for (int x = 0; x < 3; x++) {
+          double v = 0.0;
+          for (int b = 0; b < G; b++) v = v + part[3 * (size_t)b + x];
+          dF[x] = v;
+        }
+        // F_n(i, q_kk, :) = tau_zero*v*omega + tau_zero*DeltaF   (iterations.f90:206-207)
+        const int i = sp.y, qk = sp.z;
+        const double tau = a.tau[st];
+        const double w = a.omega[(size_t)i * a.nptk + qk];
+        for (int x = 0; x < 3; x++) {
+          const double v = a.vel[((size_t)x * a.nb + i) * a.nptk + qk];
+          row0[3 * i + x] = tau * v * w + tau * dF[x];
+        }
+        // F_n(:, q_kk, :) <- symmetrizers(:,:,q_kk) applied to every branch   (iterations.f90:208)
+        const double *S = a.symm + 9 * (size_t)qk;
+        for (int b = 0; b < a.nb; b++) {
+          double xv[3], y[3];
+          for (int x = 0; x < 3; x++) xv[x] = row0[3 * b + x];
+          for (int x = 0; x < 3; x++) {
+            double t = 0.0;
+            for (int c = 0; c < 3; c++) t = t + S[x + 3 * c] * xv[c];
+            y[x] = t;
+          }
+          for (int x = 0; x < 3; x++) FQ(a.F, a.nb, qk, b, x) = y[x];
+        }
+      }
+    }
+    __syncthreads();
+    if (stop) return;
+  }
+}
+
+__global__ void k_zero_u32(unsigned int *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
 }
 
 // iteration0: block per irreducible point, threads over (branch, star member).
@@ -691,6 +820,65 @@ int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero) {
   st.W = s->use_w ? s->W.as<double>() : nullptr;
   st.wcols = s->wcols;
   long long launches = 0;
+  if (s->use_w && s->persistent && m.nb <= kSweepMaxNb) {
+    // the whole sweep in one cooperative launch (k_sweep_w)
+    std::vector<int4> steps;
+    std::vector<double> taus;
+    steps.reserve((size_t)m.nptk * m.nb);
+    for (int ll = 0; ll < m.nlist; ll++)
+      for (int i = 0; i < m.nb; i++)
+        for (int kk = 0; kk < s->h_nequi[ll]; kk++) {
+          const size_t at = (size_t)kk + (size_t)s->nsymm * ll;
+          steps.push_back(make_int4(ll * m.nb + i, i, s->h_alleq[at] - 1, s->h_type[at] - 1));
+          taus.push_back(tau_zero[i + (size_t)m.nb * ll]);
+        }
+    const long long want = (s->wcols + kStepThreads * kPerThread - 1) / (kStepThreads * kPerThread);
+    const int G = (int)std::min<long long>(std::max<long long>(want, 1), kMaxStepBlocks);
+    int dev = 0, ncu = 0, per_cu = 0;
+    BCHECK(hipGetDevice(&dev));
+    BCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    BCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_sweep_w),
+                                                        kStepThreads, 0));
+    if ((long long)per_cu * ncu < G) return bfail(FP4_ERR_DEVICE, "sweep: %d blocks cannot be co-resident", G);
+    if ((rc = s->steps.ensure(sizeof(int4) * steps.size())) || (rc = s->taus.ensure(sizeof(double) * taus.size())) ||
+        (rc = s->bar.ensure(2 * sizeof(unsigned int))) || (rc = s->part.ensure(sizeof(double) * 6 * (size_t)G)))
+      return rc;
+    BCHECK(hipMemcpyAsync(s->steps.p, steps.data(), sizeof(int4) * steps.size(), hipMemcpyHostToDevice, m.stream));
+    BCHECK(hipMemcpyAsync(s->taus.p, taus.data(), sizeof(double) * taus.size(), hipMemcpyHostToDevice, m.stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, m.stream, s->bar.as<unsigned int>(), 2);
+    SweepW sw{};
+    sw.steps = s->steps.as<int4>();
+    sw.tau = s->taus.as<double>();
+    sw.nsteps = (int)steps.size();
+    sw.W = s->W.as<double>();
+    sw.wcols = s->wcols;
+    sw.ideq = s->ideq.as<int>();
+    sw.symm = s->symm.as<double>();
+    sw.omega = m.omega;
+    sw.vel = m.vel;
+    sw.F = s->F.as<double>();
+    sw.part = s->part.as<double>();
+    sw.bar = s->bar.as<unsigned int>();
+    sw.err = reinterpret_cast<int *>(s->bar.as<unsigned int>() + 1);
+    sw.nb = m.nb;
+    sw.nptk = m.nptk;
+    sw.nsymm = s->nsymm;
+    sw.spin_limit = 1LL << 24;
+    BCHECK(hipEventRecord(s->ev0, m.stream));
+    void *args[] = {&sw};
+    BCHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_sweep_w), dim3(G), dim3(kStepThreads), args,
+                                      0, m.stream));
+    BCHECK(hipEventRecord(s->ev1, m.stream));
+    int herr = 0;
+    BCHECK(hipMemcpyAsync(&herr, sw.err, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    BCHECK(hipStreamSynchronize(m.stream));
+    if (herr) return bfail(FP4_ERR_DEVICE, "sweep: a grid barrier wait timed out");
+    float ms = 0.f;
+    BCHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms;
+    s->last_launches = 1;
+    return 0;
+  }
   BCHECK(hipEventRecord(s->ev0, m.stream));
   for (int ll = 0; ll < m.nlist; ll++)
     for (int i = 0; i < m.nb; i++) {
@@ -725,6 +913,12 @@ int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero) {
   BCHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->last_ms = ms;
   s->last_launches = launches;
+  return 0;
+}
+
+int bte_set_persistent(BteState *s, bool on) {
+  if (!s) return bfail(FP4_ERR_STATE, "no iteration state");
+  s->persistent = on;
   return 0;
 }
 

@@ -66,6 +66,10 @@ int bte_w_begin(BteState *s, const BteModes &m, double **W, long long *cols);
 int bte_w_finish(BteState *s, const BteModes &m);
 int bte_iteration0(BteState *s, const BteModes &m, const double *tau_zero);  // tau_zero(Nb,Nlist) host
 int bte_sweep(BteState *s, const BteModes &m, const double *tau_zero);       // one iteration_4ph
+// With the aggregated operator a sweep is one cooperative launch (on, the
+// default) or one launch per (ll, i, kk) step (off); the results are the same
+// bit for bit.
+int bte_set_persistent(BteState *s, bool on);
 int bte_symmetrize_all(BteState *s, const BteModes &m);                      // F_n(:,q,:) for every q
 int bte_get_F(BteState *s, const BteModes &m, double *F_n);                  // F_n(Nb,nptk,3) host
 int bte_put_F(BteState *s, const BteModes &m, const double *F_n);

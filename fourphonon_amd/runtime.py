@@ -41,7 +41,7 @@ EXPORTS = [
     "fp4_ctx_kernel_stats",
     "fp4_ctx_reset_stats", "fp4_ctx_set_batch_capacity", "fp4_ctx_dense_info", "fp4_ctx_dense_shape", "fp4_ctx_dense_kernel", "fp4_ctx_dense_candidates", "fp4_ctx_set_dense_shape", "fp4_ctx_dense_phase_profile",
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
-    "fp4_bte_iterate", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
+    "fp4_bte_iterate", "fp4_bte_set_sweep", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
     "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
     "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense", "fp4_bte_assemble_operator",
@@ -413,6 +413,10 @@ class Context:
         _check(load().fp4_iteration_4ph(self.h, *args, _p(tau), _p(F)))
         del keep
         return F
+
+    def bte_set_sweep(self, persistent):
+        """Aggregated-operator sweeps as one cooperative launch (True) or one launch per step (False)."""
+        _check(load().fp4_bte_set_sweep(self.h, C.c_int32(int(bool(persistent)))))
 
     def bte_iterate(self, tau_zero, F_n):
         tau = np.asfortranarray(tau_zero, np.float64).reshape((self.nb, self.nlist), order="F")

@@ -437,6 +437,13 @@ int fp4_batch_rta_4ph(fp4_batch *b, int32_t precision, int32_t use_graph, int32_
    fp4_bte_set_operator / fp4_iteration_4ph call; F_n updated in place. */
 int fp4_bte_iterate(fp4_ctx *ctx, const double *tau_zero, double *F_n);
 
+/* No counterpart in the reference (a launch choice of its sweep): with the
+   aggregated operator (fp4_bte_assemble_operator) a sweep of iteration_4ph
+   is one cooperative launch over all (ll, i, kk) steps with a grid barrier
+   per step (persistent = 1, the default) or one launch per step
+   (persistent = 0).  Both give the same F_n bit for bit. */
+int fp4_bte_set_sweep(fp4_ctx *ctx, int32_t persistent);
+
 /* TConduct, Src/conductivity.f90:11-37, at the context's T.  V is the cell
    volume in nm^3 (the reference's module global V).  ThConductivity(Nb,3,3)
    and, unless NULL, ThConductivityMode(nptk,Nb,3,3) are overwritten. */

@@ -330,3 +330,47 @@ def test_operator_assembled_in_pieces_equals_one_call(ctx):
     assert rel(r3, r3a) <= 1e-13 and rel(r4, r4a) <= 1e-13
     assert pieces["niter"] == one["niter"]
     assert rel(pieces["kappa_iter"], one["kappa_iter"]) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh", [(4, 4, 4), (8, 8, 8), (12, 12, 12), (16, 16, 16)],
+                         ids=["4cube", "8cube", "12cube", "16cube"])
+def test_persistent_sweep_equals_one_launch_per_step(mesh):
+    """With the aggregated operator, a sweep in one cooperative launch (a
+    grid barrier per (ll, i, kk) step; 1, 2, 6 and 12 blocks at these
+    meshes) gives F_n bit for bit equal to one k_iter_step launch per step,
+    over two sweeps and over the whole convergence loop, run twice, and
+    counts one launch per sweep."""
+    case = crystal.config_case(1, ngrid=mesh)
+    banner(f"persistent sweep vs one launch per step, diamond {mesh}")
+    c = rt.Context(case)
+    try:
+        n3 = c.NP_driver()
+        _, _, R3 = c.Ind_driver(n3[0], n3[2])
+        r4 = c.RTA_driver_4ph(rt.PREC_FP16X2)["rate_scatt_4ph"]
+        rate = R3["rate_scatt"] + r4
+        tau = np.asfortranarray(np.where(rate > 0, 1.0 / np.where(rate > 0, rate, 1.0), 0.0))
+        F0 = c.iteration0(tau)               # (sets the orbits without the symmetry types)
+        c.bte_assemble_operator(rt.PREC_FP16X2)
+        out = {}
+        for persistent in (False, True, True):
+            c.bte_set_sweep(persistent)
+            c.reset_stats()
+            F1 = c.bte_iterate(tau, F0)
+            F2 = c.bte_iterate(tau, F1)
+            ms, launches = c.kernel_stats(4)
+            sol = c.bte_solve(tau, maxiter=60)
+            if persistent in out:
+                # the second persistent run repeats the first bit for bit
+                r1 = out[persistent]
+                assert np.array_equal(r1[0], F1) and np.array_equal(r1[1], F2)
+                assert np.array_equal(r1[2]["F_n"], sol["F_n"])
+            out[persistent] = (F1, F2, sol, launches, ms)
+        (a1, a2, sa, na, ta), (b1, b2, sb, nb_, tb) = out[False], out[True]
+        print(f"two sweeps: {na} launches {ta:.2f} ms / {nb_} launches {tb:.2f} ms; niter {sb['niter']}", flush=True)
+        assert np.abs(a1 - F0).max() > 0
+        assert np.array_equal(a1, b1) and np.array_equal(a2, b2)
+        assert sa["niter"] == sb["niter"] and sb["niter"] > 1
+        assert np.array_equal(sa["F_n"], sb["F_n"]) and np.array_equal(sa["kappa_iter"], sb["kappa_iter"])
+        assert nb_ == 2 and na == 2 * case.modes.nptk * case.nbands
+    finally:
+        c.close()
