@@ -145,6 +145,21 @@ def part_cfg2lists():
                         modes=allm, N=N, P=P, H=H, seconds=time.time() - t0, threads=THREADS)
 
 
+def part_cfg3lists():
+    """cfg3 at full size: counts, phase space and list hashes of all 1140
+    modes of the 12x12x8 wurtzite in all three channels (the whole
+    6.2e10-process list, hashed)."""
+    case = crystal.config_case(3)
+    o = Oracle(case)
+    allm = np.arange(1, o.nmodes + 1, dtype=np.int32)
+    log("cfg3lists: counts + streamed hashes for", len(allm), "modes on", THREADS, "threads")
+    t0 = time.time()
+    N, P, H = streamed_counts_and_hashes(o, allm)
+    log("cfg3lists: took", time.time() - t0)
+    np.savez_compressed(os.path.join(OUT, "cfg3_lists.npz"), ngrid=np.array([12, 12, 8]), shell=2, workload=3,
+                        modes=allm, N=N, P=P, H=H, seconds=time.time() - t0, threads=THREADS)
+
+
 def part_iter6():
     """The joint 3ph+4ph iterative solution on the 6x6x6 diamond (1st-shell
     FC3/FC4, 300 K): the oracle's counts, tau0, iteration count, kappa per

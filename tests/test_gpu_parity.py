@@ -299,6 +299,26 @@ def test_golden_diamond_16cube_shell2_all_lists():
     c.close()
 
 
+def test_golden_wurtzite_12x12x8_shell2_all_lists():
+    """Workload 3 at full size: the counts and order-sensitive hashes of the
+    lists of all 1140 modes of the 12x12x8 wurtzite (Nb = 12) in all three
+    channels equal the oracle's, and the phase space agrees to 1e-10."""
+    g = _golden("cfg3_lists")
+    banner("golden wurtzite 12x12x8 shell 2: all 1140 modes, counts and list hashes")
+    c = rt.Context(crystal.config_case(3))
+    modes = g["modes"]
+    assert len(modes) == c.nmodes
+    N, H = c.list_hash_4ph_modes(modes)
+    assert np.array_equal(N, g["N"])
+    bad = [(ch, int(modes[t])) for ch in range(3) for t in np.nonzero(H[ch] != g["H"][ch])[0]]
+    assert not bad, f"{len(bad)} list hashes differ, first {bad[:5]}"
+    Nd = c.NP_driver_4ph()
+    for ch in range(3):
+        assert np.array_equal(Nd[2 * ch], g["N"][ch]), ch
+        assert_close(Nd[2 * ch + 1].ravel(order="F"), g["P"][ch], RTOL, f"P {ch}")
+    c.close()
+
+
 def test_golden_wurtzite_12x12x8_shell2():
     g = _golden("cfg3")
     banner("golden wurtzite 12x12x8 shell 2 (workload 3, Nb = 12)")
