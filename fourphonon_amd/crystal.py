@@ -299,7 +299,58 @@ def _keys(sites):
     return [tuple(x) for x in sites.reshape(len(sites), -1)]
 
 
-def make_ifc4(st: Structure, shell: int, seed: int, sym_iters: int = 400) -> IFC4:
+def space_group_ops(st: Structure, tol: float = 1e-6):
+    """Space-group operations x -> M x + t (fractional) of the crystal, each
+    with the image of every basis atom: M pos[I] + t = pos[amap[I]] + shift[I].
+    One translation per rotation (primitive cells)."""
+    lat = st.lattvec
+    ops = []
+    for M in point_group(st, tol):
+        mp = st.positions @ M.T
+        for J in range(st.natoms):
+            if st.types[J] != st.types[0]:
+                continue
+            t = st.positions[J] - mp[0]
+            amap, shift = [], []
+            for I in range(st.natoms):
+                x = mp[I] + t
+                d = st.positions - x
+                dr = d - np.round(d)
+                hit = np.nonzero((np.linalg.norm(dr @ lat.T, axis=1) < 1e-6) & (st.types == st.types[I]))[0]
+                if hit.size == 0:
+                    break
+                amap.append(int(hit[0]))
+                shift.append(np.round(x - st.positions[hit[0]]).astype(np.int64))
+            if len(amap) == st.natoms:
+                ops.append((M, t, np.array(amap), np.array(shift)))
+                break
+    return ops
+
+
+def _space_group_projector(st: Structure, sites):
+    """(maps, rots): for every space-group operation g, the index of the image
+    quartet/triplet g.k of every cluster k (translated so its first site is in
+    cell 0) and the Cartesian rotation of g.  The cluster set must be closed
+    under the group (it is: clusters are chosen by distances)."""
+    index = {k: i for i, k in enumerate(_keys(sites))}
+    lat = st.lattvec
+    maps, rots = [], []
+    for M, t, amap, shift in space_group_ops(st):
+        img = sites.copy()
+        atoms = sites[:, :, 0]
+        img[:, :, 0] = amap[atoms]
+        img[:, :, 1:] = shift[atoms] + np.einsum("ab,ksb->ksa", M, sites[:, :, 1:])
+        maps.append(np.array([index[k] for k in _keys(_canon(img))], dtype=np.int64))
+        rots.append(lat @ M @ np.linalg.inv(lat))
+    return maps, rots
+
+
+def make_ifc4(st: Structure, shell: int, seed: int, sym_iters: int = 400, space_group: bool = False) -> IFC4:
+    """Seeded N(0,1) fourth-order IFCs on every quartet within the shell's
+    cutoff, made symmetric under permutations of the four sites and
+    acoustic-sum-rule corrected (alternating projections); with space_group
+    also invariant under the crystal's space group, Phi(g.k) = R_g x4 Phi(k),
+    so that rates are equal over every star."""
     rc = cluster_cutoff(st, shell)
     sites = _canon(enumerate_quartets(st, rc))
     # deterministic order: by first atom, then lexicographic on the rest
@@ -334,8 +385,21 @@ def make_ifc4(st: Structure, shell: int, seed: int, sym_iters: int = 400) -> IFC
         np.add.at(s, groups, x)
         return x - (s / counts[:, None, None, None, None])[groups]
 
+    if space_group:
+        gmaps, grots = _space_group_projector(st, sites)
+
+        def spacegroup(x):
+            # Phi(k) <- average over g of R_g^T x4 Phi(g.k)
+            acc = np.zeros_like(x)
+            for gm, R in zip(gmaps, grots):
+                acc += np.einsum("ia,jb,kc,ld,nijkl->nabcd", R, R, R, R, x[gm], optimize=True)
+            return acc / len(gmaps)
+    else:
+        def spacegroup(x):
+            return x
+
     for _ in range(sym_iters):
-        phi = symmetrise(asr(phi))
+        phi = spacegroup(symmetrise(asr(phi)))
         s = np.zeros((len(counts), 3, 3, 3, 3))
         np.add.at(s, groups, phi)
         if np.abs(s).max() < 1e-13 * max(1.0, np.abs(phi).max()):

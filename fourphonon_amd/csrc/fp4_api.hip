@@ -124,6 +124,15 @@ struct fp4_ctx {
   std::vector<double> masses;
   std::vector<int> types;
   bool have_modes = false, have_ifc = false, have_ifc3 = false;
+  bool have_eig = false;         // eigenvectors given for the current mode data
+  std::vector<double> h_energy, h_vel;   // host copies of the mode data on the device
+  std::vector<int32_t> h_ijk, h_list;
+  // the last 4ph count pass (per-item counts and phase-space terms in cnt /
+  // psum), reused while its modes and q' window are asked for again and no
+  // other pass or new mode data has touched those buffers
+  bool cnt_valid = false;
+  std::vector<int> cnt_modes;
+  int cnt_lo0 = 0, cnt_nii = 0;
   // mode data
   DevBuf energy, vel, vbound, eig, ijk, list;
   // clusters
@@ -238,6 +247,9 @@ int count_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
   pl.items = 3L * nsub * pl.nii;
   const int nseg = 3 * nsub;
   int rc;
+  if (c->cnt_valid && c->cnt_lo0 == pl.ii_lo0 && c->cnt_nii == pl.nii && c->cnt_modes == pl.modes)
+    return finish_counts(c, pl, nseg, want_offsets);
+  c->cnt_valid = false;
   if ((rc = c->modes_sub.ensure(sizeof(int) * std::max(nsub, 1)))) return rc;
   if ((rc = c->cnt.ensure(sizeof(int) * std::max(pl.items, 1L)))) return rc;
   if ((rc = c->psum.ensure(sizeof(double) * std::max(pl.items, 1L)))) return rc;
@@ -258,7 +270,12 @@ int count_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
          return launch_prefilter(a, 0, c->stream) ? fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", c->nb) : 0;
        })))
     return rc;
-  return finish_counts(c, pl, nseg, want_offsets);
+  if ((rc = finish_counts(c, pl, nseg, want_offsets))) return rc;
+  c->cnt_valid = true;
+  c->cnt_modes = pl.modes;
+  c->cnt_lo0 = pl.ii_lo0;
+  c->cnt_nii = pl.nii;
+  return 0;
 }
 
 // Per-segment totals of the count pass and, when asked, the exclusive item
@@ -424,6 +441,7 @@ int rate_batch(fp4_ctx *c, const Plan &pl, const Batch &b, bool wide, bool store
 // ------------------------------------------------------- three phonons
 // Items are (channel, mode-in-subset, j); the q' window is win_lo0/win_n.
 int count3_pass(fp4_ctx *c, Plan &pl, bool want_offsets) {
+  c->cnt_valid = false;   // cnt / psum are this pass's now
   const int nsub = (int)pl.modes.size();
   pl.nii = c->nb;
   pl.items = 2L * nsub * c->nb;
@@ -513,6 +531,7 @@ int check_ready3(const fp4_ctx *c, bool need_ifc3) {
   if (!c) return fail(FP4_ERR_ARG, "null context");
   if (!c->have_modes) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes has not been called");
   if (need_ifc3 && !c->have_ifc3) return fail(FP4_ERR_STATE, "fp4_ctx_set_ifc3 has not been called");
+  if (need_ifc3 && !c->have_eig) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes was called without eigenvectors");
   HIPCHECK(hipSetDevice(c->device));
   return 0;
 }
@@ -552,6 +571,7 @@ int check_ready(const fp4_ctx *c, bool need_ifc) {
   if (!c) return fail(FP4_ERR_ARG, "null context");
   if (!c->have_modes) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes has not been called");
   if (need_ifc && !c->have_ifc) return fail(FP4_ERR_STATE, "fp4_ctx_set_ifc4 has not been called");
+  if (need_ifc && !c->have_eig) return fail(FP4_ERR_STATE, "fp4_ctx_set_modes was called without eigenvectors");
   HIPCHECK(hipSetDevice(c->device));
   return 0;
 }
@@ -657,7 +677,7 @@ int fp4_ctx_set_temperature(fp4_ctx *c, double T) {
 
 int fp4_ctx_set_modes(fp4_ctx *c, const double *energy, const double *velocity, const double *eigenvect,
                       const int32_t *IJK, int32_t nlist, const int32_t *List) {
-  if (!c || !energy || !velocity || !eigenvect || !IJK || !List) return fail(FP4_ERR_ARG, "null argument");
+  if (!c || !energy || !velocity || !IJK || !List) return fail(FP4_ERR_ARG, "null argument");
   if (nlist < 1 || nlist > c->nptk) return fail(FP4_ERR_ARG, "Nlist out of range");
   for (int t = 0; t < nlist; t++)
     if (List[t] < 1 || List[t] > c->nptk) return fail(FP4_ERR_ARG, "List(%d)=%d out of range", t + 1, List[t]);
@@ -666,20 +686,45 @@ int fp4_ctx_set_modes(fp4_ctx *c, const double *energy, const double *velocity, 
       if (IJK[3 * q + t] < 0 || IJK[3 * q + t] >= c->ngrid[t]) return fail(FP4_ERR_ARG, "IJK out of range");
   HIPCHECK(hipSetDevice(c->device));
   const size_t np = c->nptk, nb = c->nb;
+  // the reference's drivers take the mode data as arguments, so a caller
+  // passes the same arrays again and again: unchanged frequencies,
+  // velocities, grid and irreducible list stay on the device (and keep the
+  // cached count pass valid); only eigenvectors, when given, are uploaded
+  const bool same = c->have_modes && c->nlist == nlist && c->h_energy.size() == np * nb &&
+                    std::memcmp(c->h_energy.data(), energy, sizeof(double) * np * nb) == 0 &&
+                    std::memcmp(c->h_vel.data(), velocity, sizeof(double) * np * nb * 3) == 0 &&
+                    std::memcmp(c->h_ijk.data(), IJK, sizeof(int32_t) * 3 * np) == 0 &&
+                    std::memcmp(c->h_list.data(), List, sizeof(int32_t) * nlist) == 0;
+  if (same) {
+    if (eigenvect) {
+      std::vector<double> eig(2 * np * nb * nb);
+      for (size_t comp = 0; comp < nb; comp++)
+        for (size_t b = 0; b < nb; b++)
+          for (size_t q = 0; q < np; q++) {
+            const size_t src = q + np * (b + nb * comp), dst = (q * nb + b) * nb + comp;
+            eig[2 * dst] = eigenvect[2 * src];
+            eig[2 * dst + 1] = eigenvect[2 * src + 1];
+          }
+      HIPCHECK(hipMemcpy(c->eig.p, eig.data(), sizeof(double) * 2 * np * nb * nb, hipMemcpyHostToDevice));
+      c->have_eig = true;
+    }
+    return FP4_OK;
+  }
   int rc;
   if ((rc = c->energy.ensure(sizeof(double) * np * nb)) || (rc = c->vel.ensure(sizeof(double) * np * nb * 3)) ||
       (rc = c->vbound.ensure(sizeof(float) * np * nb)) || (rc = c->eig.ensure(sizeof(double) * 2 * np * nb * nb)) ||
       (rc = c->ijk.ensure(sizeof(int) * 3 * np)) || (rc = c->list.ensure(sizeof(int) * nlist)))
     return rc;
   // eigenvectors: reference (q, b, comp) with q fastest -> [q][b][comp]
-  std::vector<double> eig(2 * np * nb * nb);
-  for (size_t comp = 0; comp < nb; comp++)
-    for (size_t b = 0; b < nb; b++)
-      for (size_t q = 0; q < np; q++) {
-        const size_t src = q + np * (b + nb * comp), dst = (q * nb + b) * nb + comp;
-        eig[2 * dst] = eigenvect[2 * src];
-        eig[2 * dst + 1] = eigenvect[2 * src + 1];
-      }
+  std::vector<double> eig(eigenvect ? 2 * np * nb * nb : 0);
+  if (eigenvect)
+    for (size_t comp = 0; comp < nb; comp++)
+      for (size_t b = 0; b < nb; b++)
+        for (size_t q = 0; q < np; q++) {
+          const size_t src = q + np * (b + nb * comp), dst = (q * nb + b) * nb + comp;
+          eig[2 * dst] = eigenvect[2 * src];
+          eig[2 * dst + 1] = eigenvect[2 * src + 1];
+        }
   // conservative broadening bound per mode (see fp4_prefilter.hip)
   double bm = 0.0;
   for (int nu = 0; nu < 3; nu++) {
@@ -698,11 +743,17 @@ int fp4_ctx_set_modes(fp4_ctx *c, const double *energy, const double *velocity, 
   HIPCHECK(hipMemcpy(c->energy.p, energy, sizeof(double) * np * nb, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->vel.p, velocity, sizeof(double) * np * nb * 3, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->vbound.p, vb.data(), sizeof(float) * np * nb, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->eig.p, eig.data(), sizeof(double) * 2 * np * nb * nb, hipMemcpyHostToDevice));
+  if (eigenvect) HIPCHECK(hipMemcpy(c->eig.p, eig.data(), sizeof(double) * 2 * np * nb * nb, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->ijk.p, IJK, sizeof(int) * 3 * np, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(c->list.p, List, sizeof(int) * nlist, hipMemcpyHostToDevice));
   c->nlist = nlist;
   c->have_modes = true;
+  c->have_eig = eigenvect != nullptr;
+  c->cnt_valid = false;
+  c->h_energy.assign(energy, energy + np * nb);
+  c->h_vel.assign(velocity, velocity + np * nb * 3);
+  c->h_ijk.assign(IJK, IJK + 3 * np);
+  c->h_list.assign(List, List + nlist);
   return FP4_OK;
 }
 
@@ -1218,6 +1269,7 @@ static int sample_launch(const SampleArgs &a, int nch, int mode, hipStream_t st)
 }
 
 static int sample_count_pass(fp4_ctx *c, Plan &pl, int32_t nsample, uint64_t seed, int nch) {
+  c->cnt_valid = false;   // cnt / psum are this pass's now
   const int nsub = (int)pl.modes.size();
   pl.ii_lo0 = 0;
   pl.nii = 1;

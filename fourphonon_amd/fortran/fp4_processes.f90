@@ -79,7 +79,7 @@ module fp4_processes
        import :: c_int, c_ptr, c_double, c_double_complex, c_int32_t
        type(c_ptr), value :: ctx
        real(c_double), intent(in) :: energy(*), velocity(*)
-       complex(c_double_complex), intent(in) :: eigenvect(*)
+       complex(c_double_complex), intent(in), optional :: eigenvect(*)   ! absent: counts and lists only
        integer(c_int32_t), intent(in) :: IJK(*), List(*)
        integer(c_int32_t), value :: nlist
      end function c_set_modes
@@ -421,15 +421,12 @@ contains
     call check(c_set_ifc4(ctx, Ntri, Phi, R_j, R_k, R_l, Index_i, Index_j, Index_k, Index_l), "fp4_ctx_set_ifc4")
   end subroutine upload_ifc4
 
-  ! NP_driver_4ph has no eigenvectors; zeros stand in until a rate driver
-  ! uploads the real ones.
+  ! NP_driver_4ph and Ind_only_driver_4ph have no eigenvectors: the mode data
+  ! go up without them (a null pointer on the C side)
   subroutine upload_modes_noeig(energy, velocity, Nlist, List, IJK)
     real(kind=8), intent(in) :: energy(:,:), velocity(:,:,:)
     integer(kind=4), intent(in) :: Nlist, List(:), IJK(:,:)
-    complex(kind=8), allocatable :: zero(:,:,:)
-    allocate(zero(nptk_, nbands_, nbands_))
-    zero = (0.d0, 0.d0)
-    call upload_modes(energy, velocity, zero, Nlist, List, IJK)
+    call check(c_set_modes(ctx, energy, velocity, IJK=IJK, nlist=Nlist, List=List), "fp4_ctx_set_modes")
   end subroutine upload_modes_noeig
 
   subroutine NP_driver_4ph(energy, velocity, Nlist, List, IJK, &

@@ -414,6 +414,16 @@ class Context:
         del keep
         return F
 
+    def set_modes(self, modes=None, eigenvectors=True):
+        """fp4_ctx_set_modes with the case's (or the given) mode data; eigenvectors=False passes none
+        (counts and lists only)."""
+        m = self.case.modes if modes is None else modes
+        ev = _p(np.asfortranarray(m.eigenvect, np.complex128)) if eigenvectors else None
+        _check(load().fp4_ctx_set_modes(self.h, _p(np.asfortranarray(m.energy, np.float64)),
+                                        _p(np.asfortranarray(m.velocity, np.float64)), ev,
+                                        _p(np.asfortranarray(m.IJK, np.int32)), C.c_int32(m.Nlist),
+                                        _p(np.ascontiguousarray(m.List, np.int32))))
+
     def bte_set_sweep(self, persistent):
         """Aggregated-operator sweeps as one cooperative launch (True) or one launch per step (False)."""
         _check(load().fp4_bte_set_sweep(self.h, C.c_int32(int(bool(persistent)))))

@@ -69,12 +69,20 @@ const char *fp4_last_error(void);
 
 int fp4_ctx_create(const fp4_config *cfg, fp4_ctx **out);
 int fp4_ctx_destroy(fp4_ctx *ctx);
-/* T and scalebroad may change between calls (temperature loop of
+/* T may change between calls (temperature loop of
    Src/ShengBTE.f90:787-1469) without re-uploading anything. */
 int fp4_ctx_set_temperature(fp4_ctx *ctx, double T);
 
 /* energy(nptk,Nb), velocity(nptk,Nb,3), eigenvect(nptk,Nb,Nb) complex,
-   IJK(3,nptk), List(Nlist) -- the arrays of Src/ShengBTE.f90:142-149. */
+   IJK(3,nptk), List(Nlist) -- the arrays of Src/ShengBTE.f90:142-149.
+   eigenvect may be NULL for callers that only count and list processes
+   (NP_driver_4ph and Ind_only_driver_4ph take no eigenvectors); the rate,
+   Vp and three-phonon rate entry points then return FP4_ERR_STATE until a
+   call with eigenvectors.  The 4ph count pass of one mode subset and q'
+   window is kept and reused by the next count or list call with the same
+   subset and window (Ind_only_driver_4ph after NP_driver_4ph), until new
+   mode data arrive; a call with the same frequencies, velocities, IJK and
+   List as the last one uploads only the eigenvectors (if given). */
 int fp4_ctx_set_modes(fp4_ctx *ctx, const double *energy, const double *velocity, const double *eigenvect,
                       const int32_t *IJK, int32_t nlist, const int32_t *List);
 

@@ -48,13 +48,14 @@ def test_driver_reports_missing_device(tmp_path):
 
 STUB_C = r"""
 #include <stdint.h>
-static int dummy, calls, nb_, np_, nl_;
+static int dummy, calls, nb_, np_, nl_, ev_null;
 const char *fp4_last_error(void) { return "stub"; }
 int fp4_ctx_create(const int32_t *cfg, void **out) { *out = &dummy; nb_ = cfg[3]; np_ = cfg[0] * cfg[1] * cfg[2]; return 0; }
 int fp4_ctx_destroy(void *c) { return 0; }
 int fp4_ctx_set_modes(void *c, const double *e, const double *v, const double *ev, const int32_t *ijk, int32_t nlist,
-                      const int32_t *L) { nl_ = nlist; return 0; }
+                      const int32_t *L) { nl_ = nlist; ev_null = (ev == 0); return 0; }
 int fp4_np_4ph(void *c, int64_t *a, double *b, int64_t *cc, double *d, int64_t *e, double *f) {
+  if (!ev_null) return 5;   /* NP_driver_4ph passes no eigenvectors (a null pointer) */
   for (int i = 0; i < nb_ * nl_; i++) { a[i] = cc[i] = e[i] = 0; b[i] = d[i] = f[i] = 0; }
   return 0;
 }

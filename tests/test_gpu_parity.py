@@ -336,3 +336,42 @@ def test_golden_wurtzite_12x12x8_shell2():
                     ("rate_scatt_minusminus", "rate_mm"), ("WP4_plusminus", "wp4_pm")):
         assert_close(r[key].ravel(order="F")[rm - 1], g[wk], RTOL, key)
     c.close()
+
+
+def test_count_pass_reused_and_modes_without_eigenvectors(case4):
+    """Ind_only_driver_4ph after NP_driver_4ph reuses the count pass (no second
+    prefilter launch) and still gives the oracle's lists; sending the same mode
+    data again keeps it, new frequencies count again.  Mode data without
+    eigenvectors allow counts and lists but not rates (FP4_ERR_STATE) until
+    eigenvectors come back."""
+    import dataclasses
+    from oracle.oracle import Oracle
+    banner("count-pass reuse and mode data without eigenvectors")
+    c = rt.Context(case4)
+    try:
+        o = Oracle(case4)
+        No, _ = o.np_driver()
+        c.reset_stats()
+        N = c.NP_driver_4ph()
+        assert c.kernel_stats(rt.Context.KS_PREFILTER)[1] == 1
+        c.set_modes(eigenvectors=False)           # the same data again, as the Fortran drivers send it
+        got = c.Ind_only_driver_4ph(*No)
+        assert c.kernel_stats(rt.Context.KS_PREFILTER)[1] == 1
+        want = o.ind_only_driver(No)
+        for ch in range(3):
+            assert np.array_equal(N[2 * ch], No[ch])
+            for t in range(3):
+                assert np.array_equal(got[ch][t], want[ch][t]), (ch, t)
+        r1 = c.RTA_driver_4ph(rt.PREC_FP64)      # eigenvectors of the same data are kept
+        m2 = dataclasses.replace(case4.modes, energy=np.asfortranarray(case4.modes.energy * (1.0 + 1e-9)))
+        c.set_modes(m2, eigenvectors=False)
+        c.NP_driver_4ph()
+        assert c.kernel_stats(rt.Context.KS_PREFILTER)[1] == 2
+        with pytest.raises(rt.FP4Error):
+            c.RTA_driver_4ph(rt.PREC_FP64)
+        c.set_modes()
+        r2 = c.RTA_driver_4ph(rt.PREC_FP64)
+        for k in r1:
+            assert np.array_equal(r1[k], r2[k]), k
+    finally:
+        c.close()

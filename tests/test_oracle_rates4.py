@@ -213,3 +213,70 @@ def test_rta_rates_rebuilt_in_numpy_other_temperature():
     """The Bose-Einstein factors enter through T: 100 K, and scalebroad 0.5."""
     case = crystal.config_case(1, ngrid=(3, 3, 3), shell=1, T=100.0, scalebroad=0.5)
     _check_rates(case, np.arange(1, case.modes.nbands * case.modes.Nlist + 1, 3), min_processes=100)
+
+
+def _degenerate_sets(w, tol=1e-7):
+    sets, b = [], 0
+    while b < len(w):
+        e = b + 1
+        while e < len(w) and abs(w[e] - w[b]) <= tol * max(1.0, abs(w[b])):
+            e += 1
+        sets.append(list(range(b, e)))
+        b = e
+    return sets
+
+
+def test_matrix_elements_covariant_under_the_space_group():
+    """With fourth-order IFCs invariant under the crystal's space group (the
+    48 operations of diamond), |Vp|^2 summed over the degenerate branch sets
+    of the four phonons (a sum that does not depend on the eigenvector basis
+    inside each set) is the same for a process (q, q', q'', q''') and for its
+    image under every operation of the grid's symmetry list (time reversal
+    included), in all three channels, to 1e-9.  This pins the q-dependence of
+    the oracle's matrix elements: the phase of each cluster, which phonon
+    takes which eigenvector and conjugate, and the partner of each channel
+    must all transform covariantly.  With the workloads' IFCs, which are only
+    permutation-symmetric, the sums differ (a negative control).  (The rates
+    themselves are not equal over a star in the reference's formulation:
+    sigma takes the largest projection on the three reciprocal basis vectors,
+    which rotations need not permute, and degenerate modes' velocities depend
+    on the basis inside their set.)"""
+    st = crystal.diamond()
+    s = crystal.SEED_BASE + 10
+    case = crystal.config_case(1, ngrid=(4, 4, 4), symmetric=False)
+    sym = crystal.Case(case.structure, case.modes, crystal.make_ifc4(st, 1, seed=s + 4, space_group=True))
+    m = case.modes
+    ngrid = np.asarray(m.ngrid)
+    ID = crystal.symmetry_tables(st, ngrid)[8]
+    E = np.asarray(m.energy)
+    rng = np.random.default_rng(3)
+    spreads = {}
+    for label, cs in (("space group", sym), ("permutation only", case)):
+        o = Oracle(cs)
+        worst = 0.0
+        for ch, (s2, s3) in enumerate([(1, 1), (1, -1), (-1, -1)]):
+            for _ in range(2):
+                q, qp, qdp = (int(x) for x in rng.integers(2, m.nptk + 1, size=3))
+                vals = []
+                for op in rng.permutation(ID.shape[0])[:12]:
+                    pts = [int(ID[op, x - 1]) for x in (q, qp, qdp)]
+                    g = (m.IJK[:, pts[0] - 1] + s2 * m.IJK[:, pts[1] - 1] + s3 * m.IJK[:, pts[2] - 1]) % ngrid
+                    pts.append(int(g[0] + ngrid[0] * (g[1] + ngrid[1] * g[2])) + 1)
+                    sets = [_degenerate_sets(E[p - 1]) for p in pts]
+                    # |V|^2 summed over each combination of degenerate sets, in frequency order
+                    tot = []
+                    for A in sets[0]:
+                        for B in sets[1]:
+                            for Cs in sets[2]:
+                                for D in sets[3]:
+                                    tot.append(sum(abs(o.vp(ch, i + 1, j + 1, k + 1, l + 1, *pts)) ** 2
+                                                   for i in A for j in B for k in Cs for l in D))
+                    vals.append(np.array(tot))
+                ref = np.abs(vals[0]).max()
+                assert ref > 0
+                for v in vals[1:]:
+                    assert len(v) == len(vals[0])
+                    worst = max(worst, float(np.abs(v - vals[0]).max() / ref))
+        spreads[label] = worst
+    assert spreads["space group"] <= 1e-9, spreads
+    assert spreads["permutation only"] > 1e-3, spreads
