@@ -148,38 +148,6 @@ __device__ __forceinline__ double tile_sigma(const ModeView &m, int ch, const do
   return m.scalebroad * base_sigma(m, d[0], d[1], d[2]);
 }
 
-// Whether the triple (q, q', q'') of a work item can have a surviving
-// process, conservatively: some (k, l) with c = w''' -+ w'' within
-// vbound''_k + vbound'''_l (each vbound is a rounded-up bound of that mode's
-// share of sigma, sigma_kl <= vbound''_k + vbound'''_l) of the item's range
-// [xlo, xhi] of x1 = w_i +- w'_j.  2^-40 (|x1|max + |w''| + |w'''|) covers
-// the FP64 rounding of the exact test and of this one, so a triple this
-// rejects has no survivor and the visited processes are unchanged.
-template <int NB>
-__device__ __forceinline__ bool triple_may_survive(const ModeView &m, int ch, int jj, int ss, double xlo, double xhi,
-                                                   double xabs) {
-  double od[NB], vk[NB];
-#pragma unroll
-  for (int k = 0; k < NB; k++) {
-    const double e = E(m, jj, k);
-    od[k] = (ch == 0) ? e : -e;
-    vk[k] = (double)m.vbound[(size_t)k * m.nptk + jj];
-  }
-  bool any = false;
-#pragma unroll 1
-  for (int l = 0; l < NB; l++) {
-    const double ot = E(m, ss, l);
-    const double vl = (double)m.vbound[(size_t)l * m.nptk + ss];
-#pragma unroll
-    for (int k = 0; k < NB; k++) {
-      const double c = ot - od[k];
-      const double W = (vk[k] + vl) + 0x1p-40 * (xabs + fabs(od[k]) + fabs(ot));
-      any = any || ((c + W >= xlo) && (c - W <= xhi));
-    }
-  }
-  return any;
-}
-
 // Float estimate of sigma for the cull: b_nu / N_nu and scalebroad as floats,
 // velocity differences formed in FP64 and rounded once.  Its relative error
 // is below 1e-6, far inside the 1e-4 band that is sent to the exact test.

@@ -176,6 +176,59 @@ __device__ __forceinline__ int partner(const ModeView &m, int ch, const int *q, 
   return (t2 * m.n2 + t1) * m.n1 + t0;
 }
 
+// Whether the triple (q, q', q'') of a work item can have a surviving
+// process, conservatively: some (k, l) with c = w''' -+ w'' within
+// vbound''_k + vbound'''_l (each vbound is a rounded-up bound of that mode's
+// share of sigma, sigma_kl <= vbound''_k + vbound'''_l) of the item's range
+// [xlo, xhi] of x1 = w_i +- w'_j.  2^-40 (|x1|max + |w''| + |w'''|) covers
+// the FP64 rounding of the exact test and of this one, so a triple this
+// rejects has no survivor and the visited processes are unchanged.
+template <int NB>
+__device__ __forceinline__ bool triple_may_survive(const ModeView &m, int ch, int jj, int ss, double xlo, double xhi,
+                                                   double xabs) {
+  double od[NB], vk[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const double e = E(m, jj, k);
+    od[k] = (ch == 0) ? e : -e;
+    vk[k] = (double)m.vbound[(size_t)k * m.nptk + jj];
+  }
+  bool any = false;
+#pragma unroll 1
+  for (int l = 0; l < NB; l++) {
+    const double ot = E(m, ss, l);
+    const double vl = (double)m.vbound[(size_t)l * m.nptk + ss];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const double c = ot - od[k];
+      const double W = (vk[k] + vl) + 0x1p-40 * (xabs + fabs(od[k]) + fabs(ot));
+      any = any || ((c + W >= xlo) && (c - W <= xhi));
+    }
+  }
+  return any;
+}
+
+// The same test with rolled loops and an early exit, for kernels that run
+// close to their register budget (the prefilter at three waves per SIMD).
+template <int NB>
+__device__ __forceinline__ bool triple_may_survive_lean(const ModeView &m, int ch, int jj, int ss, double xlo,
+                                                        double xhi, double xabs) {
+#pragma unroll 1
+  for (int k = 0; k < NB; k++) {
+    const double e = E(m, jj, k);
+    const double od = (ch == 0) ? e : -e;
+    const double vk = (double)m.vbound[(size_t)k * m.nptk + jj];
+#pragma unroll 1
+    for (int l = 0; l < NB; l++) {
+      const double ot = E(m, ss, l);
+      const double c = ot - od;
+      const double W = (vk + (double)m.vbound[(size_t)l * m.nptk + ss]) + 0x1p-40 * (xabs + fabs(od) + fabs(ot));
+      if ((c + W >= xlo) && (c - W <= xhi)) return true;
+    }
+  }
+  return false;
+}
+
 __device__ __forceinline__ double bose(double w, double T) { return 1.0 / (exp(kHbar * w / kKb / T) - 1.0); }
 
 // WP4 of one process (processes.f90:3895-3901, 3988-3994, 4095-4101), with

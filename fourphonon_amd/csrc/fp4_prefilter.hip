@@ -16,7 +16,9 @@
 //   4. for the rest, the exact FP64 window test |delta| <= sigma with
 //      base_sigma computed in the reference's operation order.
 // Steps 2 and 3 only decide where the answer cannot differ from step 4, so
-// counts and lists are bit-exact.
+// counts and lists are bit-exact.  Before step 1, a lane skips a jj whose
+// (k, l) pairs all put w''' -+ w'' farther than the sigma bounds from the
+// item's range of w +- w'_j (triple_may_survive): such a jj has no survivor.
 //
 // Count pass: per-item survivor count (int32) and phase-space partial sum.
 // Emit pass: per-lane counts, a wavefront exclusive scan in lane (= jj)
@@ -208,6 +210,15 @@ k_prefilter(PrefilterArgs a) {
 #pragma unroll
     for (int b = 0; b < NB; b++) Eii[b] = E(m, ii0, b);
     const bool active = (w != 0.0) && (w <= m.omega_max);
+    // range of x1 = w +- w'_j over j, for the q'' screen (triple_may_survive)
+    double xlo = (ch == 2) ? w - Eii[0] : w + Eii[0], xhi = xlo;
+#pragma unroll
+    for (int b = 1; b < NB; b++) {
+      const double x = (ch == 2) ? w - Eii[b] : w + Eii[b];
+      xlo = fmin(xlo, x);
+      xhi = fmax(xhi, x);
+    }
+    const double xabs = fmax(fabs(xlo), fabs(xhi));
     const int jj_first = (ch == 1) ? 0 : ii0;
 
     int cnt = 0;
@@ -222,6 +233,7 @@ k_prefilter(PrefilterArgs a) {
           for (int t = 0; t < 3; t++) qdp[t] = m.ijk[3 * jj0 + t];
           const int ss0 = partner(m, ch, q, qp, qdp);
           if (ch != 0 && ss0 < jj0) continue;
+          if (!triple_may_survive_lean<NB>(m, ch, jj0, ss0, xlo, xhi, xabs)) continue;
           scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int k, int l, double delta) {
             cnt += 1;
             double sigma = pair_sigma(m, ch, jj0, k, ss0, l);
@@ -257,6 +269,7 @@ k_prefilter(PrefilterArgs a) {
           for (int t = 0; t < 3; t++) qdp[t] = m.ijk[3 * jj0 + t];
           ss0 = partner(m, ch, q, qp, qdp);
           if (ch != 0 && ss0 < jj0) ok = false;
+          if (ok && !triple_may_survive_lean<NB>(m, ch, jj0, ss0, xlo, xhi, xabs)) ok = false;
         }
         if (ok)
           scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int j, int k, int l, double) {
@@ -299,6 +312,7 @@ k_prefilter(PrefilterArgs a) {
           for (int t = 0; t < 3; t++) qdp[t] = m.ijk[3 * jj0 + t];
           ss0 = partner(m, ch, q, qp, qdp);
           if (ch != 0 && ss0 < jj0) ok = false;
+          if (ok && !triple_may_survive_lean<NB>(m, ch, jj0, ss0, xlo, xhi, xabs)) ok = false;
         }
         if (ok) scan_candidates<NB>(m, ch, i0, qi0, w, ii0, jj0, ss0, Eii, [&](int, int, int, double) { c++; });
         const int off = wave_excl_scan(c, lane);
