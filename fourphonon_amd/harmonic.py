@@ -41,10 +41,13 @@ WS_TOL = float(np.float32(1e-6))
 
 # Natural-abundance isotopes (mass in amu, abundance in %) for the elements of
 # the reference's examples; the average mass is the abundance-weighted mean
-# (data_calc_mandg, Src/data.f90:907-929).  Standard values (NIST).
+# (data_calc_mandg, Src/data.f90:907-929).  The values are the reference's
+# own table's (Src/data.f90:35-895): its boron abundances are 19.8 / 80.2 %,
+# and a 19.9 / 80.1 % table moves the mass by 9e-5 and the BAs example's
+# optical frequencies by 5e-5.
 ISOTOPES = {
     "Si": [(27.976928, 92.23), (28.976496, 4.67), (29.973772, 3.1)],
-    "B": [(10.012937, 19.9), (11.009305, 80.1)],
+    "B": [(10.012938, 19.8), (11.009305, 80.2)],
     "As": [(74.921596, 100.0)],
     "Li": [(6.015123, 7.42), (7.016005, 92.58)],
     "Co": [(58.933198, 100.0)],
@@ -286,7 +289,8 @@ def _phonopy_images(lattvec, cart, scell):
     return out
 
 
-def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False, epsilon=None, born=None):
+def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False, epsilon=None, born=None,
+                    images=None):
     """omega (nk,Nb) rad/ps, velocity (nk,Nb,3) km/s and eigenvect (nk,Nb,Nb)
     [mode, 3*(atom-1)+alpha] for the kpoints (nk,3) in nm^-1 from the
     reduced constants of read_phonopy_fc2 (phonon_phonopy,
@@ -294,7 +298,8 @@ def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False
     supercell images of e^{-i q.R}/neq fc, dD/dq alike, and, when
     nonanalytic, the dielectric correction (Born charges born(3,3,natoms),
     dielectric tensor epsilon(3,3)) added to every cell's constants and to
-    dD/dq, with q taken at its shortest image (no correction at Gamma)."""
+    dD/dq, with q taken at its shortest image (no correction at Gamma), or
+    at images (nk,3) when given."""
     lattvec = np.asarray(st.lattvec, np.float64)
     rlatt = st.rlattvec
     nat = st.natoms
@@ -310,17 +315,25 @@ def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False
     eps = np.eye(3) if epsilon is None else np.asarray(epsilon, np.float64)
     bz = np.zeros((3, 3, nat)) if born is None else np.asarray(born, np.float64)
     V = st.volume
-    # the 1st-BZ image of each q point (Src/phonon_routines.f90:143-159)
-    shortest = k.copy()
-    best = np.sqrt(np.sum(k * k, axis=1))
-    for x in range(-2, 3):
-        for y in range(-2, 3):
-            for z in range(-2, 3):
-                r = k + x * rlatt[:, 0] + y * rlatt[:, 1] + z * rlatt[:, 2]
-                d = np.sqrt(np.sum(r * r, axis=1))
-                better = d < best
-                best = np.where(better, d, best)
-                shortest[better] = r[better]
+    # the 1st-BZ image of each q point (Src/phonon_routines.f90:143-159): q
+    # itself, replaced by an image in x, y, z order only when strictly
+    # shorter.  On the zone boundary several images are equally short and
+    # give non-analytic terms of different directions; the strict comparison
+    # then turns on the last bit of the norm (BLAS dnrm2 in the reference),
+    # so callers that must follow one particular run pass its images.
+    if images is not None:
+        shortest = np.array(images, np.float64).reshape(nk, 3)
+    else:
+        shortest = k.copy()
+        best = np.sqrt(np.sum(k * k, axis=1))
+        for x in range(-2, 3):
+            for y in range(-2, 3):
+                for z in range(-2, 3):
+                    r = k + x * rlatt[:, 0] + y * rlatt[:, 1] + z * rlatt[:, 2]
+                    d = np.sqrt(np.sum(r * r, axis=1))
+                    better = d < best
+                    best = np.where(better, d, best)
+                    shortest[better] = r[better]
     # dielectric correction per q: dyn_nac and ddyn_nac (:168-197)
     dyn_nac = np.zeros((nk, nb, nb))
     ddyn_nac = np.zeros((nk, nb, nb, 3))
@@ -390,13 +403,16 @@ def phonons_phonopy(kpoints, st: crystal.Structure, scell, fc, nonanalytic=False
     return omega, vel, eig
 
 
-def modes_from_phonopy(st: crystal.Structure, fc, scell, ngrid, nonanalytic=False, epsilon=None, born=None):
+def modes_from_phonopy(st: crystal.Structure, fc, scell, ngrid, nonanalytic=False, epsilon=None, born=None,
+                       images=None):
     """eigenDM with the phonopy front end on the full grid (as
     modes_from_espresso: q-point symmetrisers on v, Gamma acoustic omega and
-    v set to zero), with the irreducible wedge and symmetry tables."""
+    v set to zero), with the irreducible wedge and symmetry tables.  images
+    (nptk,3), grid-index order: the non-analytic directions, as in
+    phonons_phonopy."""
     ngrid = np.asarray(ngrid, np.int32)
     kp = grid_kpoints(st.rlattvec, ngrid)
-    omega, vel, eig = phonons_phonopy(kp, st, scell, fc, nonanalytic, epsilon, born)
+    omega, vel, eig = phonons_phonopy(kp, st, scell, fc, nonanalytic, epsilon, born, images)
     Lst, Nequi, ALLEq, TypeS, ns, Cr, Qr, symm, ID = crystal.symmetry_tables(st, ngrid)
     vel = np.einsum("abq,qnb->qna", symm, vel)
     raw_gamma = omega[0, :3].copy()

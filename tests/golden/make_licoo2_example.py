@@ -46,9 +46,11 @@ def floats(path):
         return np.array([float(x) for x in f.read().replace("D", "E").split()])
 
 
-def main():
-    out = os.path.join(SRC, "output")
-    c = formats.read_control(os.path.join(SRC, "input", "CONTROL"))
+def convert(src, dest):
+    """Parses one reference example run (inputs and outputs under src) into
+    the fixture file dest."""
+    out = os.path.join(src, "output")
+    c = formats.read_control(os.path.join(src, "input", "CONTROL"))
     lattvec = np.asarray(c.lattvec_nm, np.float64)
     positions = np.asarray(c.positions, np.float64).T                       # (natoms, 3) fractional
     types = np.asarray(c.types, np.int32)
@@ -56,8 +58,8 @@ def main():
     ngrid = np.asarray(c.ngrid, np.int32)
     scell = np.asarray(c.scell, np.int32)
     natoms = len(types)
-    fc2 = harmonic.read_phonopy_fc2(os.path.join(SRC, "input", "FORCE_CONSTANTS_2ND"), natoms, scell, masses, types)
-    fc3 = harmonic.read_force_constants_3rd(os.path.join(SRC, "input", "FORCE_CONSTANTS_3RD"), lattvec)
+    fc2 = harmonic.read_phonopy_fc2(os.path.join(src, "input", "FORCE_CONSTANTS_2ND"), natoms, scell, masses, types)
+    fc3 = harmonic.read_force_constants_3rd(os.path.join(src, "input", "FORCE_CONSTANTS_3RD"), lattvec)
     nptk = int(np.prod(ngrid))
     q = floats(os.path.join(out, "BTE.qpoints")).reshape(-1, 6)
     nlist = len(q)
@@ -85,10 +87,14 @@ def main():
     log = open(os.path.join(out, "log.out")).read()
     n3 = [int(re.search(r"Ntotal_%s\s*=\s*(\d+)" % s, log).group(1)) for s in ("plus", "minus")]
     n4 = [int(x) for x in re.search(r"Ntotal_minusminus=\s*\n\s*(\d+)\s+(\d+)\s+(\d+)", log).groups()]
-    assert N4.shape == (3, nlist * nb) and list(N4.sum(axis=1)) == n4
+    # LiCoO2's log totals are the sums of BTE.Numprocess_4ph; the BAs example
+    # ships a Numprocess_4ph from another run than its log.out (sums 7.9e8 /
+    # 5.9e9 / 3.6e8 against 2.2e10 / 1.4e11 / 2.7e10), so both are kept as
+    # they are and the tests say which one they compare against
+    assert N4.shape == (3, nlist * nb)
     born = np.zeros((3, 3, natoms)) if c.born is None else np.asarray(c.born, np.float64)
     epsilon = np.eye(3) if c.epsilon is None else np.asarray(c.epsilon, np.float64)
-    np.savez_compressed(OUT, lattvec=lattvec, positions=positions, types=types, masses=masses,
+    np.savez_compressed(dest, lattvec=lattvec, positions=positions, types=types, masses=masses,
                         elements=np.array(c.elements), scell=scell, fc2=fc2, nonanalytic=bool(c.nonanalytic),
                         born=born, epsilon=epsilon,
                         fc3_phi=fc3.Phi, fc3_Rj=fc3.R_j, fc3_Rk=fc3.R_k,
@@ -96,7 +102,11 @@ def main():
                         ngrid=ngrid, T=float(c.T), scalebroad=float(c.scalebroad), List=List, Nequi=Nequi,
                         omega_irr=omega_irr, velocity=velocity, P3=P3, P4=P4, N4=N4, WP3=WP3, WP4=WP4, W3=W3,
                         Ntotal3=np.array(n3, np.int64), Ntotal4=np.array(n4, np.int64))
-    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    print("wrote", dest, os.path.getsize(dest), "bytes")
+
+
+def main():
+    convert(SRC, OUT)
 
 
 if __name__ == "__main__":
