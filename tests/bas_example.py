@@ -39,6 +39,6 @@ def modes(g=None, images=True):
 
 def bas_case(g=None):
     g = load() if g is None else g
-    c = licoo2_example.licoo2_case(g)
-    return crystal.Case(structure(g), modes(g), c.ifc4, T=c.T, scalebroad=c.scalebroad, ifc3=c.ifc3,
-                        meta={"workload": "BAs example 16x16x16"})
+    ifc3, zero4 = licoo2_example.ifcs(g)
+    return crystal.Case(structure(g), modes(g), zero4, T=float(g["T"]), scalebroad=float(g["scalebroad"]),
+                        ifc3=ifc3, meta={"workload": "BAs example 16x16x16"})

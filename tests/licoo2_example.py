@@ -34,15 +34,19 @@ def modes(g=None):
                                        epsilon=g["epsilon"], born=g["born"])
 
 
-def licoo2_case(g=None):
-    g = load() if g is None else g
-    st = structure(g)
-    m = modes(g)
+def ifcs(g):
+    """(third-order constants of the fixture, one zero 4ph cluster)."""
     idx = g["fc3_idx"]
     ifc3 = crystal.IFC3(g["fc3_phi"].shape[-1], np.asfortranarray(g["fc3_phi"]), np.asfortranarray(g["fc3_Rj"]),
                         np.asfortranarray(g["fc3_Rk"]), *[np.ascontiguousarray(idx[a], np.int32) for a in range(3)],
                         g["fc3_cells"])
     zero4 = crystal.IFC4(1, np.zeros((3, 3, 3, 3, 1), order="F"), *[np.zeros((3, 1), order="F")] * 3,
                          *[np.ones(1, np.int32)] * 4, np.zeros((1, 3, 3), np.int64))
-    return crystal.Case(st, m, zero4, T=float(g["T"]), scalebroad=float(g["scalebroad"]), ifc3=ifc3,
-                        meta={"workload": "LiCoO2 example 10x10x10"})
+    return ifc3, zero4
+
+
+def licoo2_case(g=None):
+    g = load() if g is None else g
+    ifc3, zero4 = ifcs(g)
+    return crystal.Case(structure(g), modes(g), zero4, T=float(g["T"]), scalebroad=float(g["scalebroad"]),
+                        ifc3=ifc3, meta={"workload": "LiCoO2 example 10x10x10"})
