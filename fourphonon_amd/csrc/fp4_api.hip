@@ -11,10 +11,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -1822,6 +1824,97 @@ int espresso_images(const fp4_harmonic *h, const double *fc_in, HarmonicImages &
     }
   return 0;
 }
+
+// rlattvec = 2 pi inv(lattvec)^T, columns b_i; returns det(lattvec)
+double reciprocal(const double *A, double *rl) {
+  const double det = A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+                     A[6] * (A[1] * A[5] - A[4] * A[2]);
+  if (!(std::fabs(det) > 0.0)) return 0.0;
+  const double cof[9] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8], A[3] * A[7] - A[4] * A[6],
+                         A[2] * A[7] - A[1] * A[8], A[0] * A[8] - A[2] * A[6], A[1] * A[6] - A[0] * A[7],
+                         A[1] * A[5] - A[2] * A[4], A[2] * A[3] - A[0] * A[5], A[0] * A[4] - A[1] * A[3]};
+  // b_i . a_j = 2 pi delta_ij: b_i = 2 pi (a_j x a_k) / det
+  for (int i = 0; i < 3; i++)
+    for (int c = 0; c < 3; c++) rl[c + 3 * i] = 2.0 * M_PI * cof[c + 3 * i] / det;
+  return det;
+}
+
+int check_harmonic(const fp4_harmonic *h) {
+  if (!h || !h->positions || !h->masses || !h->types) return fail(FP4_ERR_ARG, "null argument");
+  const int nat = h->natoms, nb = 3 * nat;
+  if (nat < 1 || nat > 64) return fail(FP4_ERR_ARG, "natoms out of range");
+  if (!nb_supported(nb)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d not built", nb);
+  for (int c = 0; c < 3; c++)
+    if (h->ngrid[c] < 1 || h->scell[c] < 1) return fail(FP4_ERR_ARG, "bad ngrid or scell");
+  for (int i = 0; i < nat; i++)
+    if (h->types[i] < 1 || h->types[i] > h->nelements) return fail(FP4_ERR_ARG, "types out of range");
+  return 0;
+}
+
+// phonon_phonopy's images (Src/phonon_routines.f90:210-240): per atom pair
+// (row atom a1, column atom a2) and supercell cell, the supercell images of
+// r = r_a1 - r_a2 + R_cell nearest to the origin, ties within 1e-5 nm kept in
+// the loop order, each with weight 1/neq and the block
+// Phi_ij = fc(a2, j, cell, a1, i) of read2fc's reduced constants.
+void phonopy_images(const fp4_harmonic *h, const double *fc, HarmonicImages &out, std::vector<double> &w) {
+  const int nat = h->natoms, s1 = h->scell[0], s2 = h->scell[1], s3 = h->scell[2];
+  const double *L = h->lattvec;
+  auto at = [&](int a, int i, int x, int y, int z, int b, int j) -> size_t {
+    return (size_t)a + (size_t)nat * (i + 3 * (x + (size_t)s1 * (y + (size_t)s2 * (z + (size_t)s3 * (b + (size_t)nat * j)))));
+  };
+  std::vector<double> cart((size_t)3 * nat);
+  for (int i = 0; i < nat; i++)
+    for (int c = 0; c < 3; c++) {
+      double x = 0.0;
+      for (int k = 0; k < 3; k++) x = x + L[c + 3 * k] * h->positions[k + 3 * i];
+      cart[3 * i + c] = x;
+    }
+  out.start.assign(1, 0);
+  out.t.clear();
+  out.fc.clear();
+  w.clear();
+  for (int a1 = 0; a1 < nat; a1++)
+    for (int a2 = 0; a2 < nat; a2++) {
+      for (int ix = 0; ix < s1; ix++)
+        for (int iy = 0; iy < s2; iy++)
+          for (int iz = 0; iz < s3; iz++) {
+            double rcell[3], r[3];
+            for (int c = 0; c < 3; c++) {
+              rcell[c] = L[c] * ix + L[c + 3] * iy + L[c + 6] * iz;
+              r[c] = cart[3 * a1 + c] - cart[3 * a2 + c] + rcell[c];
+            }
+            double dmin = std::numeric_limits<double>::max();
+            std::vector<std::array<double, 3>> rr;
+            for (int x = -2; x <= 2; x++)
+              for (int y = -2; y <= 2; y++)
+                for (int z = -2; z <= 2; z++) {
+                  double rl[3], n2 = 0.0;
+                  for (int c = 0; c < 3; c++) {
+                    rl[c] = x * s1 * L[c] + y * s2 * L[c + 3] + z * s3 * L[c + 6];
+                    n2 = n2 + (rl[c] + r[c]) * (rl[c] + r[c]);
+                  }
+                  const double rn = std::sqrt(n2);
+                  const std::array<double, 3> v{rl[0] + rcell[0], rl[1] + rcell[1], rl[2] + rcell[2]};
+                  if (std::fabs(rn - dmin) > 1e-5) {
+                    if (rn < dmin) {
+                      dmin = rn;
+                      rr.assign(1, v);
+                    }
+                  } else {
+                    rr.push_back(v);
+                  }
+                }
+            const double wt = 1.0 / (double)rr.size();
+            for (const auto &v : rr) {
+              for (int c = 0; c < 3; c++) out.t.push_back(v[c]);
+              for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) out.fc.push_back(fc[at(a2, j, ix, iy, iz, a1, i)] * wt);
+              w.push_back(wt);
+            }
+          }
+      out.start.push_back((int)(out.t.size() / 3));
+    }
+}
 }  // namespace
 
 int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double *symmetrizers, double *omega,
@@ -1840,19 +1933,7 @@ int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double 
   std::vector<double> mm;
   espresso_images(h, fc, im, mm);
   double rl[9];
-  {
-    // rlattvec = 2 pi inv(lattvec)^T, columns b_i
-    const double *A = h->lattvec;
-    const double det = A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
-                       A[6] * (A[1] * A[5] - A[4] * A[2]);
-    if (!(std::fabs(det) > 0.0)) return fail(FP4_ERR_ARG, "singular lattice");
-    const double cof[9] = {A[4] * A[8] - A[5] * A[7], A[5] * A[6] - A[3] * A[8], A[3] * A[7] - A[4] * A[6],
-                           A[2] * A[7] - A[1] * A[8], A[0] * A[8] - A[2] * A[6], A[1] * A[6] - A[0] * A[7],
-                           A[1] * A[5] - A[2] * A[4], A[2] * A[3] - A[0] * A[5], A[0] * A[4] - A[1] * A[3]};
-    // b_i . a_j = 2 pi delta_ij: b_i = 2 pi (a_j x a_k) / det
-    for (int i = 0; i < 3; i++)
-      for (int c = 0; c < 3; c++) rl[c + 3 * i] = 2.0 * M_PI * cof[c + 3 * i] / det;
-  }
+  if (!(std::fabs(reciprocal(h->lattvec, rl)) > 0.0)) return fail(FP4_ERR_ARG, "singular lattice");
   HIPCHECK(hipSetDevice(h->device));
   DevBuf d_start, d_t, d_fc, d_mm, d_symm, d_om, d_eig, d_vel;
   int rc;
@@ -1887,6 +1968,116 @@ int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double 
   a.omega = d_om.as<double>();
   a.eig = d_eig.as<double2>();
   a.vel = d_vel.as<double>();
+  if (launch_eigen_dm(a, nullptr)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", nb);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(omega, d_om.p, sizeof(double) * (size_t)nptk * nb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(eigenvect, d_eig.p, sizeof(double2) * (size_t)nptk * nb * nb, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(velocity, d_vel.p, sizeof(double) * (size_t)nptk * nb * 3, hipMemcpyDeviceToHost));
+  return FP4_OK;
+}
+
+int fp4_eigen_dm_phonopy(const fp4_harmonic *h, const double *fc, const double *born, const double *epsilon,
+                         const double *nac_images, const double *symmetrizers, double *omega, double *eigenvect,
+                         double *velocity) {
+  int rc;
+  if ((rc = check_harmonic(h))) return rc;
+  if (!fc || !omega || !eigenvect || !velocity) return fail(FP4_ERR_ARG, "null argument");
+  if (!born != !epsilon) return fail(FP4_ERR_ARG, "born and epsilon go together");
+  const int nat = h->natoms, nb = 3 * nat;
+  const int nptk = h->ngrid[0] * h->ngrid[1] * h->ngrid[2];
+  double rl[9];
+  const double det = reciprocal(h->lattvec, rl);
+  if (!(std::fabs(det) > 0.0)) return fail(FP4_ERR_ARG, "singular lattice");
+  HarmonicImages im;
+  std::vector<double> wimg;
+  phonopy_images(h, fc, im, wimg);
+  std::vector<double> ones((size_t)nat * nat, 1.0), nmm((size_t)nat * nat);
+  for (int i = 0; i < nat; i++)
+    for (int j = 0; j < nat; j++) {
+      const double mi = h->masses[h->types[i] - 1], mj = h->masses[h->types[j] - 1];
+      nmm[(size_t)i * nat + j] = (i == j) ? mi : std::sqrt(mi * mj);
+    }
+  // the 1st-BZ image of each q (Src/phonon_routines.f90:143-159): q itself
+  // unless an image in x, y, z order is strictly shorter
+  std::vector<double> qimg;
+  if (born) {
+    if (nac_images) {
+      qimg.assign(nac_images, nac_images + 3 * (size_t)nptk);
+    } else {
+      qimg.resize(3 * (size_t)nptk);
+      for (int q0 = 0; q0 < nptk; q0++) {
+        const int i1 = q0 % h->ngrid[0], i2 = (q0 / h->ngrid[0]) % h->ngrid[1], i3 = q0 / (h->ngrid[0] * h->ngrid[1]);
+        double k[3];
+        for (int c = 0; c < 3; c++)
+          k[c] = ((rl[c] * (double)i1) / (double)h->ngrid[0] + (rl[3 + c] * (double)i2) / (double)h->ngrid[1]) +
+                 (rl[6 + c] * (double)i3) / (double)h->ngrid[2];
+        double best = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]), sh[3] = {k[0], k[1], k[2]};
+        for (int x = -2; x <= 2; x++)
+          for (int y = -2; y <= 2; y++)
+            for (int z = -2; z <= 2; z++) {
+              double r[3];
+              for (int c = 0; c < 3; c++) r[c] = k[c] + x * rl[c] + y * rl[3 + c] + z * rl[6 + c];
+              const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+              if (d < best) {
+                best = d;
+                for (int c = 0; c < 3; c++) sh[c] = r[c];
+              }
+            }
+        for (int c = 0; c < 3; c++) qimg[3 * (size_t)q0 + c] = sh[c];
+      }
+    }
+  }
+  HIPCHECK(hipSetDevice(h->device));
+  DevBuf d_start, d_t, d_fc, d_w, d_mm, d_nmm, d_q, d_born, d_symm, d_om, d_eig, d_vel;
+  if ((rc = d_start.ensure(sizeof(int) * im.start.size())) || (rc = d_t.ensure(sizeof(double) * im.t.size())) ||
+      (rc = d_fc.ensure(sizeof(double) * im.fc.size())) || (rc = d_w.ensure(sizeof(double) * wimg.size())) ||
+      (rc = d_mm.ensure(sizeof(double) * ones.size())) || (rc = d_nmm.ensure(sizeof(double) * nmm.size())) ||
+      (rc = d_om.ensure(sizeof(double) * (size_t)nptk * nb)) ||
+      (rc = d_eig.ensure(sizeof(double2) * (size_t)nptk * nb * nb)) ||
+      (rc = d_vel.ensure(sizeof(double) * (size_t)nptk * nb * 3)) ||
+      (born && ((rc = d_q.ensure(sizeof(double) * qimg.size())) || (rc = d_born.ensure(sizeof(double) * 9 * nat)))) ||
+      (symmetrizers && (rc = d_symm.ensure(sizeof(double) * 9 * (size_t)nptk))))
+    return rc;
+  HIPCHECK(hipMemcpy(d_start.p, im.start.data(), sizeof(int) * im.start.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_t.p, im.t.data(), sizeof(double) * im.t.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_fc.p, im.fc.data(), sizeof(double) * im.fc.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_w.p, wimg.data(), sizeof(double) * wimg.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_mm.p, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_nmm.p, nmm.data(), sizeof(double) * nmm.size(), hipMemcpyHostToDevice));
+  if (born) {
+    HIPCHECK(hipMemcpy(d_q.p, qimg.data(), sizeof(double) * qimg.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_born.p, born, sizeof(double) * 9 * nat, hipMemcpyHostToDevice));
+  }
+  if (symmetrizers)
+    HIPCHECK(hipMemcpy(d_symm.p, symmetrizers, sizeof(double) * 9 * (size_t)nptk, hipMemcpyHostToDevice));
+  EigenDMArgs a{};
+  a.nptk = nptk;
+  a.nb = nb;
+  a.nat = nat;
+  a.n1 = h->ngrid[0];
+  a.n2 = h->ngrid[1];
+  a.n3 = h->ngrid[2];
+  for (int t = 0; t < 9; t++) a.b[t] = rl[t];
+  // constants already in (rad/ps)^2 per amu-reduced unit, lengths in nm
+  a.bohr2nm = 1.0;
+  a.tothz = 1.0;
+  a.img_start = d_start.as<int>();
+  a.img_t = d_t.as<double>();
+  a.img_fc = d_fc.as<double>();
+  a.mm = d_mm.as<double>();
+  a.symm = symmetrizers ? d_symm.as<double>() : nullptr;
+  a.omega = d_om.as<double>();
+  a.eig = d_eig.as<double2>();
+  a.vel = d_vel.as<double>();
+  if (born) {
+    a.img_w = d_w.as<double>();
+    a.nac_q = d_q.as<double>();
+    a.nac_mm = d_nmm.as<double>();
+    a.born = d_born.as<double>();
+    for (int t = 0; t < 9; t++) a.eps[t] = epsilon[t];
+    // prefactor 1745.91429109 THz^2 amu nm^3 is a default-real literal (Src/phonon_routines.f90:91)
+    a.nac_scale = (double)1745.91429109f / std::fabs(det) / ((double)h->scell[0] * h->scell[1] * h->scell[2]);
+  }
   if (launch_eigen_dm(a, nullptr)) return fail(FP4_ERR_UNSUPPORTED, "Nbands=%d", nb);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpy(omega, d_om.p, sizeof(double) * (size_t)nptk * nb, hipMemcpyDeviceToHost));

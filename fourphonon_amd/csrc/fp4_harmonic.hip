@@ -1,11 +1,16 @@
-// Harmonic front end on the device: eigenDM for Quantum Espresso 2nd-order
-// constants (Src/phonon_routines.f90:12-85, 311-677).
+// Harmonic front end on the device: eigenDM for Quantum Espresso and phonopy
+// 2nd-order constants (Src/phonon_routines.f90:12-307, 311-677).
 //
 // One 64-wide wavefront per grid point q:
-//   1. D(q) and dD/dq from the Wigner-Seitz-weighted supercell images
-//      (the image list with its weights is built on the host):
+//   1. D(q) and dD/dq from the weighted supercell images (the image list
+//      with its weights is built on the host: Wigner-Seitz weights for
+//      espresso, 1/neq over the nearest images for phonopy):
 //      D_ij += Phi_ij(image) w exp(-i k.t), dD_ij += -i t Phi_ij w exp(-i k.t),
-//      both divided by sqrt(m_i m_j), held in LDS;
+//      both divided by sqrt(m_i m_j), held in LDS; with phonopy's dielectric
+//      correction, its dyn_nac / ncell enters every image's constants alike,
+//      so D_ij gets S dyn_nac_ij / ncell and dD_ij gets
+//      (-i Zr dyn_nac_ij + S ddyn_nac_ij) / ncell, with S = sum w exp(-i k.t)
+//      and Zr = sum w t exp(-i k.t) over the pair's images;
 //   2. the Hermitian eigenproblem by cyclic complex Jacobi rotations in FP64
 //      (one rotation at a time, the row and column updates spread over the
 //      lanes), eigenvalues sorted ascending;
@@ -53,6 +58,7 @@ __global__ void __launch_bounds__(64) k_eigen_dm(EigenDMArgs a) {
     const int iat = i / 3, ia = i % 3, jat = j / 3, ja = j % 3;
     const int pair = iat * a.nat + jat;
     double2 d = make_double2(0.0, 0.0), g0 = d, g1 = d, g2 = d;
+    double2 S = d, Z0 = d, Z1 = d, Z2 = d;
     for (int m = a.img_start[pair]; m < a.img_start[pair + 1]; m++) {
       const double *t = a.img_t + 3 * (size_t)m;
       const double kt = k[0] * t[0] + k[1] * t[1] + k[2] * t[2];
@@ -65,12 +71,51 @@ __global__ void __launch_bounds__(64) k_eigen_dm(EigenDMArgs a) {
       g0.x -= t[0] * f * sn; g0.y -= t[0] * f * cs;
       g1.x -= t[1] * f * sn; g1.y -= t[1] * f * cs;
       g2.x -= t[2] * f * sn; g2.y -= t[2] * f * cs;
+      if (a.nac_q) {
+        // w exp(-i kt) and w t exp(-i kt)
+        const double w = a.img_w[m];
+        S.x += w * cs; S.y -= w * sn;
+        Z0.x += w * t[0] * cs; Z0.y -= w * t[0] * sn;
+        Z1.x += w * t[1] * cs; Z1.y -= w * t[1] * sn;
+        Z2.x += w * t[2] * cs; Z2.y -= w * t[2] * sn;
+      }
     }
     const double mm = a.mm[pair];
-    A[e] = make_double2(d.x / mm, d.y / mm);
-    dA[e] = make_double2(g0.x / mm, g0.y / mm);
-    dA[NN + e] = make_double2(g1.x / mm, g1.y / mm);
-    dA[2 * NN + e] = make_double2(g2.x / mm, g2.y / mm);
+    d = make_double2(d.x / mm, d.y / mm);
+    g0 = make_double2(g0.x / mm, g0.y / mm);
+    g1 = make_double2(g1.x / mm, g1.y / mm);
+    g2 = make_double2(g2.x / mm, g2.y / mm);
+    if (a.nac_q) {
+      const double *qs = a.nac_q + 3 * (size_t)q0;
+      if (qs[0] != 0.0 || qs[1] != 0.0 || qs[2] != 0.0) {
+        // dyn_nac and ddyn_nac of phonon_phonopy (Src/phonon_routines.f90:168-197)
+        double eq[3], t1 = 0.0, t2 = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+          eq[x] = a.eps[x] * qs[0] + a.eps[x + 3] * qs[1] + a.eps[x + 6] * qs[2];
+          t1 += qs[x] * a.born[x + 3 * (ia + 3 * iat)];
+          t2 += qs[x] * a.born[x + 3 * (ja + 3 * jat)];
+        }
+        const double tmp3 = qs[0] * eq[0] + qs[1] * eq[1] + qs[2] * eq[2];
+        const double nm = a.nac_mm[pair];
+        const double dn = t1 * t2 / nm * a.nac_scale / tmp3;
+        double dd[3];
+#pragma unroll
+        for (int x = 0; x < 3; x++)
+          dd[x] = (t1 * a.born[x + 3 * (ja + 3 * jat)] + t2 * a.born[x + 3 * (ia + 3 * iat)] -
+                   2.0 * t1 * t2 * eq[x] / tmp3) / nm * a.nac_scale / tmp3;
+        d.x += S.x * dn;
+        d.y += S.y * dn;
+        // -i Zr dn + S dd
+        g0.x += Z0.y * dn + S.x * dd[0]; g0.y += -Z0.x * dn + S.y * dd[0];
+        g1.x += Z1.y * dn + S.x * dd[1]; g1.y += -Z1.x * dn + S.y * dd[1];
+        g2.x += Z2.y * dn + S.x * dd[2]; g2.y += -Z2.x * dn + S.y * dd[2];
+      }
+    }
+    A[e] = d;
+    dA[e] = g0;
+    dA[NN + e] = g1;
+    dA[2 * NN + e] = g2;
     V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();

@@ -166,6 +166,13 @@ struct EigenDMArgs {
   double *omega;            // omega(nptk,Nb)
   double2 *eig;             // eigenvect(nptk,Nb,Nb)
   double *vel;              // velocity(nptk,Nb,3)
+  // phonopy's dielectric correction (phonon_phonopy); nac_q null: none
+  const double *img_w;      // [nimg] weight of each image (1/neq)
+  const double *nac_q;      // [nptk][3] the q image the correction's direction comes from, nm^-1
+  const double *nac_mm;     // [nat*nat] m_i on the diagonal, sqrt(m_i m_j) off it, amu
+  const double *born;       // born(3,3,nat) column-major
+  double eps[9];            // epsilon(3,3) column-major
+  double nac_scale;         // prefactor / V / (scell(1) scell(2) scell(3))
 };
 int launch_eigen_dm(const EigenDMArgs &a, hipStream_t st);
 

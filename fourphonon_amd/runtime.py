@@ -43,7 +43,7 @@ EXPORTS = [
     "fp4_selftest_mfma", "fp4_ctx_set_symmetry", "fp4_iteration0", "fp4_bte_set_operator", "fp4_iteration_4ph",
     "fp4_bte_iterate", "fp4_bte_set_sweep", "fp4_tconduct", "fp4_bte_solve", "fp4_ctx_set_ifc3", "fp4_np_3ph", "fp4_ind_only_3ph",
     "fp4_rta_3ph", "fp4_rta_3ph_modes", "fp4_rta_3ph_using_ind", "fp4_ind_3ph", "fp4_vp_3ph",
-    "fp4_eigen_dm_espresso", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
+    "fp4_eigen_dm_espresso", "fp4_eigen_dm_phonopy", "fp4_np_4ph_sample", "fp4_rta_4ph_sample", "fp4_rta_4ph_sample_modes",
     "fp4_np_3ph_sample", "fp4_rta_3ph_sample", "fp4_rta_3ph_sample_modes", "fp4_ctx_autotune_dense", "fp4_bte_assemble_operator",
     "fp4_bte_operator_begin", "fp4_bte_operator_add", "fp4_bte_operator_end",
     "fp4_batch_create", "fp4_batch_destroy", "fp4_batch_rta_4ph",
@@ -123,6 +123,42 @@ def eigen_dm_espresso(structure, fc, scell, ngrid, symmetrizers=None, device: in
     sym = None if symmetrizers is None else np.asfortranarray(symmetrizers, np.float64)
     _check(load().fp4_eigen_dm_espresso(C.byref(h), _p(fcf), _p(sym) if sym is not None else None,
                                         _p(om), _p(eig), _p(vel)))
+    return om, eig, vel
+
+
+def eigen_dm_phonopy(structure, fc, scell, ngrid, born=None, epsilon=None, images=None, symmetrizers=None,
+                     device: int = 0):
+    """Device eigenDM for read2fc's reduced phonopy constants
+    fc(nat,3,s1,s2,s3,nat,3) in (rad/ps)^2, with the dielectric correction
+    when born(3,3,nat) and epsilon(3,3) are given (its direction from
+    images (nptk,3) when given) -> (omega (nptk,Nb), eigenvect (nptk,Nb,Nb),
+    velocity (nptk,Nb,3)), F-order."""
+    st = structure
+    nat = st.natoms
+    nb = 3 * nat
+    nptk = int(np.prod(ngrid))
+    if (born is None) != (epsilon is None):
+        raise ValueError("born and epsilon go together")
+    pos = np.asfortranarray(np.asarray(st.positions, np.float64).T)
+    masses = np.ascontiguousarray(st.masses, np.float64)
+    types = np.ascontiguousarray(st.types, np.int32)
+    fcf = np.asfortranarray(fc, np.float64)
+    h = Harmonic()
+    h.ngrid[:] = [int(x) for x in ngrid]
+    h.natoms, h.nelements = nat, len(masses)
+    h.scell[:] = [int(x) for x in scell]
+    h.lattvec[:] = list(np.asarray(st.lattvec, np.float64).ravel(order="F"))
+    h.positions, h.masses, h.types = _p(pos), _p(masses), _p(types)
+    h.device = device
+    om = np.zeros((nptk, nb), order="F")
+    eig = np.zeros((nptk, nb, nb), np.complex128, order="F")
+    vel = np.zeros((nptk, nb, 3), order="F")
+    bz = None if born is None else np.asfortranarray(born, np.float64)
+    ep = None if epsilon is None else np.asfortranarray(epsilon, np.float64)
+    qi = None if images is None else np.ascontiguousarray(images, np.float64).reshape(nptk, 3)
+    sym = None if symmetrizers is None else np.asfortranarray(symmetrizers, np.float64)
+    opt = [None if x is None else _p(x) for x in (bz, ep, qi, sym)]
+    _check(load().fp4_eigen_dm_phonopy(C.byref(h), _p(fcf), *opt, _p(om), _p(eig), _p(vel)))
     return om, eig, vel
 
 

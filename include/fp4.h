@@ -314,6 +314,21 @@ typedef struct fp4_harmonic {
 } fp4_harmonic;
 int fp4_eigen_dm_espresso(const fp4_harmonic *h, const double *fc, const double *symmetrizers, double *omega,
                           double *eigenvect, double *velocity);
+/* eigenDM for phonopy 2nd-order constants (phonon_phonopy,
+   Src/phonon_routines.f90:84-307) on the device, outputs as above.  fc is
+   read2fc's fc(natoms,3,scell(1),scell(2),scell(3),natoms,3) (Src/input.f90:
+   18-79): divided by sqrt(m_i m_j) (m_i on the diagonal) and scaled to
+   (rad/ps)^2.  born(3,3,natoms) and epsilon(3,3), both or neither, switch on
+   the dielectric correction (nonanalytic = .TRUE.): dyn_nac / ncell added to
+   every cell's constants and its derivative to dD/dq, no correction at
+   Gamma.  Its direction comes from nac_images(3,nptk) when given, else from
+   the reference's search (q kept unless an image q + x b1 + y b2 + z b3,
+   x, y, z = -2..2 in that order, is strictly shorter); on the zone boundary
+   several images tie and that comparison turns on the norm's last bit, so a
+   caller that must follow one run passes that run's images. */
+int fp4_eigen_dm_phonopy(const fp4_harmonic *h, const double *fc, const double *born, const double *epsilon,
+                         const double *nac_images, const double *symmetrizers, double *omega, double *eigenvect,
+                         double *velocity);
 
 /* ---- iterative solution (Src/iterations.f90, Src/conductivity.f90) ------
    F_n(Nb,nptk,3) and the scattering operator stay on the device between

@@ -1,0 +1,106 @@
+"""Device eigenDM for phonopy constants (fp4_eigen_dm_phonopy through the C
+ABI) against the host front end (harmonic.phonons_phonopy) and the
+reference's printed spectra: the LiCoO2 example (4 atoms, nonanalytic on
+with zero Born charges) and the BAs example (2 atoms, Born charges and
+dielectric tensor), the latter with the run's zone-boundary images and with
+the library's own image search.  Eigenvectors are compared through the
+projector onto each degenerate set (a basis inside a set is the solver's
+own)."""
+import numpy as np
+import pytest
+
+from fourphonon_amd import harmonic
+from fourphonon_amd import runtime as rt
+from tests import bas_example, licoo2_example
+
+pytestmark = pytest.mark.gpu
+
+
+def _projector_err(e_dev, e_host, w, qs, rtol=1e-6):
+    worst = 0.0
+    for q in qs:
+        used = np.zeros(w.shape[1], bool)
+        for b in range(w.shape[1]):
+            if used[b]:
+                continue
+            grp = np.nonzero(np.abs(w[q] - w[q, b]) <= rtol * max(abs(w[q, b]), 1.0))[0]
+            used[grp] = True
+            Pd = e_dev[q, grp, :].T @ e_dev[q, grp, :].conj()
+            Ph = e_host[q, grp, :].T @ e_host[q, grp, :].conj()
+            worst = max(worst, float(np.abs(Pd - Ph).max()))
+    return worst
+
+
+def _set_sums(v, w, rtol=1e-6):
+    """Velocities summed over each degenerate set of each q (a set's split
+    between its modes follows the solver's basis)."""
+    out = np.array(v, copy=True)
+    for q in range(w.shape[0]):
+        used = np.zeros(w.shape[1], bool)
+        for b in range(w.shape[1]):
+            if used[b]:
+                continue
+            grp = np.nonzero(np.abs(w[q] - w[q, b]) <= rtol * max(abs(w[q, b]), 1.0))[0]
+            used[grp] = True
+            out[q, grp, :] = v[q, grp, :].sum(axis=0)
+    return out
+
+
+def _run(g, m, images=None, nac=True):
+    st = bas_example.structure(g) if "nac_images" in g.files else licoo2_example.structure(g)
+    print(f"\n[gpu] eigen_dm_phonopy {st.name} {tuple(int(x) for x in g['ngrid'])}", flush=True)
+    return rt.eigen_dm_phonopy(st, g["fc2"], tuple(int(x) for x in g["scell"]), tuple(int(x) for x in g["ngrid"]),
+                               born=g["born"] if nac else None, epsilon=g["epsilon"] if nac else None,
+                               images=images, symmetrizers=m.symmetrizers)
+
+
+def _check_against(g, m, om, eig, vel):
+    wmax, vmax = np.abs(m.energy).max(), np.abs(m.velocity).max()
+    w = np.asarray(m.energy)
+    ew = np.abs(om - m.energy).max() / wmax
+    ev = np.abs(_set_sums(vel, w) - _set_sums(np.asarray(m.velocity), w)).max() / vmax
+    ep = _projector_err(eig, np.asarray(m.eigenvect), np.asarray(m.energy), range(1, m.nptk, 37))
+    wref = g["omega_irr"]
+    er = np.abs(om[m.List - 1] - wref).max() / np.abs(wref).max()
+    print(f"omega vs host {ew:.2e}, v vs host {ev:.2e}, projectors {ep:.2e}, omega vs the run {er:.2e}", flush=True)
+    return ew, ev, ep, er
+
+
+def test_licoo2_spectrum_on_the_device():
+    g = licoo2_example.load()
+    m = licoo2_example.modes(g)
+    om, eig, vel = _run(g, m)
+    ew, ev, ep, er = _check_against(g, m, om, eig, vel)
+    assert ew <= 1e-12 and ev <= 1e-10 and ep <= 1e-8 and er <= 1e-9
+    w = np.asarray(m.energy)
+    assert np.abs(_set_sums(vel, w) - _set_sums(g["velocity"], w)).max() <= 1e-9 * np.abs(g["velocity"]).max()
+
+
+def test_bas_spectrum_with_the_dielectric_correction_on_the_device():
+    g = bas_example.load()
+    m = bas_example.modes(g)
+    om, eig, vel = _run(g, m, images=g["nac_images"])
+    ew, ev, ep, er = _check_against(g, m, om, eig, vel)
+    assert ew <= 1e-12 and ev <= 1e-10 and ep <= 1e-8 and er <= 1e-9
+    w = np.asarray(m.energy)
+    assert np.abs(_set_sums(vel, w) - _set_sums(g["velocity"], w)).max() <= 1e-9 * np.abs(g["velocity"]).max()
+    # the correction is on: without it the LO branch moves by 0.4%
+    om0, _, _ = _run(g, m, nac=False)
+    assert np.abs(om0 - om).max() / np.abs(om).max() > 1e-3
+
+
+def test_bas_own_image_search_differs_only_at_ties():
+    g = bas_example.load()
+    m0 = bas_example.modes(g, images=False)
+    om, eig, vel = _run(g, m0)
+    d = np.abs(om - m0.energy).max(axis=1)
+    diff = np.flatnonzero(d > 1e-9 * np.abs(om).max())
+    print(f"device search vs host search: {len(diff)} q points differ", flush=True)
+    st = bas_example.structure(g)
+    kp = harmonic.grid_kpoints(st.rlattvec, g["ngrid"])
+    b = st.rlattvec
+    shifts = np.array([(x, y, z) for x in range(-2, 3) for y in range(-2, 3) for z in range(-2, 3)], float)
+    for iq in diff:
+        dn = np.linalg.norm(kp[iq][None, :] + shifts @ b.T, axis=1)
+        assert np.sum(dn <= dn.min() * (1 + 1e-12)) >= 2, iq
+    assert len(diff) <= 265
