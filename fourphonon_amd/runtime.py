@@ -133,12 +133,12 @@ def eigen_dm_phonopy(structure, fc, scell, ngrid, born=None, epsilon=None, image
     when born(3,3,nat) and epsilon(3,3) are given (its direction from
     images (nptk,3) when given) -> (omega (nptk,Nb), eigenvect (nptk,Nb,Nb),
     velocity (nptk,Nb,3)), F-order."""
+    if (born is None) != (epsilon is None):
+        raise ValueError("born and epsilon go together")
     st = structure
     nat = st.natoms
     nb = 3 * nat
     nptk = int(np.prod(ngrid))
-    if (born is None) != (epsilon is None):
-        raise ValueError("born and epsilon go together")
     pos = np.asfortranarray(np.asarray(st.positions, np.float64).T)
     masses = np.ascontiguousarray(st.masses, np.float64)
     types = np.ascontiguousarray(st.types, np.int32)
