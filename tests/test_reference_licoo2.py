@@ -81,6 +81,12 @@ def test_three_phonon_rates(g, case):
         assert _rel(r[key][modes - 1], g["WP3"][ch][modes - 1]) <= 2e-4, key
 
 
+def test_the_example_files_are_one_run(g):
+    """The per-mode 4ph counts sum to the log's totals (the BAs example's do
+    not: its folder mixes two runs)."""
+    assert [int(g["N4"][ch].sum()) for ch in range(3)] == [int(x) for x in g["Ntotal4"]]
+
+
 def test_four_phonon_counts_of_sample_modes(g, case):
     """BTE.Numprocess_4ph for a few modes (the GPU tests cover all 1392)."""
     o = Oracle(case)
