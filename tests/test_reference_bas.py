@@ -81,6 +81,18 @@ def test_shortest_image_search_differs_only_at_zone_boundary_ties(g, case):
         assert abs(np.linalg.norm(g["nac_images"][iq]) - d.min()) <= 1e-12 * d.min(), iq
 
 
+def test_the_recorded_images_are_shortest_images_of_their_q(g, case):
+    st = case.structure
+    kp = harmonic.grid_kpoints(st.rlattvec, g["ngrid"])
+    img = g["nac_images"]
+    n = np.linalg.solve(st.rlattvec, (img - kp).T).T            # reciprocal-lattice coefficients
+    assert np.abs(n - np.round(n)).max() <= 1e-9
+    b = st.rlattvec
+    shifts = np.array([(x, y, z) for x in range(-2, 3) for y in range(-2, 3) for z in range(-2, 3)], float)
+    dmin = np.min(np.linalg.norm(kp[:, None, :] + (shifts @ b.T)[None, :, :], axis=2), axis=1)
+    assert np.all(np.abs(np.linalg.norm(img, axis=1) - dmin) <= 1e-12 * np.maximum(dmin, 1.0))
+
+
 def test_three_phonon_phase_space_at_scalebroad_0_1(g, case):
     N, P = Oracle(case).np3_driver(nthreads=8)
     for ch in range(2):
